@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""RK4 cell-steps/s of the single-instance drop-in workload on one MI355X.
+
+Workload: level-8 icosahedral SCVT mesh (655,362 cells, 1,966,080 edges,
+1,310,720 vertices), FP64 shallow-water RK4 with dt = 60 s, balanced
+zonal flow plus a seeded 10 m Gaussian bump, 4 passive tracers (seeded
+sums of 8 Gaussian blobs) advected with FCT, del2 tracer diffusion 1e3 m^2/s
+and del2 viscosity 1e5 m^2/s.  One step = one full RK4 step (4 stages) over
+the whole mesh with the state resident in HBM.
+
+Prints one JSON line.  `roofline` is for the kernel with the largest share
+of the step (HIP events around each launch, compulsory bytes per launch);
+`cpu_baseline` times the oracle (the C++ restatement of the reference's
+task loops, one core) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mpas-o-flecsi-2.0_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "RK4 cell-steps/sec + %HBM peak; adjoint/forward cost ratio at ≤1e-10 rel err"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def model_bytes_per_step(ctx, mesh, n_tr):
+    """Counted compulsory bytes of one RK4 step: static mesh arrays once per
+    stage plus 18 touches of the prognostic state (u, h, tracers)."""
+    C, E, V = mesh.nCells, mesh.nEdges, mesh.nVertices
+    s_eoe = int(mesh["nEdgesOnEdge"].sum())
+    s_eoc = int(mesh["nEdgesOnCell"].sum())
+    s_adv = int(ctx.setup_field("nAdvCellsForEdge").sum())
+    stage = (36 * E + 12 * s_eoe + 20 * C + 4 * s_eoc + 64 * V + 8 * E
+             + 20 * s_adv + 4 * E + 4 * s_eoc)
+    state = 18 * 8 * (C * (1 + n_tr) + E)
+    return 4 * stage + state
+
+
+def cpu_baseline(mesh, u, h, tr, h_s, dt, steps, n_tr):
+    from oracle.oracle import OracleMesh, OracleSW
+    t0 = time.perf_counter()
+    om = OracleMesh(mesh, nL=1)
+    o = OracleSW(om, nL=1, nT=n_tr, tracer_adv=2, visc2=1e5, kappa2=1e3)
+    o.set("h", h)
+    o.set("u", u)
+    o.set("tracers", tr)
+    o.set("h_s", h_s)
+    o.init_diagnostics()
+    setup_s = time.perf_counter() - t0
+    sec = o.run(steps, dt)
+    return {"value": mesh.nCells * steps / sec, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} RK4 steps of the same L8 workload on the oracle (C++ restatement "
+                      f"of the reference task loops, g++ -O2, 1 thread); step loop only, "
+                      f"{sec:.2f} s timed, {setup_s:.1f} s setup untimed"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--tracers", type=int, default=4)
+    ap.add_argument("--dt", type=float, default=60.0)
+    ap.add_argument("--prof-steps", type=int, default=50,
+                    help="steps of the per-kernel event-timed pass")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU step "
+              "(multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
+        sys.exit(2)
+
+    from mpas_hip import TRACER_FCT, Context, Mesh, sw_workload_state
+
+    t0 = time.perf_counter()
+    mesh = Mesh(args.level)
+    u, h, tr = sw_workload_state(mesh, args.tracers, seed=1)
+    h_s = np.full(mesh.nCells, 4500.0)
+    log(f"mesh L{args.level}: {mesh.nCells} cells, {mesh.nEdges} edges "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    ctx = Context(mesh, nTracers=args.tracers, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    ctx.sw_upload(h, u, tr, h_s)
+    log(f"context + upload {time.perf_counter() - t0:.1f} s")
+
+    ctx.rk4_step(args.dt, args.warmup)
+    ctx.sync()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    ctx.rk4_step(args.dt, args.steps)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = mesh.nCells * args.steps / elapsed
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step")
+
+    # per-kernel HIP-event pass on the context's stream
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    ctx.rk4_step(args.dt, args.prof_steps)
+    ctx.sync()
+    prof_ms_step = 1e3 * (time.perf_counter() - t0) / max(1, args.prof_steps)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    tot = sum(r["total_ms"] for r in rep)
+    for r in sorted(rep, key=lambda r: -r["total_ms"]):
+        avg = r["total_ms"] / r["launches"]
+        log(f"  {r['kernel']:16s} {r['launches']:6d} launches  avg {1e3 * avg:8.1f} us  "
+            f"{100 * r['total_ms'] / tot:5.1f}%  {r['bytes_per_launch'] / avg / 1e6:7.0f} GB/s")
+    dom = max(rep, key=lambda r: r["total_ms"])
+    avg_s = dom["total_ms"] / dom["launches"] / 1e3
+    achieved = dom["bytes_per_launch"] / avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom["kernel"], {}).get("hbm_bytes_per_launch")
+
+    h1, _, _ = ctx.sw_download()
+    area = mesh["areaCell"]
+    mass_drift = float(np.sum(h1 * area) / np.sum(h * area) - 1.0)
+    model = model_bytes_per_step(ctx, mesh, args.tracers)
+    ctx.close()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosahedral SCVT level {args.level} ({mesh.nCells} cells, "
+                               f"{mesh.nEdges} edges, {mesh.nVertices} vertices), FP64 SW RK4 "
+                               f"dt={args.dt:g} s, {args.tracers} FCT tracers, kappa2=1e3, "
+                               f"visc2=1e5",
+                   "cells": mesh.nCells, "edges": mesh.nEdges, "vertices": mesh.nVertices,
+                   "tracers": args.tracers, "dt": args.dt, "parallelism": "single GPU"},
+        "hbm_frac_model": {"bytes_per_cell_step": model / mesh.nCells,
+                           "achieved_GBs": model * args.steps / elapsed / 1e9,
+                           "frac": model * args.steps / elapsed / 1e9 / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "bytes_per_launch": dom["bytes_per_launch"],
+                     "avg_launch_us": avg_s * 1e6, "profiled_ms_per_step": prof_ms_step},
+        "mass_drift": mass_drift,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.cpu_steps,
+                                           args.tracers)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

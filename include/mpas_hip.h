@@ -1,0 +1,296 @@
+/*
+ * mpas_hip.h — C ABI of the MI355X (gfx950) shallow-water RK4 step with FCT
+ * tracer advection and del2 mixing on the spherical Voronoi (TRiSK) mesh.
+ *
+ * The entry points stand in for the FleCSI tasks of MPAS-O-FleCSI 2.0
+ * (reference tree ipdemes/mpas-o-flecsi-2.0):
+ *   apps/sw/tasks/solve.hh:12-136      shallow-water tasks
+ *   apps/ocean/tasks/solve.hh:12-111   FCT / del2 tracer tasks
+ *   mpasoflecsi/specialization/mesh.cc:67-116  topology flattening
+ * A reference task `f(mesh::accessor m, field accessors..., scalars)` maps to
+ * `mpas_<group>_f(ctx, device pointers..., scalars)`: the context plays the
+ * mesh accessor (it owns the flattened connectivity and the static mesh
+ * fields on the device), the field accessors become device pointers in the
+ * reference's own layouts:
+ *   vltensor  (per entity, [nLevels])            -> double[n * nLevels]
+ *   vltracer  (per cell,  [nLevels][nTracers])   -> double[nCells * nLevels * nTracers]
+ * All calls are asynchronous on the context's stream and return 0 on
+ * success or a negative mpas_status; mpas_last_error() describes the last
+ * failure of the calling thread.  No C++ exception crosses this boundary
+ * (the reference aborts through flog_fatal / flog_assert instead,
+ * mpasoflecsi/specialization/definition.hh:197-200).  A context is not
+ * thread-safe; distinct contexts may be used from distinct threads.
+ */
+#ifndef MPAS_HIP_H
+#define MPAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpas_ctx mpas_ctx;
+typedef struct mpas_host_mesh mpas_host_mesh;
+
+enum mpas_status {
+  MPAS_OK = 0,
+  MPAS_EINVAL = -1, /* bad argument, unknown name, inconsistent mesh */
+  MPAS_EHIP = -2,   /* HIP runtime error (no device, launch failure, ...) */
+  MPAS_ESHAPE = -3, /* buffer size / shape mismatch */
+  MPAS_ESTATE = -4, /* call out of order (e.g. step before state upload) */
+  MPAS_ENOMEM = -5  /* device allocation failed */
+};
+
+/* Tracer transport inside the shallow-water RK4 stage. */
+enum mpas_tracer_scheme {
+  MPAS_TRACER_NONE = 0,    /* no tracers (nTracers must be 0)            */
+  MPAS_TRACER_CENTRED = 1, /* compute_scalar_tend, apps/sw/tasks/solve.cc:237-270 */
+  MPAS_TRACER_FCT = 2      /* monotone FCT, apps/ocean/tasks/solve.cc:40-326 */
+};
+
+/*
+ * Mesh as read from an MPAS mesh file.  Connectivity keeps the file's
+ * conventions (1-based ids, 0 = padding, row-major with the given strides);
+ * the library drops the zeros and rebases the ids exactly as
+ * read_connectivity_2D does (mpasoflecsi/specialization/definition.hh:670-718)
+ * and derives the transposed lists in ascending source id as transpose_cnx
+ * does (mpasoflecsi/specialization/mesh.cc:108-116).  Everything is copied
+ * during mpas_ctx_create; the caller keeps ownership of its arrays.
+ */
+typedef struct mpas_mesh_desc {
+  int32_t nCells, nEdges, nVertices;
+  int32_t maxEdges;     /* row stride of verticesOnCell/edgesOnCell/cellsOnCell */
+  int32_t maxEdges2;    /* row stride of edgesOnEdge/weightsOnEdge (16: the
+                           reference reads 16 columns, io/helpers.hh:172-184) */
+  int32_t vertexDegree; /* row stride of kiteAreasOnVertex (3) */
+  const int32_t *verticesOnCell; /* [nCells * maxEdges]  */
+  const int32_t *edgesOnCell;    /* [nCells * maxEdges]  */
+  const int32_t *cellsOnCell;    /* [nCells * maxEdges]  */
+  const int32_t *verticesOnEdge; /* [nEdges * 2]         */
+  const int32_t *edgesOnEdge;    /* [nEdges * maxEdges2] */
+  const double *areaCell;          /* [nCells]                 */
+  const double *dvEdge, *dcEdge;   /* [nEdges]                 */
+  const double *areaTriangle;      /* [nVertices]              */
+  const double *kiteAreasOnVertex; /* [nVertices*vertexDegree] */
+  const double *fVertex;           /* [nVertices]              */
+  const double *weightsOnEdge;     /* [nEdges * maxEdges2]     */
+  /* Cartesian positions, used only by the FCT stencil setup
+     (apps/ocean/tasks/init_functions.cc:83-328); may be NULL otherwise. */
+  const double *xCell, *yCell, *zCell, *xVertex, *yVertex, *zVertex;
+  const double *meshDensity;    /* [nCells]; NULL = 1 everywhere          */
+  const int32_t *maxLevelCell;  /* [nCells]; NULL = nLevels everywhere     */
+  const int32_t *boundaryCell;  /* [nCells * nLevels]; NULL = 0 everywhere */
+} mpas_mesh_desc;
+
+typedef struct mpas_params {
+  int32_t nLevels;       /* reference nVertLevels (constants.hh:16)          */
+  int32_t nTracers;      /* reference maxTracers  (constants.hh:18)          */
+  int32_t tracer_scheme; /* enum mpas_tracer_scheme                           */
+  int32_t freeze_u;      /* SW test case 1 keeps u fixed (solve.cc:307-313)   */
+  double gravity;        /* mpas_constants::gravity = 9.80616                 */
+  double sphere_radius;  /* a, used by the FCT stencil setup                  */
+  double coef3rd;        /* 3rd/4th-order blend, 0.25 (ocean/solve.cc:11)     */
+  double visc2;          /* del2 viscosity on u (m^2/s); 0 = off              */
+  double kappa2;         /* del2 tracer diffusion coefficient; 0 = off        */
+} mpas_params;
+
+const char *mpas_last_error(void);
+
+/* Flattens the topology (mesh.cc:67-116), builds the FCT stencils
+   (init_functions.cc:18-447) when tracer_scheme == FCT or kappa2 != 0, and
+   uploads every static mesh array to the current HIP device. */
+int mpas_ctx_create(const mpas_mesh_desc *mesh, const mpas_params *params,
+                    mpas_ctx **out);
+int mpas_ctx_destroy(mpas_ctx *ctx);
+/* Stream for subsequent launches (hipStream_t); NULL = the context's own. */
+int mpas_ctx_set_stream(mpas_ctx *ctx, void *hip_stream);
+int mpas_ctx_sync(mpas_ctx *ctx);
+/* Sizes: out[0..4] = nCells, nEdges, nVertices, nLevels, nTracers. */
+int mpas_ctx_dims(const mpas_ctx *ctx, int64_t out[5]);
+
+/* The context's current stream (hipStream_t). */
+void *mpas_ctx_stream(const mpas_ctx *ctx);
+
+/* Per-kernel timing of the fused step (no reference counterpart: the
+   reference has no timers).  While enabled, every kernel launch of
+   mpas_rk4_step is bracketed by HIP events on the context's stream and
+   tagged with its compulsory HBM bytes (every array element the launch
+   reads or writes, counted once).  mpas_prof_report drains the stream and
+   writes a JSON array of {"kernel", "launches", "total_ms",
+   "bytes_per_launch"} into buf (NUL-terminated, at most cap bytes);
+   enabling clears earlier records. */
+int mpas_prof_enable(mpas_ctx *ctx, int on);
+int mpas_prof_report(mpas_ctx *ctx, char *buf, size_t cap);
+
+/* Device buffers for callers without their own allocator (tests, tools). */
+int mpas_buf_alloc(mpas_ctx *ctx, size_t bytes, void **dptr);
+int mpas_buf_free(mpas_ctx *ctx, void *dptr);
+int mpas_buf_upload(mpas_ctx *ctx, void *dptr, const void *host, size_t bytes);
+int mpas_buf_download(mpas_ctx *ctx, void *host, const void *dptr, size_t bytes);
+
+/* Host-side mesh preparation alone (no device needed): the topology
+   flattening of mesh::initialize (mesh.cc:67-116) and, when the parameters
+   ask for FCT or del2 mixing, the setup of init_functions.cc:18-447.
+   mpas_ctx_create runs the same preparation; mpas_ctx_host_mesh returns the
+   context's copy (owned by the context). */
+int mpas_host_mesh_create(const mpas_mesh_desc *mesh, const mpas_params *params,
+                          mpas_host_mesh **out);
+int mpas_host_mesh_destroy(mpas_host_mesh *hm);
+const mpas_host_mesh *mpas_ctx_host_mesh(const mpas_ctx *ctx);
+
+/* Host copies of the flattened connectivity, for checks.  which = "c2v",
+   "c2e", "c2c", "e2v", "e2e", "v2c", "e2c", "v2e".  offsets gets rows+1
+   entries, indices *count entries (0-based); pass NULL arrays to query the
+   sizes in *rows / *count. */
+int mpas_mesh_connectivity(const mpas_host_mesh *hm, const char *which,
+                           int32_t *offsets, int32_t *indices, int64_t *rows,
+                           int64_t *count);
+/* Host copies of the one-time setup products in the reference layouts:
+   int32: maxLevelEdgeTop[E], nAdvCellsForEdge[E], advCellsForEdge[E*16],
+          highOrderAdvMask[E*nLevels];
+   double: edgeSignOnCell[C*8], meshScalingDel2[E], derivTwo[E*16*2],
+          advCoef[E*16], advCoef_3rd[E*16]. */
+int mpas_mesh_setup_field(const mpas_host_mesh *hm, const char *name,
+                          void *host, int64_t n);
+
+/* ---- shallow-water tasks (apps/sw/tasks/solve.hh) ----------------------
+   Device pointers; sizes follow the context (n*nLevels, tracers
+   n*nLevels*nTracers). */
+
+/* solve.hh:12-27 / solve.cc:11-107 */
+int mpas_sw_compute_solve_diagnostics(mpas_ctx *ctx, const double *h,
+                                      const double *u, double *h_edge,
+                                      double *h_vertex, double *circulation,
+                                      double *vorticity, double *ke,
+                                      double *pv_edge, double *pv_vertex);
+/* solve.hh:29-49 / solve.cc:110-141 */
+int mpas_sw_init_provis(mpas_ctx *ctx, const double *u_old, const double *h_old,
+                        const double *tracers_old, const double *h_edge_old,
+                        const double *ke_old, const double *pv_edge_old,
+                        double *u_provis, double *h_provis,
+                        double *tracers_provis, double *h_edge_provis,
+                        double *ke_provis, double *pv_edge_provis);
+/* solve.hh:51-61 / solve.cc:144-166 */
+int mpas_sw_setup_timestep(mpas_ctx *ctx, const double *u_old,
+                           const double *h_old, const double *tracers_old,
+                           double *u_new, double *h_new, double *tracers_new);
+/* solve.hh:63-76 / solve.cc:169-234 (h_s = bottomDepth, [nCells]) */
+int mpas_sw_compute_tend(mpas_ctx *ctx, const double *h_edge, const double *h,
+                         const double *h_s, const double *u, const double *ke,
+                         const double *pv_edge, double *tend_h, double *tend_u);
+/* solve.hh:78-85 / solve.cc:237-270 */
+int mpas_sw_compute_scalar_tend(mpas_ctx *ctx, const double *h_edge,
+                                const double *u, const double *tracers,
+                                double *tracer_tend);
+/* solve.hh:87-97 / solve.cc:273-314 */
+int mpas_sw_compute_substep(mpas_ctx *ctx, double *u_provis,
+                            const double *u_old, const double *u_tend,
+                            double *h_provis, const double *h_old,
+                            const double *h_tend, double *tracers_provis,
+                            const double *tracers_old,
+                            const double *tracers_tend, double rkweight);
+/* solve.hh:100-107 / solve.cc:317-341 */
+int mpas_sw_accumulate_update(mpas_ctx *ctx, double *u_new, const double *u_tend,
+                              double *h_new, const double *h_tend,
+                              double *tracers_new, const double *tracers_tend,
+                              double rkweight);
+/* solve.hh:110-115 / solve.cc:344-367 */
+int mpas_sw_finalize_timestep(mpas_ctx *ctx, const double *u_old,
+                              double *u_new, const double *h_new,
+                              double *tracers_new);
+/* solve.hh:117-129 / solve.cc:370-401 */
+int mpas_sw_timeshift(mpas_ctx *ctx, double *u_old, double *h_old,
+                      double *tracers_old, double *h_edge_old, double *ke_old,
+                      double *pv_edge_old, const double *u_new,
+                      const double *h_new, const double *tracers_new,
+                      const double *h_edge_new, const double *ke_new,
+                      const double *pv_edge_new);
+/* solve.hh:131-136 / solve.cc:404-436: case-1 relative L2 error of h
+   (level 0) against the advected cosine bell; *err is written on the host
+   after the stream has drained.  latCell/lonCell: device [nCells]. */
+int mpas_sw_compute_error(mpas_ctx *ctx, const double *h,
+                          const double *latCell, const double *lonCell,
+                          double sphere_radius, double elapsed, double *err);
+/* No reference counterpart (the reference's compute_tend has no dissipation,
+   solve.cc:228-231): del2 viscosity on u after MPAS-Ocean's del2 momentum
+   mixing, tend_u += meshScalingDel2 * visc2 * (grad(div) - curl(vorticity)),
+   with divergence [nCells*nLevels] written as a by-product. */
+int mpas_sw_del2_viscosity(mpas_ctx *ctx, const double *u,
+                           const double *vorticity, double *divergence,
+                           double *tend_u);
+
+/* ---- FCT / del2 tracer tasks (apps/ocean/tasks/solve.hh) --------------- */
+
+/* solve.hh:12-15 / solve.cc:13-26 */
+int mpas_fct_copy_tracers(mpas_ctx *ctx, const double *tracers,
+                          double *tracersNew, double *tracersPrev);
+/* solve.hh:17-18 / solve.cc:28-38 */
+int mpas_fct_reinitialize_tracer_tendencies(mpas_ctx *ctx, double *tend);
+/* solve.hh:20-23 / solve.cc:40-57 */
+int mpas_fct_calc_normal_thickness_flux(mpas_ctx *ctx,
+                                        const double *layerThicknessEdge,
+                                        const double *normalVelocity,
+                                        double *normalThicknessFlux);
+/* solve.hh:25-36 / solve.cc:60-97 (the vertical velocity w is not read) */
+int mpas_fct_calc_provisional_layer_thicknesses(
+    mpas_ctx *ctx, const double *layerThickness,
+    const double *normalThicknessFlux, double *hProv, double *hProvInv,
+    double *hNewInv, double dt);
+/* solve.hh:38-42 / solve.cc:99-118 (lowOrderFlux: [nEdges*nLevels*nTracers]) */
+int mpas_fct_low_order(mpas_ctx *ctx, const double *normalThicknessFlux,
+                       const double *tracers, double *lowOrderFlux);
+/* solve.hh:44-54 / solve.cc:120-167 */
+int mpas_fct_high_order(mpas_ctx *ctx, const double *normalThicknessFlux,
+                        const double *tracers, double *highOrderFlux);
+/* solve.hh:56-74 / solve.cc:169-285 (tend accumulates, as in the
+   reference, so clear it first with reinitialize_tracer_tendencies) */
+int mpas_fct_correct_and_rescale(mpas_ctx *ctx, const double *hProvInv,
+                                 const double *layerThickness,
+                                 const double *normalThicknessFlux,
+                                 const double *lowOrderFlux,
+                                 const double *tracers, double *highOrderFlux,
+                                 double *flxIn, double *flxOut, double *tend,
+                                 double *tracerMax, double *tracerMin,
+                                 double dt);
+/* solve.hh:76-87 / solve.cc:287-326 */
+int mpas_fct_accumulate_tend(mpas_ctx *ctx, const double *hProvInv,
+                             const double *layerThickness,
+                             const double *highOrderFlux, const double *tracers,
+                             double *tracersProv, double *tend, double dt);
+/* solve.hh:89-99 / solve.cc:328-366 */
+int mpas_hmix_del2_tend(mpas_ctx *ctx, const double *layerThicknessEdge,
+                        const double *tracers, double *tend, double coef);
+/* solve.hh:101-107 / solve.cc:368-399 */
+int mpas_fct_step_tracers_forward(mpas_ctx *ctx, const double *tend,
+                                  const double *tracers, double *tracersNew,
+                                  double *tracersPrev, double rk_weight,
+                                  double rk_subweight);
+/* solve.hh:109-111 / solve.cc:401-414 */
+int mpas_fct_update_tracers(mpas_ctx *ctx, const double *tracersNew,
+                            double *tracers);
+
+/* ---- fused, device-resident shallow-water run (apps/sw/run.cc) ---------
+   The context owns the prognostic state (u, h, tracers) and every
+   diagnostic and tendency field of apps/sw/state.hh:74-87. */
+
+/* Host arrays: h [nCells*nLevels], u [nEdges*nLevels], tracers
+   [nCells*nLevels*nTracers] (NULL when nTracers == 0), h_s [nCells].
+   Runs the initial compute_solve_diagnostics (apps/sw/run.cc:88-92). */
+int mpas_sw_state_upload(mpas_ctx *ctx, const double *h, const double *u,
+                         const double *tracers, const double *h_s);
+int mpas_sw_state_download(mpas_ctx *ctx, double *h, double *u,
+                           double *tracers);
+/* Host copy of a field of the current state: "h", "u", "tracers",
+   "h_edge", "ke", "pv_edge", "vorticity", "pv_vertex", "h_s". */
+int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host,
+                           int64_t n);
+/* nsteps calls of advance_timestep (apps/sw/run.cc:13-70, loop 97-106),
+   with the stage-fused kernels; the result equals the per-task sequence. */
+int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPAS_HIP_H */

@@ -1,0 +1,576 @@
+// Context lifetime, device layouts of the mesh, and the device-resident
+// shallow-water state (the counterpart of mesh::initialize plus the field
+// storage FleCSI owns in the reference, mpasoflecsi/specialization/mesh.cc:67-116,
+// apps/sw/state.hh:74-87).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "ctx.h"
+
+namespace mpasx {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int check_ctx(const mpas_ctx *ctx) {
+  if (!ctx) MPAS_FAIL(MPAS_EINVAL, "context is NULL");
+  MPAS_HIP(hipSetDevice(ctx->device));
+  return 0;
+}
+
+int after_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) MPAS_FAIL(MPAS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot);
+int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps);
+
+namespace {
+
+template <class T>
+int upload(mpas_ctx *ctx, const std::vector<T> &v, const T **out) {
+  *out = nullptr;
+  if (v.empty()) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * v.size()) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(T) * v.size()) +
+                               " bytes for the mesh failed");
+  ctx->mesh_bufs.push_back(p);
+  MPAS_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  *out = static_cast<const T *>(p);
+  return 0;
+}
+
+int alloc_field(mpas_ctx *ctx, size_t n, double **out) {
+  *out = nullptr;
+  if (!n) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * n) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(double) * n) +
+                               " bytes for the state failed");
+  ctx->mesh_bufs.push_back(p);
+  MPAS_HIP(hipMemset(p, 0, sizeof(double) * n));
+  *out = static_cast<double *>(p);
+  return 0;
+}
+
+// Builds every device array of the DMesh view from the flattened host mesh.
+int build_device_mesh(mpas_ctx *ctx) {
+  const HostMesh &m = ctx->host;
+  DMesh &M = ctx->dm;
+  const int C = m.nC, E = m.nE, V = m.nV;
+  M.nC = C;
+  M.nE = E;
+  M.nV = V;
+  M.nL = m.nL;
+  M.nT = ctx->params.nTracers;
+  M.gravity = ctx->params.gravity;
+  M.coef3rd = ctx->params.coef3rd;
+  M.wC = max_row(m.c2e);
+  M.wEoE = max_row(m.e2e);
+  if (M.wC > 8) MPAS_FAIL(MPAS_EINVAL, "cells with more than 8 edges are not supported");
+  if (M.wEoE > 16) MPAS_FAIL(MPAS_EINVAL, "edgesOnEdge rows hold at most 16 entries");
+
+  int rc = 0;
+  std::vector<int2> e2c(E), e2v(E);
+  for (int e = 0; e < E; e++) {
+    e2c[e] = make_int2(m.e2c.at(e, 0), m.e2c.at(e, 1));
+    e2v[e] = make_int2(m.e2v.at(e, 0), m.e2v.at(e, 1));
+  }
+  if ((rc = upload(ctx, e2c, &M.e2c)) || (rc = upload(ctx, e2v, &M.e2v)) ||
+      (rc = upload(ctx, m.dcEdge, &M.dc)) || (rc = upload(ctx, m.dvEdge, &M.dv)))
+    return rc;
+  {
+    std::vector<uint8_t> n(E);
+    std::vector<double> w(size_t(M.wEoE) * E, 0.0);
+    for (int e = 0; e < E; e++) {
+      n[e] = uint8_t(m.e2e.size(e));
+      for (int j = 0; j < m.e2e.size(e); j++) w[size_t(j) * E + e] = m.weightsOnEdge[size_t(e) * 16 + j];
+    }
+    if ((rc = upload(ctx, n, &M.nEoE)) || (rc = upload(ctx, to_ell(m.e2e, M.wEoE), &M.eoe)) ||
+        (rc = upload(ctx, w, &M.woe)))
+      return rc;
+  }
+  // cells
+  {
+    std::vector<uint8_t> n(C), neg(C, 0);
+    std::vector<uint32_t> perm(C, 0);
+    Csr opp;
+    opp.off = m.c2e.off;
+    opp.idx.resize(m.c2e.idx.size());
+    bool aligned = true;
+    for (int c = 0; c < C; c++) {
+      const int ne = m.c2e.size(c);
+      n[c] = uint8_t(ne);
+      int slot[8];
+      for (int j = 0; j < ne; j++) {
+        const int e = m.c2e.at(c, j);
+        const int c1 = m.e2c.at(e, 0), c2 = m.e2c.at(e, 1);
+        if (c == c1) neg[c] |= uint8_t(1u << j);
+        opp.idx[m.c2e.off[c] + j] = (c == c1) ? c2 : c1;
+        if (j >= m.c2c.size(c) || m.c2c.at(c, j) != opp.idx[m.c2e.off[c] + j]) aligned = false;
+        slot[j] = j;
+      }
+      std::sort(slot, slot + ne, [&](int a, int b) { return m.c2e.at(c, a) < m.c2e.at(c, b); });
+      for (int r = 0; r < ne; r++) perm[c] |= uint32_t(slot[r]) << (4 * r);
+      if (m.c2c.size(c) != ne) aligned = false;
+    }
+    ctx->aligned_c2c = aligned;
+    if ((rc = upload(ctx, m.areaCell, &M.area)) || (rc = upload(ctx, n, &M.nEoC)) ||
+        (rc = upload(ctx, neg, &M.sgnNeg)) || (rc = upload(ctx, perm, &M.ascPerm)) ||
+        (rc = upload(ctx, to_ell(m.c2e, M.wC), &M.c2e)))
+      return rc;
+    if (aligned) {
+      if ((rc = upload(ctx, to_ell(m.c2c, M.wC), &M.c2c))) return rc;
+      M.cOpp = M.c2c;  // cellsOnCell[j] lies across edgesOnCell[j]: one array serves both
+    } else {
+      if ((rc = upload(ctx, to_ell(opp, M.wC), &M.cOpp))) return rc;
+      int wcc = std::max(1, max_row(m.c2c));
+      if (wcc > 8) MPAS_FAIL(MPAS_EINVAL, "cellsOnCell rows hold at most 8 entries");
+      if ((rc = upload(ctx, to_ell(m.c2c, std::max(wcc, M.wC)), &M.c2c))) return rc;
+    }
+  }
+  // vertices
+  {
+    std::vector<double> kite(size_t(3) * V);
+    std::vector<int32_t> v2e(size_t(3) * V);
+    for (int v = 0; v < V; v++)
+      for (int i = 0; i < 3; i++) {
+        kite[size_t(i) * V + v] = m.kite[size_t(v) * 3 + i];
+        if (m.v2e.size(v) != 3)
+          MPAS_FAIL(MPAS_EINVAL, "vertex " + std::to_string(v) + " has " +
+                                     std::to_string(m.v2e.size(v)) + " edges (expected 3)");
+        const int e = m.v2e.at(v, i);
+        const bool first = m.e2v.at(e, 0) == v;
+        v2e[size_t(i) * V + v] = int32_t(uint32_t(e) | (first ? 0x80000000u : 0u));
+      }
+    if ((rc = upload(ctx, m.areaTriangle, &M.areaTri)) || (rc = upload(ctx, m.fVertex, &M.fV)) ||
+        (rc = upload(ctx, kite, &M.kite)) || (rc = upload(ctx, to_ell(m.v2c, 3), &M.v2c)) ||
+        (rc = upload(ctx, v2e, &M.v2e)))
+      return rc;
+  }
+  // FCT / del2 setup products
+  M.msd = m.fct ? m.meshScalingDel2[0] : 1.0;
+  if (m.fct) {
+    M.wAdv = 0;
+    for (int e = 0; e < E; e++) M.wAdv = std::max(M.wAdv, int(m.nAdvCellsForEdge[e]));
+    std::vector<uint8_t> na(E);
+    std::vector<int32_t> adv(size_t(M.wAdv) * E);
+    std::vector<double> c1(size_t(M.wAdv) * E, 0.0), c3(size_t(M.wAdv) * E, 0.0);
+    for (int e = 0; e < E; e++) {
+      na[e] = uint8_t(m.nAdvCellsForEdge[e]);
+      for (int i = 0; i < M.wAdv; i++) {
+        const bool live = i < m.nAdvCellsForEdge[e];
+        adv[size_t(i) * E + e] = live ? m.advCellsForEdge[size_t(e) * 16 + i] : m.e2c.at(e, 0);
+        c1[size_t(i) * E + e] = live ? m.advCoef[size_t(e) * 16 + i] : 0.0;
+        c3[size_t(i) * E + e] = live ? m.advCoef_3rd[size_t(e) * 16 + i] : 0.0;
+      }
+    }
+    if ((rc = upload(ctx, na, &M.nAdv)) || (rc = upload(ctx, adv, &M.adv)) ||
+        (rc = upload(ctx, c1, &M.advCoef)) || (rc = upload(ctx, c3, &M.advCoef3)))
+      return rc;
+    bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
+                            [&](int x) { return x == m.nL; });
+    if (!full) {
+      if ((rc = upload(ctx, m.maxLevelCell, &M.mlc)) ||
+          (rc = upload(ctx, m.maxLevelEdgeTop, &M.mlet)))
+        return rc;
+    }
+    bool all_ho = std::all_of(m.highOrderAdvMask.begin(), m.highOrderAdvMask.end(),
+                              [](int x) { return x == 1; });
+    if (!all_ho && (rc = upload(ctx, m.highOrderAdvMask, &M.hoMask))) return rc;
+  }
+  return 0;
+}
+
+int release(mpas_ctx *ctx) {
+  for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
+  ctx->prof.pool.clear();
+  for (void *p : ctx->mesh_bufs) (void)hipFree(p);
+  ctx->mesh_bufs.clear();
+  for (void *p : ctx->user_bufs) (void)hipFree(p);
+  ctx->user_bufs.clear();
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  ctx->own_stream = nullptr;
+  return 0;
+}
+
+int alloc_state(mpas_ctx *ctx) {
+  if (ctx->sw) return 0;
+  auto W = std::make_unique<SwState>();
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
+  const size_t T = C * M.nT;
+  int rc = 0;
+  double **fields_e[] = {&W->u, &W->nu, &W->pu[0], &W->pu[1], &W->h_edge[0], &W->h_edge[1],
+                         &W->pv_edge[0], &W->pv_edge[1]};
+  double **fields_c[] = {&W->h, &W->nh, &W->ph[0], &W->ph[1], &W->ke[0], &W->ke[1],
+                         &W->div[0], &W->div[1]};
+  double **fields_t[] = {&W->tr, &W->ntr, &W->ptr[0], &W->ptr[1], &W->tlo};
+  for (auto f : fields_e)
+    if ((rc = alloc_field(ctx, E, f))) return rc;
+  for (auto f : fields_c)
+    if ((rc = alloc_field(ctx, C, f))) return rc;
+  for (auto f : fields_t)
+    if ((rc = alloc_field(ctx, T, f))) return rc;
+  if ((rc = alloc_field(ctx, V, &W->vort[0])) || (rc = alloc_field(ctx, V, &W->vort[1])) ||
+      (rc = alloc_field(ctx, V, &W->pv_vertex)) || (rc = alloc_field(ctx, size_t(M.nC), &W->h_s)) ||
+      (rc = alloc_field(ctx, E * M.nT, &W->anti)) || (rc = alloc_field(ctx, 2 * T, &W->rfac)))
+    return rc;
+  ctx->sw = std::move(W);
+  return 0;
+}
+
+}  // namespace
+}  // namespace mpasx
+
+using namespace mpasx;
+
+extern "C" {
+
+const char *mpas_last_error(void) { return g_err.c_str(); }
+
+static int prepare_host_mesh(const mpas_mesh_desc *mesh, const mpas_params *params,
+                             HostMesh &hm) {
+  if (!mesh || !params) MPAS_FAIL(MPAS_EINVAL, "mesh or params is NULL");
+  const mpas_params &p = *params;
+  if (p.nLevels < 1 || p.nLevels > 64) MPAS_FAIL(MPAS_EINVAL, "nLevels must be in [1,64]");
+  if (p.nTracers < 0 || p.nTracers > 8) MPAS_FAIL(MPAS_EINVAL, "nTracers must be in [0,8]");
+  if (p.tracer_scheme < MPAS_TRACER_NONE || p.tracer_scheme > MPAS_TRACER_FCT)
+    MPAS_FAIL(MPAS_EINVAL, "unknown tracer_scheme");
+  if (p.tracer_scheme != MPAS_TRACER_NONE && p.nTracers == 0)
+    MPAS_FAIL(MPAS_EINVAL, "a tracer scheme needs nTracers > 0");
+  if (!(p.gravity > 0)) MPAS_FAIL(MPAS_EINVAL, "gravity must be positive");
+  std::string err = flatten_mesh(*mesh, p.nLevels, hm);
+  if (!err.empty()) MPAS_FAIL(MPAS_EINVAL, "mesh rejected: " + err);
+  const bool need_setup = p.tracer_scheme == MPAS_TRACER_FCT || p.kappa2 != 0.0;
+  if (need_setup) {
+    if (!(p.sphere_radius > 0)) MPAS_FAIL(MPAS_EINVAL, "sphere_radius must be positive");
+    err = setup_tracer_operators(hm, p.sphere_radius);
+    if (!err.empty()) MPAS_FAIL(MPAS_EINVAL, "FCT setup failed: " + err);
+  }
+  return 0;
+}
+
+int mpas_host_mesh_create(const mpas_mesh_desc *mesh, const mpas_params *params,
+                          mpas_host_mesh **out) {
+  if (!out) MPAS_FAIL(MPAS_EINVAL, "mpas_host_mesh_create: out is NULL");
+  *out = nullptr;
+  auto hm = std::make_unique<mpas_host_mesh>();
+  if (int rc = prepare_host_mesh(mesh, params, *hm)) return rc;
+  *out = hm.release();
+  return 0;
+}
+
+int mpas_host_mesh_destroy(mpas_host_mesh *hm) {
+  delete hm;
+  return 0;
+}
+
+const mpas_host_mesh *mpas_ctx_host_mesh(const mpas_ctx *ctx) {
+  return ctx ? &ctx->host : nullptr;
+}
+
+int mpas_ctx_create(const mpas_mesh_desc *mesh, const mpas_params *params, mpas_ctx **out) {
+  if (!out) MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_create: out is NULL");
+  *out = nullptr;
+  auto ctx = std::make_unique<mpas_ctx>();
+  if (int rc = prepare_host_mesh(mesh, params, ctx->host)) return rc;
+  ctx->params = *params;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    MPAS_FAIL(MPAS_EHIP, "no HIP device is visible");
+  MPAS_HIP(hipGetDevice(&ctx->device));
+  MPAS_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+  ctx->stream = ctx->own_stream;
+  int rc = build_device_mesh(ctx.get());
+  if (rc) {
+    std::string keep = g_err;
+    release(ctx.get());
+    g_err = keep;
+    return rc;
+  }
+  *out = ctx.release();
+  return 0;
+}
+
+int mpas_ctx_destroy(mpas_ctx *ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->own_stream) (void)hipStreamSynchronize(ctx->own_stream);
+  release(ctx);
+  delete ctx;
+  return 0;
+}
+
+int mpas_ctx_set_stream(mpas_ctx *ctx, void *hip_stream) {
+  if (!ctx) MPAS_FAIL(MPAS_EINVAL, "context is NULL");
+  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+  return 0;
+}
+
+int mpas_ctx_sync(mpas_ctx *ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mpas_ctx_dims(const mpas_ctx *ctx, int64_t out[5]) {
+  if (!ctx || !out) MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_dims: NULL argument");
+  out[0] = ctx->dm.nC;
+  out[1] = ctx->dm.nE;
+  out[2] = ctx->dm.nV;
+  out[3] = ctx->dm.nL;
+  out[4] = ctx->dm.nT;
+  return 0;
+}
+
+int mpas_buf_alloc(mpas_ctx *ctx, size_t bytes, void **dptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!dptr) MPAS_FAIL(MPAS_EINVAL, "mpas_buf_alloc: dptr is NULL");
+  *dptr = nullptr;
+  if (!bytes) MPAS_FAIL(MPAS_EINVAL, "mpas_buf_alloc: zero bytes");
+  if (hipMalloc(dptr, bytes) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  ctx->user_bufs.insert(*dptr);
+  return 0;
+}
+
+int mpas_buf_free(mpas_ctx *ctx, void *dptr) {
+  if (int rc = check_ctx(ctx)) return rc;
+  auto it = ctx->user_bufs.find(dptr);
+  if (it == ctx->user_bufs.end()) MPAS_FAIL(MPAS_EINVAL, "mpas_buf_free: unknown buffer");
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  MPAS_HIP(hipFree(dptr));
+  ctx->user_bufs.erase(it);
+  return 0;
+}
+
+int mpas_buf_upload(mpas_ctx *ctx, void *dptr, const void *host, size_t bytes) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!dptr || !host) MPAS_FAIL(MPAS_EINVAL, "mpas_buf_upload: NULL pointer");
+  MPAS_HIP(hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mpas_buf_download(mpas_ctx *ctx, void *host, const void *dptr, size_t bytes) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!dptr || !host) MPAS_FAIL(MPAS_EINVAL, "mpas_buf_download: NULL pointer");
+  MPAS_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mpas_mesh_connectivity(const mpas_host_mesh *hm, const char *which, int32_t *offsets,
+                           int32_t *indices, int64_t *rows, int64_t *count) {
+  if (!hm || !which || !rows || !count) MPAS_FAIL(MPAS_EINVAL, "mpas_mesh_connectivity: NULL");
+  const HostMesh &m = *hm;
+  const std::string w(which);
+  const Csr *c = w == "c2v" ? &m.c2v : w == "c2e" ? &m.c2e : w == "c2c" ? &m.c2c
+               : w == "e2v" ? &m.e2v : w == "e2e" ? &m.e2e : w == "v2c" ? &m.v2c
+               : w == "e2c" ? &m.e2c : w == "v2e" ? &m.v2e : nullptr;
+  if (!c) MPAS_FAIL(MPAS_EINVAL, "unknown connectivity '" + w + "'");
+  *rows = c->rows();
+  *count = int64_t(c->idx.size());
+  if (offsets) std::memcpy(offsets, c->off.data(), sizeof(int32_t) * c->off.size());
+  if (indices) std::memcpy(indices, c->idx.data(), sizeof(int32_t) * c->idx.size());
+  return 0;
+}
+
+int mpas_mesh_setup_field(const mpas_host_mesh *hm, const char *name, void *host, int64_t n) {
+  if (!hm || !name || !host) MPAS_FAIL(MPAS_EINVAL, "mpas_mesh_setup_field: NULL");
+  const HostMesh &m = *hm;
+  if (!m.fct) MPAS_FAIL(MPAS_ESTATE, "the context was created without the FCT setup");
+  const std::string s(name);
+  const std::vector<int32_t> *vi = s == "maxLevelEdgeTop" ? &m.maxLevelEdgeTop
+                                 : s == "nAdvCellsForEdge" ? &m.nAdvCellsForEdge
+                                 : s == "advCellsForEdge" ? &m.advCellsForEdge
+                                 : s == "highOrderAdvMask" ? &m.highOrderAdvMask : nullptr;
+  const std::vector<double> *vd = s == "edgeSignOnCell" ? &m.edgeSignOnCell
+                                : s == "meshScalingDel2" ? &m.meshScalingDel2
+                                : s == "derivTwo" ? &m.derivTwo
+                                : s == "advCoef" ? &m.advCoef
+                                : s == "advCoef_3rd" ? &m.advCoef_3rd : nullptr;
+  if (vi) {
+    if (int64_t(vi->size()) != n)
+      MPAS_FAIL(MPAS_ESHAPE, s + " has " + std::to_string(vi->size()) + " entries");
+    std::memcpy(host, vi->data(), sizeof(int32_t) * n);
+    return 0;
+  }
+  if (vd) {
+    if (int64_t(vd->size()) != n)
+      MPAS_FAIL(MPAS_ESHAPE, s + " has " + std::to_string(vd->size()) + " entries");
+    std::memcpy(host, vd->data(), sizeof(double) * n);
+    return 0;
+  }
+  MPAS_FAIL(MPAS_EINVAL, "unknown setup field '" + s + "'");
+}
+
+int mpas_sw_state_upload(mpas_ctx *ctx, const double *h, const double *u, const double *tracers,
+                         const double *h_s) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!h || !u || !h_s) MPAS_FAIL(MPAS_EINVAL, "mpas_sw_state_upload: h, u and h_s are required");
+  const DMesh &M = ctx->dm;
+  if (M.nT && !tracers) MPAS_FAIL(MPAS_EINVAL, "mpas_sw_state_upload: tracers is NULL");
+  if (int rc = alloc_state(ctx)) return rc;
+  SwState &W = *ctx->sw;
+  const size_t cb = sizeof(double) * M.nC * M.nL, eb = sizeof(double) * M.nE * M.nL;
+  MPAS_HIP(hipMemcpyAsync(W.h, h, cb, hipMemcpyHostToDevice, ctx->stream));
+  MPAS_HIP(hipMemcpyAsync(W.u, u, eb, hipMemcpyHostToDevice, ctx->stream));
+  MPAS_HIP(hipMemcpyAsync(W.h_s, h_s, sizeof(double) * M.nC, hipMemcpyHostToDevice, ctx->stream));
+  if (M.nT)
+    MPAS_HIP(hipMemcpyAsync(W.tr, tracers, cb * M.nT, hipMemcpyHostToDevice, ctx->stream));
+  W.cur = 0;
+  W.elapsed = 0;
+  // apps/sw/run.cc:88-92: diagnostics of the initial state
+  if (int rc = sw_diagnostics(ctx, W.u, W.h, W.cur)) return rc;
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  W.ready = true;
+  return 0;
+}
+
+int mpas_sw_state_download(mpas_ctx *ctx, double *h, double *u, double *tracers) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->sw || !ctx->sw->ready) MPAS_FAIL(MPAS_ESTATE, "no state uploaded");
+  const DMesh &M = ctx->dm;
+  SwState &W = *ctx->sw;
+  const size_t cb = sizeof(double) * M.nC * M.nL, eb = sizeof(double) * M.nE * M.nL;
+  if (h) MPAS_HIP(hipMemcpyAsync(h, W.h, cb, hipMemcpyDeviceToHost, ctx->stream));
+  if (u) MPAS_HIP(hipMemcpyAsync(u, W.u, eb, hipMemcpyDeviceToHost, ctx->stream));
+  if (tracers && M.nT)
+    MPAS_HIP(hipMemcpyAsync(tracers, W.tr, cb * M.nT, hipMemcpyDeviceToHost, ctx->stream));
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host, int64_t n) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!name || !host) MPAS_FAIL(MPAS_EINVAL, "mpas_sw_field_download: NULL");
+  if (!ctx->sw || !ctx->sw->ready) MPAS_FAIL(MPAS_ESTATE, "no state uploaded");
+  const DMesh &M = ctx->dm;
+  SwState &W = *ctx->sw;
+  const std::string s(name);
+  const int64_t C = int64_t(M.nC) * M.nL, E = int64_t(M.nE) * M.nL, V = int64_t(M.nV) * M.nL;
+  const double *src = nullptr;
+  int64_t len = 0;
+  if (s == "h") src = W.h, len = C;
+  else if (s == "u") src = W.u, len = E;
+  else if (s == "tracers") src = W.tr, len = C * M.nT;
+  else if (s == "h_edge") src = W.h_edge[W.cur], len = E;
+  else if (s == "ke") src = W.ke[W.cur], len = C;
+  else if (s == "pv_edge") src = W.pv_edge[W.cur], len = E;
+  else if (s == "vorticity") src = W.vort[W.cur], len = V;
+  else if (s == "pv_vertex") src = W.pv_vertex, len = V;
+  else if (s == "h_s") src = W.h_s, len = M.nC;
+  else if (s == "divergence") src = W.div[W.cur], len = C;
+  else MPAS_FAIL(MPAS_EINVAL, "unknown field '" + s + "'");
+  if (n != len) MPAS_FAIL(MPAS_ESHAPE, s + " has " + std::to_string(len) + " entries");
+  if (len) MPAS_HIP(hipMemcpyAsync(host, src, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream));
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->sw || !ctx->sw->ready)
+    MPAS_FAIL(MPAS_ESTATE, "mpas_rk4_step: upload a state first (mpas_sw_state_upload)");
+  if (nsteps < 0) MPAS_FAIL(MPAS_EINVAL, "mpas_rk4_step: nsteps < 0");
+  if (!(dt > 0)) MPAS_FAIL(MPAS_EINVAL, "mpas_rk4_step: dt must be positive");
+  const DMesh &M = ctx->dm;
+  if (M.nT && ctx->params.tracer_scheme == MPAS_TRACER_FCT) {
+    if (M.mlc) MPAS_FAIL(MPAS_EINVAL,
+                         "the fused FCT step needs every level active (maxLevelCell == nLevels); "
+                         "use the per-task entry points for partial columns");
+    if (!ctx->aligned_c2c)
+      MPAS_FAIL(MPAS_EINVAL, "the fused FCT step needs cellsOnCell[j] across edgesOnCell[j]");
+  }
+  return sw_rk4_steps(ctx, dt, nsteps);
+}
+
+}  // extern "C"
+
+// ---- per-kernel profiling ----------------------------------------------------
+namespace mpasx {
+
+static int prof_event(mpas_ctx *ctx, hipEvent_t *ev) {
+  Prof &P = ctx->prof;
+  if (P.used == P.pool.size()) {
+    hipEvent_t e;
+    MPAS_HIP(hipEventCreate(&e));
+    P.pool.push_back(e);
+  }
+  *ev = P.pool[P.used++];
+  return 0;
+}
+
+int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes) {
+  Prof &P = ctx->prof;
+  if (!P.on) return 0;
+  Prof::Pending p{P.id(kernel), nullptr, nullptr, bytes};
+  if (int rc = prof_event(ctx, &p.start)) return rc;
+  if (int rc = prof_event(ctx, &p.stop)) return rc;
+  MPAS_HIP(hipEventRecord(p.start, ctx->stream));
+  P.pending.push_back(p);
+  return 0;
+}
+
+int prof_end(mpas_ctx *ctx) {
+  Prof &P = ctx->prof;
+  if (!P.on) return 0;
+  MPAS_HIP(hipEventRecord(P.pending.back().stop, ctx->stream));
+  return 0;
+}
+
+}  // namespace mpasx
+
+extern "C" {
+
+void *mpas_ctx_stream(const mpas_ctx *ctx) { return ctx ? ctx->stream : nullptr; }
+
+int mpas_prof_enable(mpas_ctx *ctx, int on) {
+  if (int rc = check_ctx(ctx)) return rc;
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof.on = on != 0;
+  ctx->prof.pending.clear();
+  ctx->prof.used = 0;
+  return 0;
+}
+
+int mpas_prof_report(mpas_ctx *ctx, char *buf, size_t cap) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!buf || !cap) MPAS_FAIL(MPAS_EINVAL, "mpas_prof_report: empty buffer");
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  Prof &P = ctx->prof;
+  std::vector<double> ms(P.names.size(), 0.0), bytes(P.names.size(), 0.0);
+  std::vector<long> n(P.names.size(), 0);
+  for (auto &p : P.pending) {
+    float t = 0;
+    MPAS_HIP(hipEventElapsedTime(&t, p.start, p.stop));
+    ms[p.kernel] += t;
+    bytes[p.kernel] += p.bytes;
+    n[p.kernel]++;
+  }
+  std::string out = "[";
+  for (size_t i = 0; i < P.names.size(); i++) {
+    if (!n[i]) continue;
+    char line[256];
+    std::snprintf(line, sizeof line,
+                  "%s{\"kernel\": \"%s\", \"launches\": %ld, \"total_ms\": %.6f, "
+                  "\"bytes_per_launch\": %.1f}",
+                  out.size() > 1 ? ", " : "", P.names[i].c_str(), n[i], ms[i], bytes[i] / n[i]);
+    out += line;
+  }
+  out += "]";
+  if (out.size() + 1 > cap) MPAS_FAIL(MPAS_ESHAPE, "mpas_prof_report: buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return 0;
+}
+
+}  // extern "C"

@@ -134,7 +134,7 @@ int mg_init_sw_case1(const mg_mesh *m, double a, double *u, double *h) {
   zonal_wind(m, a, u0, u);
   for (int64_t c = 0; c < nC; c++) {
     double s1 = std::sin(0.5 * (latC[c] - theta_c)), s2 = std::sin(0.5 * (lonC[c] - lambda_c));
-    double r = 2 * a * std::asin(std::sqrt(s1 * s1 + std::cos(theta_c) * std::cos(latC[c]) * s2 * s2));
+    double r = 2 * a * std::asin(std::sqrt(s1 * s1 + std::cos(theta_c) * std::cos(latC[c]) * (s2 * s2)));
     h[c] = (r < a / 3.0) ? (h0 / 2.0) * (1.0 + std::cos(kPi * r * 3.0 / a)) : h0 / 2.0;
   }
   return 0;

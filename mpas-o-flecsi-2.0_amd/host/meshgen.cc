@@ -1,9 +1,9 @@
 // Icosahedral spherical Voronoi (TRiSK) mesh generator.  See meshgen.h.
 //
 // Construction:
-//  1. icosahedron with a vertex at each pole, bisected `level` times, every
-//     new point projected to the unit sphere (these are the cell generators,
-//     the triangles are the Voronoi vertices);
+//  1. icosahedron (a vertex at each pole, then a fixed generic rotation),
+//     bisected `level` times, every new point projected to the unit sphere
+//     (these are the cell generators, the triangles are the Voronoi vertices);
 //  2. `lloyd_iters` Lloyd iterations towards a centroidal Voronoi
 //     tessellation with the triangulation held fixed (checked to stay
 //     Delaunay and well centred);
@@ -123,7 +123,7 @@ struct Tri {
 };
 
 void build(mg_mesh &M, int level, double radius, int lloyd_iters, int order,
-           double omega) {
+           double omega, int rotate) {
   if (level < 0 || level > 10) throw std::runtime_error("level must be in [0,10]");
   if (!(radius > 0)) throw std::runtime_error("radius must be positive");
 
@@ -144,6 +144,20 @@ void build(mg_mesh &M, int level, double radius, int lloyd_iters, int order,
                  -std::sin(lat0)});
   }
   P.push_back({0, 0, -1});
+  // A fixed generic rotation keeps every generator, Voronoi vertex and edge
+  // midpoint off the coordinate planes: the reference's FCT setup asserts
+  // that no arc midpoint has a zero coordinate (mpasoflecsi/eqns/metrics.hh:
+  // 104-105), which the symmetric icosahedron violates from level 6 on.
+  if (rotate) {
+    const double ax = 0.3117, ay = 0.1731, az = 0.0953;  // radians
+    const double cx = std::cos(ax), sx = std::sin(ax), cy = std::cos(ay), sy = std::sin(ay);
+    const double cz = std::cos(az), sz = std::sin(az);
+    for (auto &p : P) {
+      V3 q = {p.x, cx * p.y - sx * p.z, sx * p.y + cx * p.z};
+      q = {cy * q.x + sy * q.z, q.y, -sy * q.x + cy * q.z};
+      p = unit({cz * q.x - sz * q.y, sz * q.x + cz * q.y, q.z});
+    }
+  }
   auto U = [](int k) { return 1 + (k % 5); };
   auto L = [](int k) { return 6 + (k % 5); };
   for (int k = 0; k < 5; k++) {
@@ -515,10 +529,15 @@ extern "C" {
 
 mg_mesh *mg_generate(int level, double radius, int lloyd_iters, int order,
                      double omega) {
+  return mg_generate_oriented(level, radius, lloyd_iters, order, omega, 1);
+}
+
+mg_mesh *mg_generate_oriented(int level, double radius, int lloyd_iters, int order,
+                              double omega, int rotate) {
   try {
     auto *m = new mg_mesh();
     try {
-      build(*m, level, radius, lloyd_iters, order, omega);
+      build(*m, level, radius, lloyd_iters, order, omega, rotate);
     } catch (...) {
       delete m;
       throw;

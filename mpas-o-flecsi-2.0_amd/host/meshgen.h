@@ -26,6 +26,12 @@ enum mg_order { MG_ORDER_NATIVE = 0, MG_ORDER_HILBERT = 1 };
 // Returns NULL on failure (mg_last_error() says why).
 mg_mesh *mg_generate(int level, double radius, int lloyd_iters, int order,
                      double omega);
+// rotate = 0 keeps the icosahedron with a vertex at each pole (the usual
+// orientation of MPAS quasi-uniform meshes); mg_generate uses rotate = 1,
+// a fixed generic rotation that keeps arc midpoints off the coordinate
+// planes the reference's FCT setup may not meet.
+mg_mesh *mg_generate_oriented(int level, double radius, int lloyd_iters, int order,
+                              double omega, int rotate);
 void mg_free(mg_mesh *m);
 const char *mg_last_error(void);
 

@@ -83,7 +83,7 @@ struct SwRun {
   }
 
   // Tracer tendency of one stage.  The reference SW driver uses the centred
-  // compute_scalar_tend (apps/sw/run.cc:33-35).  With tracer_adv == 1 the
+  // compute_scalar_tend (apps/sw/run.cc:33-35).  With tracer_adv == 2 the
   // ocean FCT tasks (apps/ocean/run.cc:19-42) are driven from the provisional
   // SW state instead: layerThicknessEdge = h_edge[provis], normalVelocity =
   // u[provis], layerThickness = h[provis], tracers[prev] = tracers[provis].
@@ -91,7 +91,7 @@ struct SwRun {
     const int nL = p.nL, nT = p.nT;
     if (nT == 0) return;
     const int P = PROVIS;
-    if (p.tracer_adv == 1) {
+    if (p.tracer_adv == 2) {
       ocn_reinitialize_tracer_tendencies(*m, nL, nT, tend_tr.data());
       ocn_calc_normal_thickness_flux(*m, nL, h_edge[P].data(), u[P].data(), ntf.data());
       ocn_calc_provisional_layer_thicknesses(*m, nL, h[P].data(), ntf.data(), hProv.data(),
