@@ -65,6 +65,24 @@ def cpu_baseline(mesh, u, h, tr, h_s, dt, steps, n_tr):
                       f"{sec:.2f} s timed, {setup_s:.1f} s setup untimed"}
 
 
+def copy_calibration(nbytes):
+    """k_copy over nbytes, 10 launches: reads and writes nbytes each."""
+    from mpas_hip import Context, Mesh
+    nbytes -= nbytes % 16
+    ctx = Context(Mesh(0))
+    a, b = ctx.alloc(nbytes // 8), ctx.alloc(nbytes // 8)
+    ctx.copy_probe(b, a, nbytes)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        ctx.copy_probe(b, a, nbytes)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / 10
+    print(json.dumps({"copy_bytes": nbytes, "launches": 11, "s_per_launch": dt,
+                      "GBs_read_plus_write": 2 * nbytes / dt / 1e9}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +95,9 @@ def main():
                     help="steps of the per-kernel event-timed pass")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel k_copy over BYTES, 10 times "
+                         "(the known-size reference for HBM counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -85,6 +106,10 @@ def main():
         sys.exit(2)
 
     from mpas_hip import TRACER_FCT, Context, Mesh, sw_workload_state
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     t0 = time.perf_counter()
     mesh = Mesh(args.level)

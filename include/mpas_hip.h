@@ -129,6 +129,9 @@ int mpas_buf_alloc(mpas_ctx *ctx, size_t bytes, void **dptr);
 int mpas_buf_free(mpas_ctx *ctx, void *dptr);
 int mpas_buf_upload(mpas_ctx *ctx, void *dptr, const void *host, size_t bytes);
 int mpas_buf_download(mpas_ctx *ctx, void *host, const void *dptr, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (k_copy), asynchronous:
+   a known-size reference for bandwidth and HBM-counter calibration. */
+int mpas_buf_copy(mpas_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Host-side mesh preparation alone (no device needed): the topology
    flattening of mesh::initialize (mesh.cc:67-116) and, when the parameters

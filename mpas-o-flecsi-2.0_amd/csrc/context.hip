@@ -3,6 +3,7 @@
 // storage FleCSI owns in the reference, mpasoflecsi/specialization/mesh.cc:67-116,
 // apps/sw/state.hh:74-87).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -76,6 +77,15 @@ int build_device_mesh(mpas_ctx *ctx) {
   M.wEoE = max_row(m.e2e);
   if (M.wC > 8) MPAS_FAIL(MPAS_EINVAL, "cells with more than 8 edges are not supported");
   if (M.wEoE > 16) MPAS_FAIL(MPAS_EINVAL, "edgesOnEdge rows hold at most 16 entries");
+  M.wAdv = 0;
+  if (m.fct)
+    for (int e = 0; e < E; e++) M.wAdv = std::max(M.wAdv, int(m.nAdvCellsForEdge[e]));
+  // MPAS_HIP_WIDE_KERNELS=1 selects the 8 / 16 widths on any mesh (they are
+  // otherwise used only by meshes with heptagons or wider stencils)
+  const char *wide = std::getenv("MPAS_HIP_WIDE_KERNELS");
+  M.narrow = M.wC <= 6 && M.wEoE <= 10 && M.wAdv <= 10 && !(wide && wide[0] == '1');
+  // ELL widths as allocated: the kernels' compile-time widths
+  const int padC = M.narrow ? 6 : 8, padE = M.narrow ? 10 : 16;
 
   int rc = 0;
   std::vector<int2> e2c(E), e2v(E);
@@ -88,19 +98,19 @@ int build_device_mesh(mpas_ctx *ctx) {
     return rc;
   {
     std::vector<uint8_t> n(E);
-    std::vector<double> w(size_t(M.wEoE) * E, 0.0);
+    std::vector<double> w(size_t(padE) * E, 0.0);
     for (int e = 0; e < E; e++) {
       n[e] = uint8_t(m.e2e.size(e));
       for (int j = 0; j < m.e2e.size(e); j++) w[size_t(j) * E + e] = m.weightsOnEdge[size_t(e) * 16 + j];
     }
-    if ((rc = upload(ctx, n, &M.nEoE)) || (rc = upload(ctx, to_ell(m.e2e, M.wEoE), &M.eoe)) ||
+    if ((rc = upload(ctx, n, &M.nEoE)) || (rc = upload(ctx, to_ell(m.e2e, padE), &M.eoe)) ||
         (rc = upload(ctx, w, &M.woe)))
       return rc;
   }
   // cells
   {
     std::vector<uint8_t> n(C), neg(C, 0);
-    std::vector<uint32_t> perm(C, 0);
+    std::vector<uint32_t> perm(C, 0), rank(C, 0xffffffffu);
     Csr opp;
     opp.off = m.c2e.off;
     opp.idx.resize(m.c2e.idx.size());
@@ -118,22 +128,27 @@ int build_device_mesh(mpas_ctx *ctx) {
         slot[j] = j;
       }
       std::sort(slot, slot + ne, [&](int a, int b) { return m.c2e.at(c, a) < m.c2e.at(c, b); });
-      for (int r = 0; r < ne; r++) perm[c] |= uint32_t(slot[r]) << (4 * r);
+      for (int r = 0; r < ne; r++) {
+        perm[c] |= uint32_t(slot[r]) << (4 * r);
+        rank[c] &= ~(0xfu << (4 * slot[r]));
+        rank[c] |= uint32_t(r) << (4 * slot[r]);
+      }
       if (m.c2c.size(c) != ne) aligned = false;
     }
     ctx->aligned_c2c = aligned;
     if ((rc = upload(ctx, m.areaCell, &M.area)) || (rc = upload(ctx, n, &M.nEoC)) ||
         (rc = upload(ctx, neg, &M.sgnNeg)) || (rc = upload(ctx, perm, &M.ascPerm)) ||
-        (rc = upload(ctx, to_ell(m.c2e, M.wC), &M.c2e)))
+        (rc = upload(ctx, rank, &M.ascRank)) ||
+        (rc = upload(ctx, to_ell(m.c2e, padC), &M.c2e)))
       return rc;
     if (aligned) {
-      if ((rc = upload(ctx, to_ell(m.c2c, M.wC), &M.c2c))) return rc;
+      if ((rc = upload(ctx, to_ell(m.c2c, padC), &M.c2c))) return rc;
       M.cOpp = M.c2c;  // cellsOnCell[j] lies across edgesOnCell[j]: one array serves both
     } else {
-      if ((rc = upload(ctx, to_ell(opp, M.wC), &M.cOpp))) return rc;
+      if ((rc = upload(ctx, to_ell(opp, padC), &M.cOpp))) return rc;
       int wcc = std::max(1, max_row(m.c2c));
       if (wcc > 8) MPAS_FAIL(MPAS_EINVAL, "cellsOnCell rows hold at most 8 entries");
-      if ((rc = upload(ctx, to_ell(m.c2c, std::max(wcc, M.wC)), &M.c2c))) return rc;
+      if ((rc = upload(ctx, to_ell(m.c2c, std::max(wcc, padC)), &M.c2c))) return rc;
     }
   }
   // vertices
@@ -158,14 +173,12 @@ int build_device_mesh(mpas_ctx *ctx) {
   // FCT / del2 setup products
   M.msd = m.fct ? m.meshScalingDel2[0] : 1.0;
   if (m.fct) {
-    M.wAdv = 0;
-    for (int e = 0; e < E; e++) M.wAdv = std::max(M.wAdv, int(m.nAdvCellsForEdge[e]));
     std::vector<uint8_t> na(E);
-    std::vector<int32_t> adv(size_t(M.wAdv) * E);
-    std::vector<double> c1(size_t(M.wAdv) * E, 0.0), c3(size_t(M.wAdv) * E, 0.0);
+    std::vector<int32_t> adv(size_t(padE) * E);
+    std::vector<double> c1(size_t(padE) * E, 0.0), c3(size_t(padE) * E, 0.0);
     for (int e = 0; e < E; e++) {
       na[e] = uint8_t(m.nAdvCellsForEdge[e]);
-      for (int i = 0; i < M.wAdv; i++) {
+      for (int i = 0; i < padE; i++) {
         const bool live = i < m.nAdvCellsForEdge[e];
         adv[size_t(i) * E + e] = live ? m.advCellsForEdge[size_t(e) * 16 + i] : m.e2c.at(e, 0);
         c1[size_t(i) * E + e] = live ? m.advCoef[size_t(e) * 16 + i] : 0.0;

@@ -19,6 +19,10 @@ struct DMesh {
   int wC;    // ELL width of the per-cell lists (max edges of a cell, <= 8)
   int wEoE;  // ELL width of edgesOnEdge
   int wAdv;  // ELL width of advCellsForEdge
+  // Padded widths the fused kernels read unconditionally: 6 / 10 when every
+  // cell has <= 6 edges and every stencil <= 10 entries, else 8 / 16.  The
+  // ELL arrays are allocated with (at least) these widths.
+  int narrow;
   // edges
   const int2 *e2c;  // m.cells(e): ascending cell id (transpose_cnx)
   const int2 *e2v;  // m.vertices(e): file order
@@ -34,6 +38,7 @@ struct DMesh {
   const int *cOpp;       // [j][C] the other cell of edge c2e[j]
   const uint8_t *sgnNeg;    // bit j: c == m.cells(c2e[j])[0], edgeSignOnCell = -1
   const uint32_t *ascPerm;  // nibble r: slot j of the r-th smallest edge id
+  const uint32_t *ascRank;  // nibble j: rank of slot j's edge id (15 past the count)
   // vertices
   const double *areaTri, *fV;
   const double *kite;  // [i][V] kiteAreasOnVertex, file order
@@ -68,7 +73,40 @@ __device__ __forceinline__ double sign_on_cell(uint8_t neg, int j) {
 __device__ __forceinline__ int v2e_id(int x) { return x & 0x7fffffff; }
 __device__ __forceinline__ double v2e_sign(int x) { return x < 0 ? -1.0 : 1.0; }
 
+// Loads and stores at a 32-bit element index from a base held in scalar
+// registers: the byte offset is formed in 32 bits, which lets the compiler
+// use the scalar-base form of global memory instructions (one VGPR per
+// address instead of two).  Every array handed to the kernels is below
+// 4 GiB (checked when the context is created).
+template <class T>
+__device__ __forceinline__ T ld(const T *__restrict__ p, uint32_t i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) +
+                                      i * uint32_t(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void st(T *p, uint32_t i, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + i * uint32_t(sizeof(T))) = v;
+}
+
 // Grid-stride helpers: one work item per lane.
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Work item of this lane under XCD-aware placement.  The MI355X hands
+// workgroups to its 8 XCDs round-robin (workgroup b runs on XCD b % 8, each
+// XCD with its own L2).  Giving XCD x the contiguous range of logical blocks
+// [x * per, (x + 1) * per) keeps neighbouring mesh entities (consecutive ids
+// along the Hilbert numbering) in one L2, so data shared across block
+// boundaries is fetched once instead of once per XCD.  The grid must be
+// 8 * per blocks (xcd_grid); items past n return.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item() {
+  const int per = gridDim.x / kXcds;
+  const int b = blockIdx.x;
+  return ((b % kXcds) * per + b / kXcds) * blockDim.x + threadIdx.x;
+}
+inline int xcd_grid(int64_t n, int bs) {
+  const int64_t blocks = (n + bs - 1) / bs;
+  return int((blocks + kXcds - 1) / kXcds * kXcds);
+}
 
 }  // namespace mpasx

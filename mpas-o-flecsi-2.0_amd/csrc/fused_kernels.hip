@@ -43,39 +43,101 @@ namespace {
 constexpr int BS = 256;
 
 template <int NT>
-__device__ __forceinline__ void load_tr(const double *__restrict__ p, double (&v)[NT]) {
+__device__ __forceinline__ void load_tr(const double *__restrict__ p, uint32_t i0,
+                                        double (&v)[NT]) {
 #pragma unroll
-  for (int t = 0; t < NT; t++) v[t] = p[t];
+  for (int t = 0; t < NT; t++) v[t] = ld(p, i0 + t);
 }
 
+// Adds the live slots' values in ascending edge id, the order of the
+// reference's scatter loops: at step r the slot of rank r is selected with
+// compile-time indices only (a run-time index into a register array would
+// send the array to scratch memory).  acc -= v where the cell is the edge's
+// first cell (negative edge sign), acc += v otherwise.
+template <int W>
+__device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W],
+                                                uint32_t rank, uint8_t neg, int n) {
+#pragma unroll
+  for (int r = 0; r < W; r++) {
+    double f = 0.0;
+    bool ng = false;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const bool hit = ((rank >> (4 * j)) & 15) == uint32_t(r);
+      f = hit ? v[j] : f;
+      ng = hit ? bool((neg >> j) & 1) : ng;
+    }
+    const double x = ng ? acc - f : acc + f;
+    acc = (r < n) ? x : acc;
+  }
+  return acc;
+}
+
+// The stage kernels read every slot of a ragged list up front, with widths
+// fixed at compile time (WE edgesOnEdge, WA advection cells, W edges of a
+// cell): all index loads issue together, then all gathers, so a lane waits
+// for two memory round trips instead of two per slot.  Slots past a row's
+// count read the ELL padding (valid ids) and are masked out of every sum,
+// which still runs over the live slots in the reference's order.
+
 // ---- edges ------------------------------------------------------------------
-template <int NT>
+template <int NT, int WE, int WA>
 __global__ __launch_bounds__(BS) void k_stage_edges(DMesh M, StageIO S) {
-  const int e = gtid();
+  const int e = xcd_item();
   if (e >= M.nE) return;
-  const int L = M.nL;
-  const int2 cc = M.e2c[e];
+  const uint32_t L = M.nL, E = M.nE;
+  const int2 cc = ld(M.e2c, e);
 
   // compute_tend (+ viscosity), compute_substep, accumulate_update and, at
   // the last stage, finalize_timestep for u (solve.cc:215-233, 285-289,
   // 326-330, 359-366)
-  for (int k = 0; k < L; k++) {
-    const size_t o = size_t(e) * L + k;
-    if (S.freeze_u) {
-      if (S.stage < 3) S.yu[o] = S.u0[o];
-      continue;
+  if (S.freeze_u) {
+    if (S.stage < 3)
+      for (uint32_t k = 0; k < L; k++) st(S.yu, e * L + k, ld(S.u0, e * L + k));
+  } else {
+    const int n = ld(M.nEoE, e);
+    uint32_t id[WE];
+    double w[WE];
+#pragma unroll
+    for (int j = 0; j < WE; j++) {
+      id[j] = ld(M.eoe, j * E + e);
+      w[j] = ld(M.woe, j * E + e);
     }
-    double t = tend_u_at(M, e, k, cc, S.xu, S.h_edge, S.pv_edge, S.ke, S.xh, S.h_s);
-    if (S.use_visc) t += visc_at(M, e, k, cc, M.e2v[e], S.div, S.vort, S.visc_coef);
-    if (S.stage == 0) {
-      const double u0 = S.u0[o];
-      S.yu[o] = u0 + S.w_sub * t;
-      S.nu[o] = u0 + S.w_rk * t;
-    } else if (S.stage < 3) {
-      S.yu[o] = S.u0[o] + S.w_sub * t;
-      S.nu[o] += S.w_rk * t;
-    } else {
-      S.yu[o] = S.nu[o] + S.w_rk * t;
+    const double dc = ld(M.dc, e);
+    for (uint32_t k = 0; k < L; k++) {
+      const uint32_t o = e * L + k;
+      double uu[WE], pv[WE], he[WE];
+#pragma unroll
+      for (int j = 0; j < WE; j++) {
+        const uint32_t oj = id[j] * L + k;
+        uu[j] = ld(S.xu, oj);
+        pv[j] = ld(S.pv_edge, oj);
+        he[j] = ld(S.h_edge, oj);
+      }
+      const double pve = ld(S.pv_edge, o);
+      double q = 0;
+#pragma unroll
+      for (int j = 0; j < WE; j++) {
+        const double workpv = 0.5 * (pve + pv[j]);
+        const double qn = q + w[j] * uu[j] * workpv * he[j];
+        q = (j < n) ? qn : q;
+      }
+      const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
+      double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
+                      M.gravity * (ld(S.xh, o2) + ld(S.h_s, cc.y) - ld(S.xh, o1) -
+                                   ld(S.h_s, cc.x))) /
+                         dc;
+      if (S.use_visc) t += visc_at(M, e, k, cc, ld(M.e2v, e), S.div, S.vort, S.visc_coef);
+      if (S.stage == 0) {
+        const double u0 = ld(S.u0, o);
+        st(S.yu, o, u0 + S.w_sub * t);
+        st(S.nu, o, u0 + S.w_rk * t);
+      } else if (S.stage < 3) {
+        st(S.yu, o, ld(S.u0, o) + S.w_sub * t);
+        st(S.nu, o, ld(S.nu, o) + S.w_rk * t);
+      } else {
+        st(S.yu, o, ld(S.nu, o) + S.w_rk * t);
+      }
     }
   }
 
@@ -83,33 +145,41 @@ __global__ __launch_bounds__(BS) void k_stage_edges(DMesh M, StageIO S) {
     if (S.scheme != MPAS_TRACER_FCT) return;
     // calc_normal_thickness_flux, high-order flux and its correction by the
     // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229)
-    const double dv = M.dv[e];
-    const int n = M.nAdv[e];
-    for (int k = 0; k < L; k++) {
-      const size_t o = size_t(e) * L + k;
-      const double F = S.xu[o] * S.h_edge[o];
-      const int mask = M.hoMask ? M.hoMask[o] : 1;
+    const double dv = ld(M.dv, e);
+    const int na = ld(M.nAdv, e);
+    uint32_t ic[WA];
+    double c1[WA], c3[WA];
+#pragma unroll
+    for (int i = 0; i < WA; i++) {
+      ic[i] = ld(M.adv, i * E + e);
+      c1[i] = ld(M.advCoef, i * E + e);
+      c3[i] = ld(M.advCoef3, i * E + e) * M.coef3rd;
+    }
+    for (uint32_t k = 0; k < L; k++) {
+      const uint32_t o = e * L + k;
+      const double F = ld(S.xu, o) * ld(S.h_edge, o);
+      const int mask = M.hoMask ? ld(M.hoMask, o) : 1;
       const double wgt = F * double(mask);
       const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
       double ho[NT];
 #pragma unroll
       for (int t = 0; t < NT; t++) ho[t] = 0.0;
-      for (int i = 0; i < n; i++) {
-        const int ic = M.adv[size_t(i) * M.nE + e];
-        if (k >= lvl_cell(M, ic)) continue;
-        const double coef1 = M.advCoef[size_t(i) * M.nE + e];
-        const double coef3 = M.advCoef3[size_t(i) * M.nE + e] * M.coef3rd;
-        const double weightFactor = wgt * (coef1 + coef3 * sgn);
-        double tv[NT];
-        load_tr<NT>(S.xtr + (size_t(ic) * L + k) * NT, tv);
 #pragma unroll
-        for (int t = 0; t < NT; t++) ho[t] += tv[t] * weightFactor;
+      for (int i = 0; i < WA; i++) {
+        const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
+        double tv[NT];
+        load_tr<NT>(S.xtr, (ic[i] * L + k) * NT, tv);
+        const bool live = i < na && int(k) < lvl_cell(M, ic[i]);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const double x = ho[t] + tv[t] * weightFactor;
+          ho[t] = live ? x : ho[t];
+        }
       }
-      double *out = S.anti + o * NT;
-      if (k < lvl_edge(M, e)) {
+      if (int(k) < lvl_edge(M, e)) {
         double t1[NT], t2[NT];
-        load_tr<NT>(S.xtr + (size_t(cc.x) * L + k) * NT, t1);
-        load_tr<NT>(S.xtr + (size_t(cc.y) * L + k) * NT, t2);
+        load_tr<NT>(S.xtr, (uint32_t(cc.x) * L + k) * NT, t1);
+        load_tr<NT>(S.xtr, (uint32_t(cc.y) * L + k) * NT, t2);
         const double tracerWeight = double((mask + 1) & 1) * (dv * 0.5) * F;
         const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
 #pragma unroll
@@ -118,202 +188,185 @@ __global__ __launch_bounds__(BS) void k_stage_edges(DMesh M, StageIO S) {
           double x = ho[t];
           x += tracerWeight * (t1[t] + t2[t]);
           x -= lo;
-          out[t] = x;
+          st(S.anti, o * NT + t, x);
         }
       } else {
 #pragma unroll
-        for (int t = 0; t < NT; t++) out[t] = ho[t];
+        for (int t = 0; t < NT; t++) st(S.anti, o * NT + t, ho[t]);
       }
     }
   }
 }
 
-// ---- cells ------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
-  const int c = gtid();
-  if (c >= M.nC) return;
-  const int L = M.nL, C = M.nC;
-  const int n = M.nEoC[c];
-  const uint8_t neg = M.sgnNeg[c];
-  const uint32_t perm = M.ascPerm[c];
-  const double ar = M.area[c];
+// compute_substep / accumulate_update / finalize_timestep for one tracer of
+// one cell (solve.cc:297-305, 332-340, 350-357)
+__device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uint32_t ot,
+                                              double te, double hy) {
+  if (S.stage == 0) {
+    const double h0 = ld(S.h0, ol), t0 = ld(S.tr0, ot);
+    st(S.ytr, ot, (h0 * t0 + S.w_sub * te) / hy);
+    st(S.ntr, ot, t0 * h0 + S.w_rk * te);
+  } else if (S.stage < 3) {
+    st(S.ytr, ot, (ld(S.h0, ol) * ld(S.tr0, ot) + S.w_sub * te) / hy);
+    st(S.ntr, ot, ld(S.ntr, ot) + S.w_rk * te);
+  } else {
+    st(S.ytr, ot, (ld(S.ntr, ot) + S.w_rk * te) / hy);
+  }
+}
 
-  for (int k = 0; k < L; k++) {
-    const size_t ol = size_t(c) * L + k;
+// ---- cells ------------------------------------------------------------------
+template <int NT, int W>
+__global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
+  const int c = xcd_item();
+  if (c >= M.nC) return;
+  const uint32_t L = M.nL, C = M.nC;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const uint32_t rank = ld(M.ascRank, c);
+  const double ar = ld(M.area, c);
+  uint32_t e[W], op[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    e[j] = ld(M.c2e, j * C + c);
+    op[j] = ld(M.cOpp, j * C + c);
+  }
+  double dv[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) dv[j] = ld(M.dv, e[j]);
+
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t ol = c * L + k;
+    // ke (solve.cc:63-77) and divergence of the new state
+    {
+      double yu[W], dc[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        yu[j] = ld(S.yu, e[j] * L + k);
+        dc[j] = ld(M.dc, e[j]);
+      }
+      double ke = 0.0, dvg = 0.0;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const double kx = ke + 0.25 * dc[j] * dv[j] * (yu[j] * yu[j]);
+        const double dx = dvg + (-sign_on_cell(neg, j)) * dv[j] * yu[j];
+        ke = (j < n) ? kx : ke;
+        dvg = (j < n) ? dx : dvg;
+      }
+      st(S.ke_y, ol, ke / ar);
+      if (S.use_visc) st(S.div_y, ol, dvg / ar);
+    }
+    double xu[W], he[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      xu[j] = ld(S.xu, e[j] * L + k);
+      he[j] = ld(S.h_edge, e[j] * L + k);
+    }
     // compute_tend, thickness part: edges in ascending id (solve.cc:191-205)
-    double th = 0;
-    for (int r = 0; r < n; r++) {
-      const int j = (perm >> (4 * r)) & 15;
-      const int e = M.c2e[j * C + c];
-      const size_t oe = size_t(e) * L + k;
-      const double flux = S.xu[oe] * M.dv[e] * S.h_edge[oe];
-      if ((neg >> j) & 1)
-        th -= flux;
-      else
-        th += flux;
+    double th;
+    {
+      double fl[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) fl[j] = xu[j] * dv[j] * he[j];
+      th = sum_ascending<W>(0.0, fl, rank, neg, n);
     }
     th = th / ar;
     double hy;
     if (S.stage == 0) {
-      const double h0 = S.h0[ol];
+      const double h0 = ld(S.h0, ol);
       hy = h0 + S.w_sub * th;
-      S.nh[ol] = h0 + S.w_rk * th;
+      st(S.nh, ol, h0 + S.w_rk * th);
     } else if (S.stage < 3) {
-      hy = S.h0[ol] + S.w_sub * th;
-      S.nh[ol] += S.w_rk * th;
+      hy = ld(S.h0, ol) + S.w_sub * th;
+      st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
     } else {
-      hy = S.nh[ol] + S.w_rk * th;
+      hy = ld(S.nh, ol) + S.w_rk * th;
     }
-    S.yh[ol] = hy;
-
-    // ke (solve.cc:63-77) and divergence of the new state
-    double ke = 0.0, dvg = 0.0;
-    for (int j = 0; j < n; j++) {
-      const int e = M.c2e[j * C + c];
-      const double uy = S.yu[size_t(e) * L + k];
-      ke += 0.25 * M.dc[e] * M.dv[e] * (uy * uy);
-      if (S.use_visc) dvg += (-sign_on_cell(neg, j)) * M.dv[e] * uy;
-    }
-    S.ke_y[ol] = ke / ar;
-    if (S.use_visc) S.div_y[ol] = dvg / ar;
+    st(S.yh, ol, hy);
 
     if constexpr (NT > 0) {
-      double tc[NT];
-      load_tr<NT>(S.xtr + ol * NT, tc);
+      const double invArea = 1.0 / ar;
       if (S.scheme == MPAS_TRACER_CENTRED) {
-        // compute_scalar_tend (solve.cc:237-270), ascending edge id
-        double te[NT];
+        // compute_scalar_tend (solve.cc:237-270), ascending edge id, then
+        // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order; one
+        // tracer at a time keeps the slot arrays in registers
+#pragma unroll 1
+        for (int t = 0; t < NT; t++) {
+          const double tc = ld(S.xtr, ol * NT + t);
+          double tn[W], term[W];
 #pragma unroll
-        for (int t = 0; t < NT; t++) te[t] = 0.0;
-        for (int r = 0; r < n; r++) {
-          const int j = (perm >> (4 * r)) & 15;
-          const int e = M.c2e[j * C + c];
-          const size_t oe = size_t(e) * L + k;
-          double to[NT];
-          load_tr<NT>(S.xtr + (size_t(M.cOpp[j * C + c]) * L + k) * NT, to);
-          const bool first = (neg >> j) & 1;  // c == m.cells(e)[0]
-          const double base = S.xu[oe] * M.dv[e] * S.h_edge[oe];
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            const double tracer_edge = first ? 0.5 * (tc[t] + to[t]) : 0.5 * (to[t] + tc[t]);
-            const double flux = base * tracer_edge;
-            if (first)
-              te[t] -= flux / ar;
-            else
-              te[t] += flux / ar;
-          }
-        }
-        if (S.use_hmix) {
-          // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order
-          const double invArea = 1.0 / ar;
-          for (int j = 0; j < n; j++) {
-            const int e = M.c2e[j * C + c];
-            const double rho_scaling = M.msd * S.kappa * M.dv[e] / M.dc[e];
-            const double lte = S.h_edge[size_t(e) * L + k];
-            double to[NT];
-            load_tr<NT>(S.xtr + (size_t(M.cOpp[j * C + c]) * L + k) * NT, to);
+          for (int j = 0; j < W; j++) {
+            tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
             const bool first = (neg >> j) & 1;
-            const double sg = first ? -1.0 : 1.0;
+            const double tracer_edge = first ? 0.5 * (tc + tn[j]) : 0.5 * (tn[j] + tc);
+            term[j] = (xu[j] * dv[j] * he[j] * tracer_edge) / ar;
+          }
+          double te = sum_ascending<W>(0.0, term, rank, neg, n);
+          if (S.use_hmix) {
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-              const double turb = first ? to[t] - tc[t] : tc[t] - to[t];
-              const double flux = lte * turb * rho_scaling;
-              te[t] -= sg * flux * invArea;
+            for (int j = 0; j < W; j++) {
+              const bool first = (neg >> j) & 1;
+              const double rho_scaling = M.msd * S.kappa * dv[j] / ld(M.dc, e[j]);
+              const double turb = first ? tn[j] - tc : tc - tn[j];
+              const double flux = he[j] * turb * rho_scaling;
+              const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
+              te = (j < n) ? x : te;
             }
           }
-        }
-        // compute_substep / accumulate_update / finalize_timestep (tracers)
-        double *yt = S.ytr + ol * NT;
-        double *nt = S.ntr + ol * NT;
-        if (S.stage == 0) {
-          const double h0 = S.h0[ol];
-          double t0[NT];
-          load_tr<NT>(S.tr0 + ol * NT, t0);
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            yt[t] = (h0 * t0[t] + S.w_sub * te[t]) / hy;
-            nt[t] = t0[t] * h0 + S.w_rk * te[t];
-          }
-        } else if (S.stage < 3) {
-          const double h0 = S.h0[ol];
-          double t0[NT];
-          load_tr<NT>(S.tr0 + ol * NT, t0);
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            yt[t] = (h0 * t0[t] + S.w_sub * te[t]) / hy;
-            nt[t] += S.w_rk * te[t];
-          }
-        } else {
-#pragma unroll
-          for (int t = 0; t < NT; t++) yt[t] = (nt[t] + S.w_rk * te[t]) / hy;
+          tracer_update(S, ol, ol * NT + t, te, hy);
         }
       } else if (S.scheme == MPAS_TRACER_FCT) {
         // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
         const double dtInvAreaCell = S.dt / ar;
-        double hp = S.xh[ol];
-        for (int j = 0; j < n; j++) {
-          const int e = M.c2e[j * C + c];
-          const size_t oe = size_t(e) * L + k;
-          const double signedFactor = dtInvAreaCell * M.dv[e] * sign_on_cell(neg, j);
-          hp += signedFactor * (S.xu[oe] * S.h_edge[oe]);
+        const double lt = ld(S.xh, ol);
+        double hp = lt;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const double signedFactor = dtInvAreaCell * dv[j] * sign_on_cell(neg, j);
+          const double x = hp + signedFactor * (xu[j] * he[j]);
+          hp = (j < n) ? x : hp;
         }
         const double hProvInv = 1.0 / hp;
-        // bounds over cellsOnCell (195-214)
-        double tmax[NT], tmin[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) tmax[t] = tmin[t] = tc[t];
-        for (int j = 0; j < n; j++) {
-          double tn[NT];
-          load_tr<NT>(S.xtr + (size_t(M.c2c[j * C + c]) * L + k) * NT, tn);
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            tmax[t] = smax(tmax[t], tn[t]);
-            tmin[t] = smin(tmin[t], tn[t]);
-          }
-        }
-        // upwind tendency and antidiffusive in/out sums, file order (232-249)
-        const double invAreaCell1 = 1.0 / ar;
-        double te[NT], fin[NT], fout[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) te[t] = fin[t] = fout[t] = 0.0;
-        for (int j = 0; j < n; j++) {
-          const int e = M.c2e[j * C + c];
-          const size_t oe = size_t(e) * L + k;
-          const double signedFactor = sign_on_cell(neg, j) * invAreaCell1;
-          const double F = S.xu[oe] * S.h_edge[oe];
-          const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
-          const double dv = M.dv[e];
-          double to[NT], an[NT];
-          load_tr<NT>(S.xtr + (size_t(M.cOpp[j * C + c]) * L + k) * NT, to);
-          load_tr<NT>(S.anti + oe * NT, an);
-          const bool first = (neg >> j) & 1;
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            const double t1 = first ? tc[t] : to[t];
-            const double t2 = first ? to[t] : tc[t];
-            const double lo = dv * (ntf1 * t1 + ntf2 * t2);
-            te[t] += signedFactor * lo;
-            fin[t] += smax(0.0, signedFactor * an[t]);
-            fout[t] += smin(0.0, signedFactor * an[t]);
-          }
-        }
-        // limiter factors (253-266)
         const double eps = 1.e-10;
-        const double lt = S.xh[ol];
-        double *rf = S.rfac + ol * 2 * NT;
-        double *tl = S.tlo + ol * NT;
-#pragma unroll
+#pragma unroll 1
         for (int t = 0; t < NT; t++) {
-          const double tracerUpwindNew = (tc[t] * lt + S.dt * te[t]) * hProvInv;
-          const double tracerMaxNew = tracerUpwindNew + S.dt * fin[t] * hProvInv;
-          const double tracerMinNew = tracerUpwindNew + S.dt * fout[t] * hProvInv;
-          double scaleFactor =
-              (tmax[t] - tracerUpwindNew) / (tracerMaxNew - tracerUpwindNew + eps);
-          rf[t] = smin(1.0, smax(0.0, scaleFactor));
-          scaleFactor = (tracerUpwindNew - tmin[t]) / (tracerUpwindNew - tracerMinNew + eps);
-          rf[NT + t] = smin(1.0, smax(0.0, scaleFactor));
-          tl[t] = te[t];
+          const double tc = ld(S.xtr, ol * NT + t);
+          double tn[W], an[W];
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
+            an[j] = ld(S.anti, (e[j] * L + k) * NT + t);
+          }
+          // bounds over cellsOnCell (195-214); upwind tendency and
+          // antidiffusive in/out sums, file order (232-249)
+          double tmax = tc, tmin = tc, te = 0.0, fin = 0.0, fout = 0.0;
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const bool live = j < n;
+            tmax = live ? smax(tmax, tn[j]) : tmax;
+            tmin = live ? smin(tmin, tn[j]) : tmin;
+            const double signedFactor = sign_on_cell(neg, j) * invArea;
+            const double F = xu[j] * he[j];
+            const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
+            const bool first = (neg >> j) & 1;
+            const double t1 = first ? tc : tn[j];
+            const double t2 = first ? tn[j] : tc;
+            const double lo = dv[j] * (ntf1 * t1 + ntf2 * t2);
+            const double xe = te + signedFactor * lo;
+            const double xi = fin + smax(0.0, signedFactor * an[j]);
+            const double xo = fout + smin(0.0, signedFactor * an[j]);
+            te = live ? xe : te;
+            fin = live ? xi : fin;
+            fout = live ? xo : fout;
+          }
+          // limiter factors (253-266)
+          const double tracerUpwindNew = (tc * lt + S.dt * te) * hProvInv;
+          const double tracerMaxNew = tracerUpwindNew + S.dt * fin * hProvInv;
+          const double tracerMinNew = tracerUpwindNew + S.dt * fout * hProvInv;
+          double scaleFactor = (tmax - tracerUpwindNew) / (tracerMaxNew - tracerUpwindNew + eps);
+          st(S.rfac, ol * 2 * NT + t, smin(1.0, smax(0.0, scaleFactor)));
+          scaleFactor = (tracerUpwindNew - tmin) / (tracerUpwindNew - tracerMinNew + eps);
+          st(S.rfac, ol * 2 * NT + NT + t, smin(1.0, smax(0.0, scaleFactor)));
+          st(S.tlo, ol * NT + t, te);
         }
       }
     }
@@ -321,85 +374,73 @@ __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
 }
 
 // ---- cells: FCT rescale + accumulate, del2 mixing, tracer update ----------
-template <int NT>
+template <int NT, int W>
 __global__ __launch_bounds__(BS) void k_stage_tracer(DMesh M, StageIO S) {
-  const int c = gtid();
+  const int c = xcd_item();
   if (c >= M.nC) return;
-  const int L = M.nL, C = M.nC;
-  const int n = M.nEoC[c];
-  const uint8_t neg = M.sgnNeg[c];
-  const double ar = M.area[c];
-  const double invAreaCell1 = 1.0 / ar;
-  for (int k = 0; k < L; k++) {
-    const size_t ol = size_t(c) * L + k;
-    double te[NT], rs[2 * NT];
-    load_tr<NT>(S.tlo + ol * NT, te);
-    load_tr<2 * NT>(S.rfac + ol * 2 * NT, rs);
-    // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
-    for (int j = 0; j < n; j++) {
-      const int e = M.c2e[j * C + c];
-      const size_t oe = size_t(e) * L + k;
-      const int opp = M.cOpp[j * C + c];
-      double ro[2 * NT], an[NT];
-      load_tr<2 * NT>(S.rfac + (size_t(opp) * L + k) * 2 * NT, ro);
-      load_tr<NT>(S.anti + oe * NT, an);
-      const bool first = (neg >> j) & 1;
-      const double signedFactor = (first ? -1.0 : 1.0) * invAreaCell1;
+  const uint32_t L = M.nL, C = M.nC;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const double invAreaCell1 = 1.0 / ld(M.area, c);
+  uint32_t e[W], op[W];
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        // factors of m.cells(e)[0] and [1]: (in, out)
-        const double in1 = first ? rs[t] : ro[t], out1 = first ? rs[NT + t] : ro[NT + t];
-        const double in2 = first ? ro[t] : rs[t], out2 = first ? ro[NT + t] : rs[NT + t];
-        const double x = an[t];
-        const double hof = smax(0.0, x) * smin(out1, in2) + smin(0.0, x) * smin(in1, out2);
-        te[t] += signedFactor * hof;
-      }
-    }
-    double tc[NT];
-    load_tr<NT>(S.xtr + ol * NT, tc);
+  for (int j = 0; j < W; j++) {
+    e[j] = ld(M.c2e, j * C + c);
+    op[j] = ld(M.cOpp, j * C + c);
+  }
+  double rho[W];
+  if (S.use_hmix) {
+#pragma unroll
+    for (int j = 0; j < W; j++) rho[j] = M.msd * S.kappa * ld(M.dv, e[j]) / ld(M.dc, e[j]);
+  }
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t ol = c * L + k;
+    const double hy = ld(S.yh, ol);
+    double he[W];
     if (S.use_hmix) {
-      // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
-      for (int j = 0; j < n; j++) {
-        const int e = M.c2e[j * C + c];
-        const double rho_scaling = M.msd * S.kappa * M.dv[e] / M.dc[e];
-        const double lte = S.h_edge[size_t(e) * L + k];
-        double to[NT];
-        load_tr<NT>(S.xtr + (size_t(M.cOpp[j * C + c]) * L + k) * NT, to);
-        const bool first = (neg >> j) & 1;
-        const double sg = first ? -1.0 : 1.0;
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const double turb = first ? to[t] - tc[t] : tc[t] - to[t];
-          const double flux = lte * turb * rho_scaling;
-          te[t] -= sg * flux * invAreaCell1;
+      for (int j = 0; j < W; j++) he[j] = ld(S.h_edge, e[j] * L + k);
+    }
+#pragma unroll 1
+    for (int t = 0; t < NT; t++) {
+      double te = ld(S.tlo, ol * NT + t);
+      const double rin = ld(S.rfac, ol * 2 * NT + t), rout = ld(S.rfac, ol * 2 * NT + NT + t);
+      double an[W], oin[W], oout[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        an[j] = ld(S.anti, (e[j] * L + k) * NT + t);
+        const uint32_t oo = (op[j] * L + k) * 2 * NT;
+        oin[j] = ld(S.rfac, oo + t);
+        oout[j] = ld(S.rfac, oo + NT + t);
+      }
+      // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const bool first = (neg >> j) & 1;  // c == m.cells(e)[0]
+        const double signedFactor = (first ? -1.0 : 1.0) * invAreaCell1;
+        const double in1 = first ? rin : oin[j], out1 = first ? rout : oout[j];
+        const double in2 = first ? oin[j] : rin, out2 = first ? oout[j] : rout;
+        const double x = an[j];
+        const double hof = smax(0.0, x) * smin(out1, in2) + smin(0.0, x) * smin(in1, out2);
+        const double y = te + signedFactor * hof;
+        te = (j < n) ? y : te;
+      }
+      if (S.use_hmix) {
+        // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
+        const double tc = ld(S.xtr, ol * NT + t);
+        double tn[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const bool first = (neg >> j) & 1;
+          const double turb = first ? tn[j] - tc : tc - tn[j];
+          const double flux = he[j] * turb * rho[j];
+          const double y = te - (first ? -1.0 : 1.0) * flux * invAreaCell1;
+          te = (j < n) ? y : te;
         }
       }
-    }
-    // compute_substep / accumulate_update / finalize_timestep (tracers)
-    const double hy = S.yh[ol];
-    double *yt = S.ytr + ol * NT;
-    double *nt = S.ntr + ol * NT;
-    if (S.stage == 0) {
-      const double h0 = S.h0[ol];
-      double t0[NT];
-      load_tr<NT>(S.tr0 + ol * NT, t0);
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        yt[t] = (h0 * t0[t] + S.w_sub * te[t]) / hy;
-        nt[t] = t0[t] * h0 + S.w_rk * te[t];
-      }
-    } else if (S.stage < 3) {
-      const double h0 = S.h0[ol];
-      double t0[NT];
-      load_tr<NT>(S.tr0 + ol * NT, t0);
-#pragma unroll
-      for (int t = 0; t < NT; t++) {
-        yt[t] = (h0 * t0[t] + S.w_sub * te[t]) / hy;
-        nt[t] += S.w_rk * te[t];
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < NT; t++) yt[t] = (nt[t] + S.w_rk * te[t]) / hy;
+      tracer_update(S, ol, ol * NT + t, te, hy);
     }
   }
 }
@@ -407,36 +448,36 @@ __global__ __launch_bounds__(BS) void k_stage_tracer(DMesh M, StageIO S) {
 __global__ __launch_bounds__(BS) void k_diag_vertex(DMesh M, const double *__restrict__ h,
                                                     const double *__restrict__ u,
                                                     double *vorticity, double *pv_vertex) {
-  const int v = gtid();
+  const int v = xcd_item();
   if (v < M.nV) diag_vertex(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex);
 }
 
 __global__ __launch_bounds__(BS) void k_diag_edge(DMesh M, const double *__restrict__ h,
                                                   const double *__restrict__ pv_vertex,
                                                   double *h_edge, double *pv_edge) {
-  const int e = gtid();
+  const int e = xcd_item();
   if (e < M.nE) diag_edge(M, e, h, pv_vertex, h_edge, pv_edge);
 }
 
 // ke and divergence of a state (the cell part of the initial diagnostics)
 __global__ __launch_bounds__(BS) void k_diag_cell(DMesh M, const double *__restrict__ u,
                                                   double *ke, double *div, int use_visc) {
-  const int c = gtid();
+  const int c = xcd_item();
   if (c >= M.nC) return;
-  const int L = M.nL, C = M.nC;
-  const int n = M.nEoC[c];
-  const uint8_t neg = M.sgnNeg[c];
-  const double ar = M.area[c];
-  for (int k = 0; k < L; k++) {
+  const uint32_t L = M.nL, C = M.nC;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const double ar = ld(M.area, c);
+  for (uint32_t k = 0; k < L; k++) {
     double s = 0.0, d = 0.0;
     for (int j = 0; j < n; j++) {
-      const int e = M.c2e[j * C + c];
-      const double ue = u[size_t(e) * L + k];
-      s += 0.25 * M.dc[e] * M.dv[e] * (ue * ue);
-      if (use_visc) d += (-sign_on_cell(neg, j)) * M.dv[e] * ue;
+      const uint32_t e = ld(M.c2e, j * C + c);
+      const double ue = ld(u, e * L + k);
+      s += 0.25 * ld(M.dc, e) * ld(M.dv, e) * (ue * ue);
+      if (use_visc) d += (-sign_on_cell(neg, j)) * ld(M.dv, e) * ue;
     }
-    ke[size_t(c) * L + k] = s / ar;
-    if (use_visc) div[size_t(c) * L + k] = d / ar;
+    st(ke, c * L + k, s / ar);
+    if (use_visc) st(div, c * L + k, d / ar);
   }
 }
 
@@ -491,21 +532,23 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   return b;
 }
 
-template <int NT>
+template <int NT, int W, int WE>
 int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
                  const Bytes &b) {
   if (int rc = prof_begin(ctx, "k_stage_edges", b.edges)) return rc;
-  hipLaunchKernelGGL(k_stage_edges<NT>, dim3(blocks_for(M.nE, BS)), dim3(BS), 0, st, M, S);
+  hipLaunchKernelGGL((k_stage_edges<NT, WE, WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M,
+                     S);
   if (int rc = after_launch("k_stage_edges")) return rc;
   if (int rc = prof_end(ctx)) return rc;
   if (int rc = prof_begin(ctx, "k_stage_cells", b.cells)) return rc;
-  hipLaunchKernelGGL(k_stage_cells<NT>, dim3(blocks_for(M.nC, BS)), dim3(BS), 0, st, M, S);
+  hipLaunchKernelGGL((k_stage_cells<NT, W>), dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, S);
   if (int rc = after_launch("k_stage_cells")) return rc;
   if (int rc = prof_end(ctx)) return rc;
   if constexpr (NT > 0) {
     if (S.scheme == MPAS_TRACER_FCT) {
       if (int rc = prof_begin(ctx, "k_stage_tracer", b.tracer)) return rc;
-      hipLaunchKernelGGL(k_stage_tracer<NT>, dim3(blocks_for(M.nC, BS)), dim3(BS), 0, st, M, S);
+      hipLaunchKernelGGL((k_stage_tracer<NT, W>), dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M,
+                         S);
       if (int rc = after_launch("k_stage_tracer")) return rc;
       if (int rc = prof_end(ctx)) return rc;
     }
@@ -513,18 +556,28 @@ int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st
   return 0;
 }
 
+// Widths: cells of up to 6 edges with stencils of up to 10 (every
+// icosahedral SCVT), else up to 8 and 16 (the reference's maxEdges and
+// maxEdges2).
+template <int NT>
+int launch_stage_w(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
+                   const Bytes &b) {
+  if (M.narrow) return launch_stage<NT, 6, 10>(ctx, M, S, st, b);
+  return launch_stage<NT, 8, 16>(ctx, M, S, st, b);
+}
+
 int launch_stage_nt(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
                     const Bytes &b) {
   switch (M.nT) {
-    case 0: return launch_stage<0>(ctx, M, S, st, b);
-    case 1: return launch_stage<1>(ctx, M, S, st, b);
-    case 2: return launch_stage<2>(ctx, M, S, st, b);
-    case 3: return launch_stage<3>(ctx, M, S, st, b);
-    case 4: return launch_stage<4>(ctx, M, S, st, b);
-    case 5: return launch_stage<5>(ctx, M, S, st, b);
-    case 6: return launch_stage<6>(ctx, M, S, st, b);
-    case 7: return launch_stage<7>(ctx, M, S, st, b);
-    case 8: return launch_stage<8>(ctx, M, S, st, b);
+    case 0: return launch_stage_w<0>(ctx, M, S, st, b);
+    case 1: return launch_stage_w<1>(ctx, M, S, st, b);
+    case 2: return launch_stage_w<2>(ctx, M, S, st, b);
+    case 3: return launch_stage_w<3>(ctx, M, S, st, b);
+    case 4: return launch_stage_w<4>(ctx, M, S, st, b);
+    case 5: return launch_stage_w<5>(ctx, M, S, st, b);
+    case 6: return launch_stage_w<6>(ctx, M, S, st, b);
+    case 7: return launch_stage_w<7>(ctx, M, S, st, b);
+    case 8: return launch_stage_w<8>(ctx, M, S, st, b);
   }
   MPAS_FAIL(MPAS_EINVAL, "the fused step supports at most 8 tracers");
 }
@@ -538,13 +591,13 @@ int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot) {
   SwState &S = *ctx->sw;
   hipStream_t st = ctx->stream;
   const int use_visc = ctx->params.visc2 != 0.0;
-  hipLaunchKernelGGL(k_diag_cell, dim3(blocks_for(M.nC, BS)), dim3(BS), 0, st, M, u,
+  hipLaunchKernelGGL(k_diag_cell, dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, u,
                      S.ke[slot], S.div[slot], use_visc);
   if (int rc = after_launch("k_diag_cell")) return rc;
-  hipLaunchKernelGGL(k_diag_vertex, dim3(blocks_for(M.nV, BS)), dim3(BS), 0, st, M, h, u,
+  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, h, u,
                      S.vort[slot], S.pv_vertex);
   if (int rc = after_launch("k_diag_vertex")) return rc;
-  hipLaunchKernelGGL(k_diag_edge, dim3(blocks_for(M.nE, BS)), dim3(BS), 0, st, M, h,
+  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, h,
                      S.pv_vertex, S.h_edge[slot], S.pv_edge[slot]);
   return after_launch("k_diag_edge");
 }
@@ -601,12 +654,12 @@ int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
       const Bytes b = stage_bytes(M, S, sum_eoe, sum_adv, sum_eoc);
       if (int rc = launch_stage_nt(ctx, M, S, st, b)) return rc;
       if (int rc = prof_begin(ctx, "k_diag_vertex", b.vertex)) return rc;
-      hipLaunchKernelGGL(k_diag_vertex, dim3(blocks_for(M.nV, BS)), dim3(BS), 0, st, M, S.yh,
+      hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, S.yh,
                          S.yu, W.vort[nxt], W.pv_vertex);
       if (int rc = after_launch("k_diag_vertex")) return rc;
       if (int rc = prof_end(ctx)) return rc;
       if (int rc = prof_begin(ctx, "k_diag_edge", b.edge_diag)) return rc;
-      hipLaunchKernelGGL(k_diag_edge, dim3(blocks_for(M.nE, BS)), dim3(BS), 0, st, M, S.yh,
+      hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S.yh,
                          W.pv_vertex, W.h_edge[nxt], W.pv_edge[nxt]);
       if (int rc = after_launch("k_diag_edge")) return rc;
       if (int rc = prof_end(ctx)) return rc;

@@ -14,29 +14,29 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
                                             const double *__restrict__ u, double *h_vertex,
                                             double *circulation, double *vorticity,
                                             double *pv_vertex) {
-  const int L = M.nL, V = M.nV;
-  const int x0 = M.v2e[v], x1 = M.v2e[V + v], x2 = M.v2e[2 * V + v];
-  const int e0 = v2e_id(x0), e1 = v2e_id(x1), e2 = v2e_id(x2);
-  const int c0 = M.v2c[v], c1 = M.v2c[V + v], c2 = M.v2c[2 * V + v];
-  const double k0 = M.kite[v], k1 = M.kite[V + v], k2 = M.kite[2 * V + v];
-  const double d0 = M.dc[e0], d1 = M.dc[e1], d2 = M.dc[e2];
-  const double at = M.areaTri[v], f = M.fV[v];
-  for (int k = 0; k < L; k++) {
+  const uint32_t L = M.nL, V = M.nV;
+  const int x0 = ld(M.v2e, v), x1 = ld(M.v2e, V + v), x2 = ld(M.v2e, 2 * V + v);
+  const uint32_t e0 = v2e_id(x0), e1 = v2e_id(x1), e2 = v2e_id(x2);
+  const uint32_t c0 = ld(M.v2c, v), c1 = ld(M.v2c, V + v), c2 = ld(M.v2c, 2 * V + v);
+  const double k0 = ld(M.kite, v), k1 = ld(M.kite, V + v), k2 = ld(M.kite, 2 * V + v);
+  const double d0 = ld(M.dc, e0), d1 = ld(M.dc, e1), d2 = ld(M.dc, e2);
+  const double at = ld(M.areaTri, v), f = ld(M.fV, v);
+  for (uint32_t k = 0; k < L; k++) {
     double circ = 0.0;
-    circ = circ + v2e_sign(x0) * (d0 * u[size_t(e0) * L + k]);
-    circ = circ + v2e_sign(x1) * (d1 * u[size_t(e1) * L + k]);
-    circ = circ + v2e_sign(x2) * (d2 * u[size_t(e2) * L + k]);
+    circ = circ + v2e_sign(x0) * (d0 * ld(u, e0 * L + k));
+    circ = circ + v2e_sign(x1) * (d1 * ld(u, e1 * L + k));
+    circ = circ + v2e_sign(x2) * (d2 * ld(u, e2 * L + k));
     const double vort = circ / at;
     double hv = 0.0;
-    hv += h[size_t(c0) * L + k] * k0;
-    hv += h[size_t(c1) * L + k] * k1;
-    hv += h[size_t(c2) * L + k] * k2;
+    hv += ld(h, c0 * L + k) * k0;
+    hv += ld(h, c1 * L + k) * k1;
+    hv += ld(h, c2 * L + k) * k2;
     hv /= at;
-    const size_t o = size_t(v) * L + k;
-    if (circulation) circulation[o] = circ;
-    vorticity[o] = vort;
-    if (h_vertex) h_vertex[o] = hv;
-    pv_vertex[o] = (f + vort) / hv;
+    const uint32_t o = v * L + k;
+    if (circulation) st(circulation, o, circ);
+    st(vorticity, o, vort);
+    if (h_vertex) st(h_vertex, o, hv);
+    st(pv_vertex, o, (f + vort) / hv);
   }
 }
 
@@ -45,17 +45,17 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
 __device__ __forceinline__ void diag_edge(const DMesh &M, int e, const double *__restrict__ h,
                                           const double *__restrict__ pv_vertex,
                                           double *h_edge, double *pv_edge) {
-  const int L = M.nL;
-  const int2 cc = M.e2c[e];
-  const int2 vv = M.e2v[e];
-  const int va = min(vv.x, vv.y), vb = max(vv.x, vv.y);
-  for (int k = 0; k < L; k++) {
-    const size_t o = size_t(e) * L + k;
-    h_edge[o] = 0.5 * (h[size_t(cc.x) * L + k] + h[size_t(cc.y) * L + k]);
+  const uint32_t L = M.nL;
+  const int2 cc = ld(M.e2c, e);
+  const int2 vv = ld(M.e2v, e);
+  const uint32_t va = min(vv.x, vv.y), vb = max(vv.x, vv.y);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    st(h_edge, o, 0.5 * (ld(h, uint32_t(cc.x) * L + k) + ld(h, uint32_t(cc.y) * L + k)));
     double p = 0.0;
-    p += 0.5 * pv_vertex[size_t(va) * L + k];
-    p += 0.5 * pv_vertex[size_t(vb) * L + k];
-    pv_edge[o] = p;
+    p += 0.5 * ld(pv_vertex, va * L + k);
+    p += 0.5 * ld(pv_vertex, vb * L + k);
+    st(pv_edge, o, p);
   }
 }
 
@@ -67,20 +67,19 @@ __device__ __forceinline__ double tend_u_at(const DMesh &M, int e, int k, int2 c
                                             const double *__restrict__ ke,
                                             const double *__restrict__ h,
                                             const double *__restrict__ h_s) {
-  const int L = M.nL;
-  const double pve = pv_edge[size_t(e) * L + k];
-  const int n = M.nEoE[e];
+  const uint32_t L = M.nL, E = M.nE;
+  const double pve = ld(pv_edge, e * L + k);
+  const int n = ld(M.nEoE, e);
   double q = 0;
   for (int j = 0; j < n; j++) {
-    const int eoe = M.eoe[size_t(j) * M.nE + e];
-    const double workpv = 0.5 * (pve + pv_edge[size_t(eoe) * L + k]);
-    q = q + M.woe[size_t(j) * M.nE + e] * u[size_t(eoe) * L + k] * workpv *
-                h_edge[size_t(eoe) * L + k];
+    const uint32_t eoe = ld(M.eoe, j * E + e);
+    const double workpv = 0.5 * (pve + ld(pv_edge, eoe * L + k));
+    q = q + ld(M.woe, j * E + e) * ld(u, eoe * L + k) * workpv * ld(h_edge, eoe * L + k);
   }
-  return q - (ke[size_t(cc.y) * L + k] - ke[size_t(cc.x) * L + k] +
-              M.gravity * (h[size_t(cc.y) * L + k] + h_s[cc.y] - h[size_t(cc.x) * L + k] -
-                           h_s[cc.x])) /
-                 M.dc[e];
+  const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
+  return q - (ld(ke, o2) - ld(ke, o1) +
+              M.gravity * (ld(h, o2) + ld(h_s, cc.y) - ld(h, o1) - ld(h_s, cc.x))) /
+                 ld(M.dc, e);
 }
 
 // del2 viscosity on u (no reference counterpart):
@@ -88,9 +87,11 @@ __device__ __forceinline__ double tend_u_at(const DMesh &M, int e, int k, int2 c
 __device__ __forceinline__ double visc_at(const DMesh &M, int e, int k, int2 cc, int2 vv,
                                           const double *__restrict__ div,
                                           const double *__restrict__ vort, double coef) {
-  const int L = M.nL;
-  const double ud = (div[size_t(cc.y) * L + k] - div[size_t(cc.x) * L + k]) / M.dc[e] -
-                    (vort[size_t(vv.y) * L + k] - vort[size_t(vv.x) * L + k]) / M.dv[e];
+  const uint32_t L = M.nL;
+  const double ud = (ld(div, uint32_t(cc.y) * L + k) - ld(div, uint32_t(cc.x) * L + k)) /
+                        ld(M.dc, e) -
+                    (ld(vort, uint32_t(vv.y) * L + k) - ld(vort, uint32_t(vv.x) * L + k)) /
+                        ld(M.dv, e);
   return coef * ud;
 }
 

@@ -4,6 +4,7 @@
 // ascending order, so every sum is formed operation for operation as the
 // reference forms it and no atomics are needed.  Every expression keeps the
 // reference's association; the file is compiled with -ffp-contract=off.
+#include <algorithm>
 #include <cmath>
 
 #include "ctx.h"
@@ -455,6 +456,14 @@ __global__ __launch_bounds__(BS) void k_step_tracers(int n, const double *tend, 
   trPrev[i] = tr[i] + ws * tend[i];
 }
 
+// plain streaming copy, 16 bytes per lane per iteration
+__global__ __launch_bounds__(BS) void k_copy(int64_t n16, const double2 *__restrict__ src,
+                                             double2 *__restrict__ dst) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(BS) void k_fill(int64_t n, double *p, double v) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -796,6 +805,18 @@ int mpas_fct_step_tracers_forward(mpas_ctx *ctx, const double *tend, const doubl
   NONNULL(tend); NONNULL(tracers); NONNULL(tracersNew); NONNULL(tracersPrev);
   LAUNCH(k_step_tracers, n, n, tend, tracers, tracersNew, tracersPrev, rk_weight, rk_subweight);
   return 0;
+}
+
+int mpas_buf_copy(mpas_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  CTX_CHECK(ctx);
+  NONNULL(dst); NONNULL(src);
+  if (bytes % 16 || (uintptr_t(dst) | uintptr_t(src)) % 16)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_buf_copy: size and addresses must be multiples of 16");
+  const int64_t n16 = int64_t(bytes / 16);
+  const int grid = int(std::min<int64_t>(blocks_for(n16, BS), 256 * 32));
+  if (n16) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(BS), 0, ctx->stream, n16,
+                              static_cast<const double2 *>(src), static_cast<double2 *>(dst));
+  return after_launch("k_copy");
 }
 
 int mpas_fct_update_tracers(mpas_ctx *ctx, const double *tracersNew, double *tracers) {

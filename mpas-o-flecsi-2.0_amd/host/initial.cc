@@ -17,7 +17,7 @@
 #include <cstdint>
 #include <vector>
 
-#include "meshgen.h"
+#include "initial.h"
 
 namespace {
 

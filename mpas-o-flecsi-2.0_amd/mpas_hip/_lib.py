@@ -89,6 +89,7 @@ SIGNATURES = {
     "mpas_buf_free": (c_int, [c_void_p, c_void_p]),
     "mpas_buf_upload": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "mpas_buf_download": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "mpas_buf_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "mpas_host_mesh_create": (c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                       ctypes.POINTER(c_void_p)]),
     "mpas_host_mesh_destroy": (c_int, [c_void_p]),

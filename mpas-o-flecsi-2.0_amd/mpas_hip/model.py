@@ -319,7 +319,7 @@ class Context:
                    "vorticity": "v", "pv_vertex": "v", "h_s": "c1", "divergence": "c"}
 
     def sw_field(self, name):
-        kind = self.FIELD_SIZES[name]
+        kind = self.FIELD_SIZES.get(name, "c")  # unknown names are refused by the library
         n = {"c": self.nCells * self.nLevels, "e": self.nEdges * self.nLevels,
              "v": self.nVertices * self.nLevels, "c1": self.nCells,
              "t": self.nCells * self.nLevels * self.nTracers}[kind]
@@ -337,6 +337,11 @@ class Context:
         buf = ctypes.create_string_buffer(1 << 16)
         L.check(L.hip().mpas_prof_report(self._h, buf, len(buf)), "mpas_prof_report")
         return json.loads(buf.value.decode())
+
+    def copy_probe(self, dst, src, nbytes):
+        """Plain streaming device copy (bandwidth / counter calibration)."""
+        L.check(L.hip().mpas_buf_copy(self._h, _ptr(dst), _ptr(src), int(nbytes)),
+                "mpas_buf_copy")
 
     def rk4_step(self, dt, nsteps=1):
         L.check(L.hip().mpas_rk4_step(self._h, float(dt), int(nsteps)), "mpas_rk4_step")

@@ -39,5 +39,7 @@ def bitwise_equal(a, b):
 def max_rel(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
+    if a.size == 0 and b.size == 0:
+        return 0.0
     scale = np.maximum(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale) if a.size else 0.0

@@ -227,6 +227,16 @@ def test_fused_step_matches_oracle(name):
         assert bitwise_equal(g, r), (k, max_rel(g, r))
 
 
+def test_fused_wide_kernels_bitwise(monkeypatch):
+    """The 8 / 16-wide stage kernels (cells with more than 6 edges) give the
+    same bits as the oracle on the same workloads."""
+    monkeypatch.setenv("MPAS_HIP_WIDE_KERNELS", "1")
+    for name in ("sw_fct4", "sw_centred2"):
+        got, ref, diag = _run_pair(name, steps=2, seed=7)
+        for g, r in zip(got, ref):
+            assert bitwise_equal(g, r), name
+
+
 def test_fused_one_step_each_config_bitwise():
     for name in FUSED_CASES:
         got, ref, _ = _run_pair(name, steps=1, seed=5)
