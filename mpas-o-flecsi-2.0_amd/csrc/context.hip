@@ -523,22 +523,28 @@ static int prof_event(mpas_ctx *ctx, hipEvent_t *ev) {
   return 0;
 }
 
-int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes) {
+int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st) {
   Prof &P = ctx->prof;
   if (!P.on) return 0;
-  Prof::Pending p{P.id(kernel), nullptr, nullptr, bytes};
+  if (!st) st = ctx->stream;
+  Prof::Pending p{P.id(kernel), nullptr, nullptr, bytes, st};
   if (int rc = prof_event(ctx, &p.start)) return rc;
   if (int rc = prof_event(ctx, &p.stop)) return rc;
-  MPAS_HIP(hipEventRecord(p.start, ctx->stream));
+  MPAS_HIP(hipEventRecord(p.start, st));
   P.pending.push_back(p);
   return 0;
 }
 
-int prof_end(mpas_ctx *ctx) {
+int prof_end(mpas_ctx *ctx, hipStream_t st) {
   Prof &P = ctx->prof;
   if (!P.on) return 0;
-  MPAS_HIP(hipEventRecord(P.pending.back().stop, ctx->stream));
-  return 0;
+  if (!st) st = ctx->stream;
+  for (auto it = P.pending.rbegin(); it != P.pending.rend(); ++it)
+    if (it->stream == st) {
+      MPAS_HIP(hipEventRecord(it->stop, st));
+      return 0;
+    }
+  MPAS_FAIL(MPAS_EINVAL, "prof_end without prof_begin");
 }
 
 }  // namespace mpasx

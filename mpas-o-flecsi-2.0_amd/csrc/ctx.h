@@ -45,7 +45,7 @@ struct SwState {
   double *pv_vertex = nullptr, *h_s = nullptr;
   // FCT intermediates
   double *anti = nullptr;   // [E][nL][nT] antidiffusive flux
-  double *rfac = nullptr;   // [C][nL][2][nT] limiter factors (in, out)
+  double *rfac = nullptr;   // [C][nL][nT][2] limiter factors (in, out)
   double *tlo = nullptr;    // [C][nL][nT] upwind tendency
   int cur = 0;              // which ping-pong slot holds the current state
 };
@@ -57,6 +57,7 @@ struct Prof {
     int kernel;
     hipEvent_t start, stop;
     double bytes;
+    hipStream_t stream;
   };
   std::vector<std::string> names;
   std::vector<Pending> pending;
@@ -98,8 +99,9 @@ int check_ctx(const mpas_ctx *ctx);
 // Kernel launch epilogue: reports launch errors as MPAS_EHIP.
 int after_launch(const char *what);
 
-// Event bracket of one launch while profiling is on (no-ops otherwise).
-int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes);
-int prof_end(mpas_ctx *ctx);
+// Event bracket of one launch on stream st while profiling is on (no-ops
+// otherwise; st = null means the context's stream).
+int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st = nullptr);
+int prof_end(mpas_ctx *ctx, hipStream_t st = nullptr);
 
 }  // namespace mpasx

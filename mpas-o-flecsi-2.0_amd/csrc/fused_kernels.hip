@@ -49,6 +49,36 @@ __device__ __forceinline__ void load_tr(const double *__restrict__ p, uint32_t i
   for (int t = 0; t < NT; t++) v[t] = ld(p, i0 + t);
 }
 
+// TPL consecutive tracers of one entity: one 16-byte load / store per pair
+// (the offset i is even whenever TPL == 2: NT is even then).
+template <int TPL>
+__device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t i,
+                                         double (&v)[TPL]) {
+  if constexpr (TPL == 2) {
+    const double2 x = ld(reinterpret_cast<const double2 *>(p), i / 2);
+    v[0] = x.x;
+    v[1] = x.y;
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) v[g] = ld(p, i + g);
+  }
+}
+template <int TPL>
+__device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v)[TPL]) {
+  if constexpr (TPL == 2) {
+    st(reinterpret_cast<double2 *>(p), i / 2, make_double2(v[0], v[1]));
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) st(p, i + g, v[g]);
+  }
+}
+
+// Tracers handled by one lane: pairs when the count is even.
+template <int NT>
+constexpr int tracers_per_lane() {
+  return NT % 2 == 0 ? 2 : 1;
+}
+
 // Adds the live slots' values in ascending edge id, the order of the
 // reference's scatter loops: at step r the slot of rank r is selected with
 // compile-time indices only (a run-time index into a register array would
@@ -79,122 +109,131 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 // for two memory round trips instead of two per slot.  Slots past a row's
 // count read the ELL padding (valid ids) and are masked out of every sum,
 // which still runs over the live slots in the reference's order.
+//
+// Tracer work runs one lane per (entity, tracer): lane i handles tracer
+// i % NT of entity i / NT, so the lanes of an entity read and write its
+// tracer record as one contiguous run, and each lane keeps one tracer's
+// slot arrays in registers.
 
-// ---- edges ------------------------------------------------------------------
-template <int NT, int WE, int WA>
+// ---- edges: momentum ---------------------------------------------------------
+// compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
+// last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
+// 359-366)
+template <int WE>
 __global__ __launch_bounds__(BS) void k_stage_edges(DMesh M, StageIO S) {
   const int e = xcd_item();
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
-  const int2 cc = ld(M.e2c, e);
-
-  // compute_tend (+ viscosity), compute_substep, accumulate_update and, at
-  // the last stage, finalize_timestep for u (solve.cc:215-233, 285-289,
-  // 326-330, 359-366)
   if (S.freeze_u) {
     if (S.stage < 3)
       for (uint32_t k = 0; k < L; k++) st(S.yu, e * L + k, ld(S.u0, e * L + k));
-  } else {
-    const int n = ld(M.nEoE, e);
-    uint32_t id[WE];
-    double w[WE];
+    return;
+  }
+  const int2 cc = ld(M.e2c, e);
+  const int n = ld(M.nEoE, e);
+  uint32_t id[WE];
+  double w[WE];
+#pragma unroll
+  for (int j = 0; j < WE; j++) {
+    id[j] = ld(M.eoe, j * E + e);
+    w[j] = ld(M.woe, j * E + e);
+  }
+  const double dc = ld(M.dc, e);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    double uu[WE], pv[WE], he[WE];
 #pragma unroll
     for (int j = 0; j < WE; j++) {
-      id[j] = ld(M.eoe, j * E + e);
-      w[j] = ld(M.woe, j * E + e);
+      const uint32_t oj = id[j] * L + k;
+      uu[j] = ld(S.xu, oj);
+      pv[j] = ld(S.pv_edge, oj);
+      he[j] = ld(S.h_edge, oj);
     }
-    const double dc = ld(M.dc, e);
-    for (uint32_t k = 0; k < L; k++) {
-      const uint32_t o = e * L + k;
-      double uu[WE], pv[WE], he[WE];
+    const double pve = ld(S.pv_edge, o);
+    double q = 0;
 #pragma unroll
-      for (int j = 0; j < WE; j++) {
-        const uint32_t oj = id[j] * L + k;
-        uu[j] = ld(S.xu, oj);
-        pv[j] = ld(S.pv_edge, oj);
-        he[j] = ld(S.h_edge, oj);
-      }
-      const double pve = ld(S.pv_edge, o);
-      double q = 0;
-#pragma unroll
-      for (int j = 0; j < WE; j++) {
-        const double workpv = 0.5 * (pve + pv[j]);
-        const double qn = q + w[j] * uu[j] * workpv * he[j];
-        q = (j < n) ? qn : q;
-      }
-      const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
-      double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
-                      M.gravity * (ld(S.xh, o2) + ld(S.h_s, cc.y) - ld(S.xh, o1) -
-                                   ld(S.h_s, cc.x))) /
-                         dc;
-      if (S.use_visc) t += visc_at(M, e, k, cc, ld(M.e2v, e), S.div, S.vort, S.visc_coef);
-      if (S.stage == 0) {
-        const double u0 = ld(S.u0, o);
-        st(S.yu, o, u0 + S.w_sub * t);
-        st(S.nu, o, u0 + S.w_rk * t);
-      } else if (S.stage < 3) {
-        st(S.yu, o, ld(S.u0, o) + S.w_sub * t);
-        st(S.nu, o, ld(S.nu, o) + S.w_rk * t);
-      } else {
-        st(S.yu, o, ld(S.nu, o) + S.w_rk * t);
-      }
+    for (int j = 0; j < WE; j++) {
+      const double workpv = 0.5 * (pve + pv[j]);
+      const double qn = q + w[j] * uu[j] * workpv * he[j];
+      q = (j < n) ? qn : q;
+    }
+    const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
+    double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
+                    M.gravity * (ld(S.xh, o2) + ld(S.h_s, cc.y) - ld(S.xh, o1) -
+                                 ld(S.h_s, cc.x))) /
+                       dc;
+    if (S.use_visc) t += visc_at(M, e, k, cc, ld(M.e2v, e), S.div, S.vort, S.visc_coef);
+    if (S.stage == 0) {
+      const double u0 = ld(S.u0, o);
+      st(S.yu, o, u0 + S.w_sub * t);
+      st(S.nu, o, u0 + S.w_rk * t);
+    } else if (S.stage < 3) {
+      st(S.yu, o, ld(S.u0, o) + S.w_sub * t);
+      st(S.nu, o, ld(S.nu, o) + S.w_rk * t);
+    } else {
+      st(S.yu, o, ld(S.nu, o) + S.w_rk * t);
     }
   }
+}
 
-  if constexpr (NT > 0) {
-    if (S.scheme != MPAS_TRACER_FCT) return;
-    // calc_normal_thickness_flux, high-order flux and its correction by the
-    // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229)
-    const double dv = ld(M.dv, e);
-    const int na = ld(M.nAdv, e);
-    uint32_t ic[WA];
-    double c1[WA], c3[WA];
+// ---- edges x tracers: FCT high-order minus low-order flux -------------------
+// calc_normal_thickness_flux, the high-order flux and its correction by the
+// low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
+// lane.
+template <int NT, int WA>
+__global__ __launch_bounds__(BS) void k_stage_ho(DMesh M, StageIO S) {
+  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  const int item = xcd_item();
+  const int e = item / NPL, t0 = (item % NPL) * TPL;
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE;
+  const int2 cc = ld(M.e2c, e);
+  const double dv = ld(M.dv, e);
+  const int na = ld(M.nAdv, e);
+  uint32_t ic[WA];
+  double c1[WA], c3[WA];
+#pragma unroll
+  for (int i = 0; i < WA; i++) {
+    ic[i] = ld(M.adv, i * E + e);
+    c1[i] = ld(M.advCoef, i * E + e);
+    c3[i] = ld(M.advCoef3, i * E + e) * M.coef3rd;
+  }
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    const double F = ld(S.xu, o) * ld(S.h_edge, o);
+    const int mask = M.hoMask ? ld(M.hoMask, o) : 1;
+    const double wgt = F * double(mask);
+    const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
+    double tv[WA][TPL];
+#pragma unroll
+    for (int i = 0; i < WA; i++) ld_group<TPL>(S.xtr, (ic[i] * L + k) * NT + t0, tv[i]);
+    double ho[TPL];
+#pragma unroll
+    for (int g = 0; g < TPL; g++) ho[g] = 0.0;
 #pragma unroll
     for (int i = 0; i < WA; i++) {
-      ic[i] = ld(M.adv, i * E + e);
-      c1[i] = ld(M.advCoef, i * E + e);
-      c3[i] = ld(M.advCoef3, i * E + e) * M.coef3rd;
-    }
-    for (uint32_t k = 0; k < L; k++) {
-      const uint32_t o = e * L + k;
-      const double F = ld(S.xu, o) * ld(S.h_edge, o);
-      const int mask = M.hoMask ? ld(M.hoMask, o) : 1;
-      const double wgt = F * double(mask);
-      const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
-      double ho[NT];
+      const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
+      const bool live = i < na && int(k) < lvl_cell(M, ic[i]);
 #pragma unroll
-      for (int t = 0; t < NT; t++) ho[t] = 0.0;
-#pragma unroll
-      for (int i = 0; i < WA; i++) {
-        const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
-        double tv[NT];
-        load_tr<NT>(S.xtr, (ic[i] * L + k) * NT, tv);
-        const bool live = i < na && int(k) < lvl_cell(M, ic[i]);
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const double x = ho[t] + tv[t] * weightFactor;
-          ho[t] = live ? x : ho[t];
-        }
-      }
-      if (int(k) < lvl_edge(M, e)) {
-        double t1[NT], t2[NT];
-        load_tr<NT>(S.xtr, (uint32_t(cc.x) * L + k) * NT, t1);
-        load_tr<NT>(S.xtr, (uint32_t(cc.y) * L + k) * NT, t2);
-        const double tracerWeight = double((mask + 1) & 1) * (dv * 0.5) * F;
-        const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          const double lo = dv * (ntf1 * t1[t] + ntf2 * t2[t]);
-          double x = ho[t];
-          x += tracerWeight * (t1[t] + t2[t]);
-          x -= lo;
-          st(S.anti, o * NT + t, x);
-        }
-      } else {
-#pragma unroll
-        for (int t = 0; t < NT; t++) st(S.anti, o * NT + t, ho[t]);
+      for (int g = 0; g < TPL; g++) {
+        const double x = ho[g] + tv[i][g] * weightFactor;
+        ho[g] = live ? x : ho[g];
       }
     }
+    if (int(k) < lvl_edge(M, e)) {
+      double t1[TPL], t2[TPL];
+      ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k) * NT + t0, t1);
+      ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k) * NT + t0, t2);
+      const double tracerWeight = double((mask + 1) & 1) * (dv * 0.5) * F;
+      const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        const double lo = dv * (ntf1 * t1[g] + ntf2 * t2[g]);
+        ho[g] += tracerWeight * (t1[g] + t2[g]);
+        ho[g] -= lo;
+      }
+    }
+    st_group<TPL>(S.anti, o * NT + t0, ho);
   }
 }
 
@@ -214,30 +253,35 @@ __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uin
   }
 }
 
-// ---- cells ------------------------------------------------------------------
+// ---- cells (x tracers) -------------------------------------------------------
+// Thickness tendency and update, ke and divergence of the new state (the
+// cell's first lane stores them), and per tracer either the centred
+// tendency with its update or the FCT bounds, upwind tendency and limiter
+// factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
 template <int NT, int W>
 __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
-  const int c = xcd_item();
+  constexpr int TPL = NT > 0 ? tracers_per_lane<NT>() : 1;
+  constexpr int NPL = NT > 0 ? NT / TPL : 1;
+  const int item = xcd_item();
+  const int c = item / NPL, t0 = (item % NPL) * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
   const int n = ld(M.nEoC, c);
   const uint8_t neg = ld(M.sgnNeg, c);
   const uint32_t rank = ld(M.ascRank, c);
   const double ar = ld(M.area, c);
-  uint32_t e[W], op[W];
-#pragma unroll
-  for (int j = 0; j < W; j++) {
-    e[j] = ld(M.c2e, j * C + c);
-    op[j] = ld(M.cOpp, j * C + c);
-  }
+  const bool lead = t0 == 0;
+  uint32_t e[W];
   double dv[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) e[j] = ld(M.c2e, j * C + c);
 #pragma unroll
   for (int j = 0; j < W; j++) dv[j] = ld(M.dv, e[j]);
 
   for (uint32_t k = 0; k < L; k++) {
     const uint32_t ol = c * L + k;
-    // ke (solve.cc:63-77) and divergence of the new state
-    {
+    if (lead) {
+      // ke (solve.cc:63-77) and divergence of the new state
       double yu[W], dc[W];
 #pragma unroll
       for (int j = 0; j < W; j++) {
@@ -274,30 +318,39 @@ __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
     if (S.stage == 0) {
       const double h0 = ld(S.h0, ol);
       hy = h0 + S.w_sub * th;
-      st(S.nh, ol, h0 + S.w_rk * th);
+      if (lead) st(S.nh, ol, h0 + S.w_rk * th);
     } else if (S.stage < 3) {
       hy = ld(S.h0, ol) + S.w_sub * th;
-      st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
+      if (lead) st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
     } else {
       hy = ld(S.nh, ol) + S.w_rk * th;
     }
-    st(S.yh, ol, hy);
+    if (lead) st(S.yh, ol, hy);
 
     if constexpr (NT > 0) {
       const double invArea = 1.0 / ar;
+      const uint32_t ot = ol * NT + t0;
+      double tc[TPL];
+      ld_group<TPL>(S.xtr, ot, tc);
+      double tn[W][TPL];
+      {
+        uint32_t op[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) op[j] = ld(M.cOpp, j * C + c);
+#pragma unroll
+        for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+      }
       if (S.scheme == MPAS_TRACER_CENTRED) {
         // compute_scalar_tend (solve.cc:237-270), ascending edge id, then
-        // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order; one
-        // tracer at a time keeps the slot arrays in registers
-#pragma unroll 1
-        for (int t = 0; t < NT; t++) {
-          const double tc = ld(S.xtr, ol * NT + t);
-          double tn[W], term[W];
+        // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          double term[W];
 #pragma unroll
           for (int j = 0; j < W; j++) {
-            tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
             const bool first = (neg >> j) & 1;
-            const double tracer_edge = first ? 0.5 * (tc + tn[j]) : 0.5 * (tn[j] + tc);
+            const double tracer_edge =
+                first ? 0.5 * (tc[g] + tn[j][g]) : 0.5 * (tn[j][g] + tc[g]);
             term[j] = (xu[j] * dv[j] * he[j] * tracer_edge) / ar;
           }
           double te = sum_ascending<W>(0.0, term, rank, neg, n);
@@ -306,13 +359,13 @@ __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
             for (int j = 0; j < W; j++) {
               const bool first = (neg >> j) & 1;
               const double rho_scaling = M.msd * S.kappa * dv[j] / ld(M.dc, e[j]);
-              const double turb = first ? tn[j] - tc : tc - tn[j];
+              const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
               const double flux = he[j] * turb * rho_scaling;
               const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
               te = (j < n) ? x : te;
             }
           }
-          tracer_update(S, ol, ol * NT + t, te, hy);
+          tracer_update(S, ol, ot + g, te, hy);
         }
       } else if (S.scheme == MPAS_TRACER_FCT) {
         // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
@@ -326,57 +379,63 @@ __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
           hp = (j < n) ? x : hp;
         }
         const double hProvInv = 1.0 / hp;
-        const double eps = 1.e-10;
-#pragma unroll 1
-        for (int t = 0; t < NT; t++) {
-          const double tc = ld(S.xtr, ol * NT + t);
-          double tn[W], an[W];
+        double an[W][TPL];
 #pragma unroll
-          for (int j = 0; j < W; j++) {
-            tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
-            an[j] = ld(S.anti, (e[j] * L + k) * NT + t);
-          }
+        for (int j = 0; j < W; j++) ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
+        double rf[2 * TPL], tl[TPL];
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
           // bounds over cellsOnCell (195-214); upwind tendency and
           // antidiffusive in/out sums, file order (232-249)
-          double tmax = tc, tmin = tc, te = 0.0, fin = 0.0, fout = 0.0;
+          double tmax = tc[g], tmin = tc[g], te = 0.0, fin = 0.0, fout = 0.0;
 #pragma unroll
           for (int j = 0; j < W; j++) {
             const bool live = j < n;
-            tmax = live ? smax(tmax, tn[j]) : tmax;
-            tmin = live ? smin(tmin, tn[j]) : tmin;
+            tmax = live ? smax(tmax, tn[j][g]) : tmax;
+            tmin = live ? smin(tmin, tn[j][g]) : tmin;
             const double signedFactor = sign_on_cell(neg, j) * invArea;
             const double F = xu[j] * he[j];
             const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
             const bool first = (neg >> j) & 1;
-            const double t1 = first ? tc : tn[j];
-            const double t2 = first ? tn[j] : tc;
+            const double t1 = first ? tc[g] : tn[j][g];
+            const double t2 = first ? tn[j][g] : tc[g];
             const double lo = dv[j] * (ntf1 * t1 + ntf2 * t2);
             const double xe = te + signedFactor * lo;
-            const double xi = fin + smax(0.0, signedFactor * an[j]);
-            const double xo = fout + smin(0.0, signedFactor * an[j]);
+            const double xi = fin + smax(0.0, signedFactor * an[j][g]);
+            const double xo = fout + smin(0.0, signedFactor * an[j][g]);
             te = live ? xe : te;
             fin = live ? xi : fin;
             fout = live ? xo : fout;
           }
-          // limiter factors (253-266)
-          const double tracerUpwindNew = (tc * lt + S.dt * te) * hProvInv;
+          // limiter factors (253-266), stored (in, out) per tracer
+          const double eps = 1.e-10;
+          const double tracerUpwindNew = (tc[g] * lt + S.dt * te) * hProvInv;
           const double tracerMaxNew = tracerUpwindNew + S.dt * fin * hProvInv;
           const double tracerMinNew = tracerUpwindNew + S.dt * fout * hProvInv;
-          double scaleFactor = (tmax - tracerUpwindNew) / (tracerMaxNew - tracerUpwindNew + eps);
-          st(S.rfac, ol * 2 * NT + t, smin(1.0, smax(0.0, scaleFactor)));
+          double scaleFactor =
+              (tmax - tracerUpwindNew) / (tracerMaxNew - tracerUpwindNew + eps);
+          rf[2 * g] = smin(1.0, smax(0.0, scaleFactor));
           scaleFactor = (tracerUpwindNew - tmin) / (tracerUpwindNew - tracerMinNew + eps);
-          st(S.rfac, ol * 2 * NT + NT + t, smin(1.0, smax(0.0, scaleFactor)));
-          st(S.tlo, ol * NT + t, te);
+          rf[2 * g + 1] = smin(1.0, smax(0.0, scaleFactor));
+          tl[g] = te;
         }
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          const double pair[2] = {rf[2 * g], rf[2 * g + 1]};
+          st_group<2>(S.rfac, (ot + g) * 2, pair);
+        }
+        st_group<TPL>(S.tlo, ot, tl);
       }
     }
   }
 }
 
-// ---- cells: FCT rescale + accumulate, del2 mixing, tracer update ----------
+// ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
 template <int NT, int W>
 __global__ __launch_bounds__(BS) void k_stage_tracer(DMesh M, StageIO S) {
-  const int c = xcd_item();
+  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  const int item = xcd_item();
+  const int c = item / NPL, t0 = (item % NPL) * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
   const int n = ld(M.nEoC, c);
@@ -388,60 +447,72 @@ __global__ __launch_bounds__(BS) void k_stage_tracer(DMesh M, StageIO S) {
     e[j] = ld(M.c2e, j * C + c);
     op[j] = ld(M.cOpp, j * C + c);
   }
-  double rho[W];
-  if (S.use_hmix) {
-#pragma unroll
-    for (int j = 0; j < W; j++) rho[j] = M.msd * S.kappa * ld(M.dv, e[j]) / ld(M.dc, e[j]);
-  }
   for (uint32_t k = 0; k < L; k++) {
-    const uint32_t ol = c * L + k;
-    const double hy = ld(S.yh, ol);
-    double he[W];
-    if (S.use_hmix) {
+    const uint32_t ol = c * L + k, ot = ol * NT + t0;
+    double te[TPL], rs[2 * TPL];
+    ld_group<TPL>(S.tlo, ot, te);
 #pragma unroll
-      for (int j = 0; j < W; j++) he[j] = ld(S.h_edge, e[j] * L + k);
+    for (int g = 0; g < TPL; g++) {
+      double pair[2];
+      ld_group<2>(S.rfac, (ot + g) * 2, pair);
+      rs[2 * g] = pair[0];
+      rs[2 * g + 1] = pair[1];
     }
-#pragma unroll 1
-    for (int t = 0; t < NT; t++) {
-      double te = ld(S.tlo, ol * NT + t);
-      const double rin = ld(S.rfac, ol * 2 * NT + t), rout = ld(S.rfac, ol * 2 * NT + NT + t);
-      double an[W], oin[W], oout[W];
+    double an[W][TPL], ro[W][2 * TPL];
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        an[j] = ld(S.anti, (e[j] * L + k) * NT + t);
-        const uint32_t oo = (op[j] * L + k) * 2 * NT;
-        oin[j] = ld(S.rfac, oo + t);
-        oout[j] = ld(S.rfac, oo + NT + t);
+    for (int j = 0; j < W; j++) {
+      ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
+      const uint32_t oo = (op[j] * L + k) * NT + t0;
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        double pair[2];
+        ld_group<2>(S.rfac, (oo + g) * 2, pair);
+        ro[j][2 * g] = pair[0];
+        ro[j][2 * g + 1] = pair[1];
       }
-      // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
+    }
+    // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        const bool first = (neg >> j) & 1;  // c == m.cells(e)[0]
-        const double signedFactor = (first ? -1.0 : 1.0) * invAreaCell1;
-        const double in1 = first ? rin : oin[j], out1 = first ? rout : oout[j];
-        const double in2 = first ? oin[j] : rin, out2 = first ? oout[j] : rout;
-        const double x = an[j];
+    for (int j = 0; j < W; j++) {
+      const bool first = (neg >> j) & 1;  // c == m.cells(e)[0]
+      const double signedFactor = (first ? -1.0 : 1.0) * invAreaCell1;
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        const double rin = rs[2 * g], rout = rs[2 * g + 1];
+        const double oin = ro[j][2 * g], oout = ro[j][2 * g + 1];
+        const double in1 = first ? rin : oin, out1 = first ? rout : oout;
+        const double in2 = first ? oin : rin, out2 = first ? oout : rout;
+        const double x = an[j][g];
         const double hof = smax(0.0, x) * smin(out1, in2) + smin(0.0, x) * smin(in1, out2);
-        const double y = te + signedFactor * hof;
-        te = (j < n) ? y : te;
+        const double y = te[g] + signedFactor * hof;
+        te[g] = (j < n) ? y : te[g];
       }
-      if (S.use_hmix) {
-        // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
-        const double tc = ld(S.xtr, ol * NT + t);
-        double tn[W];
+    }
+    if (S.use_hmix) {
+      // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
+      double tc[TPL], tn[W][TPL], he[W], rho[W];
+      ld_group<TPL>(S.xtr, ot, tc);
 #pragma unroll
-        for (int j = 0; j < W; j++) tn[j] = ld(S.xtr, (op[j] * L + k) * NT + t);
+      for (int j = 0; j < W; j++) {
+        ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+        he[j] = ld(S.h_edge, e[j] * L + k);
+        rho[j] = M.msd * S.kappa * ld(M.dv, e[j]) / ld(M.dc, e[j]);
+      }
 #pragma unroll
-        for (int j = 0; j < W; j++) {
-          const bool first = (neg >> j) & 1;
-          const double turb = first ? tn[j] - tc : tc - tn[j];
+      for (int j = 0; j < W; j++) {
+        const bool first = (neg >> j) & 1;
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
           const double flux = he[j] * turb * rho[j];
-          const double y = te - (first ? -1.0 : 1.0) * flux * invAreaCell1;
-          te = (j < n) ? y : te;
+          const double y = te[g] - (first ? -1.0 : 1.0) * flux * invAreaCell1;
+          te[g] = (j < n) ? y : te[g];
         }
       }
-      tracer_update(S, ol, ol * NT + t, te, hy);
     }
+    const double hy = ld(S.yh, ol);
+#pragma unroll
+    for (int g = 0; g < TPL; g++) tracer_update(S, ol, ot + g, te[g], hy);
   }
 }
 
@@ -485,7 +556,7 @@ __global__ __launch_bounds__(BS) void k_diag_cell(DMesh M, const double *__restr
 // or writes, counted once (gathered neighbours included once).  This is
 // the algorithmic traffic the roofline of each kernel is measured against.
 struct Bytes {
-  double edges, cells, tracer, vertex, edge_diag;
+  double edges, ho, cells, tracer, vertex, edge_diag;
 };
 
 Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_adv,
@@ -493,91 +564,115 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   const double C = M.nC, E = M.nE, V = M.nV, L = M.nL, T = M.nT, d = 8.0;
   const bool fct = T > 0 && S.scheme == MPAS_TRACER_FCT;
   const bool cen = T > 0 && S.scheme == MPAS_TRACER_CENTRED;
+  // state touches of an RK4 stage: stage 0 reads prev (= provis) and writes
+  // provis and next; stages 1-2 read prev, read+write next, write provis;
+  // stage 3 reads next and writes the final state
+  const double touches = S.stage == 0 ? 2 : S.stage < 3 ? 4 : 2;
   Bytes b{};
-  // k_stage_edges
-  double e = 0;
+  // k_stage_edges: momentum
   if (S.freeze_u) {
-    e += S.stage < 3 ? 2 * d * E * L : 0;
+    b.edges = S.stage < 3 ? 2 * d * L * E : 0;
   } else {
-    e += 8 * E + 2 * d * E + 1 * E + 4 * sum_eoe + d * sum_eoe;  // e2c, dc, dv, eoe, woe
-    e += d * L * (3 * E + 2 * C) + d * C;                       // xu h_edge pv_edge, ke xh, h_s
-    if (S.use_visc) e += 8 * E + d * L * (C + V);               // e2v, div, vort
-    if (S.stage == 0) e += d * L * 2 * E;                       // nu, yu written (u0 = xu)
-    else if (S.stage < 3) e += d * L * 4 * E;                   // u0, nu rw, yu
-    else e += d * L * 2 * E;                                    // nu, yu
+    b.edges = 8 * E + d * E + 1 * E + 4 * sum_eoe + d * sum_eoe;   // e2c, dc, eoe, woe
+    b.edges += d * L * (3 * E + 2 * C) + d * C;                    // xu h_edge pv_edge, ke xh, h_s
+    if (S.use_visc) b.edges += 8 * E + d * E + d * L * (C + V);    // e2v, dv, div, vort
+    b.edges += touches * d * L * E;                                // u0 / nu / yu
   }
-  if (fct) e += 1 * E + 20 * sum_adv + d * L * T * (C + E);     // nAdv adv coefs, xtr, anti
-  b.edges = e;
+  // k_stage_ho: antidiffusive flux per edge and tracer
+  if (fct) b.ho = 8 * E + d * E + E + 20 * sum_adv + 2 * d * L * E + d * L * T * (C + E);
   // k_stage_cells
-  double c = 6 * C + d * C + 4 * sum_eoc + 2 * d * E;           // nEoC sgn perm, area, c2e, dv dc
-  c += d * L * (3 * E);                                         // xu h_edge yu
-  c += d * L * C * (S.stage == 0 ? 3 : S.stage < 3 ? 4 : 3);    // h0/nh/yh
-  c += d * L * C * (1 + (S.use_visc ? 1 : 0));                  // ke_y div_y
-  if (fct) c += 4 * sum_eoc + d * L * (C + T * C + T * E + 3 * T * C);  // c2c, xh, xtr, anti, rfac tlo
-  if (cen) c += 4 * sum_eoc + d * L * T * (S.stage == 0 ? 3 : S.stage < 3 ? 4 : 3) * C;
-  b.cells = c;
+  b.cells = 14 * C + 4 * sum_eoc + 2 * d * E;                      // nEoC sgn rank area, c2e, dv dc
+  b.cells += d * L * 3 * E;                                        // yu xu h_edge
+  b.cells += d * L * C * (touches + 1 + (S.use_visc ? 1 : 0));     // h0/nh/yh, ke_y, div_y
+  if (fct) b.cells += 4 * sum_eoc + d * L * (C + T * C + T * E + 3 * T * C);  // cOpp, xh, xtr, anti, rfac tlo
+  if (cen) b.cells += 4 * sum_eoc + d * L * T * C * (touches + 1);  // cOpp, xtr + tr0/ntr/ytr
   // k_stage_tracer
-  double t = 0;
   if (fct) {
-    t = 2 * C + d * C + 8 * sum_eoc;                            // nEoC sgn, area, c2e cOpp
-    t += d * L * T * (C + 2 * C + E + C);                       // tlo rfac anti xtr
-    t += d * L * C;                                             // yh
-    t += d * L * T * C * (S.stage == 0 ? 2 : S.stage < 3 ? 4 : 2);  // tr0/ntr/ytr
-    if (S.stage > 0 && S.stage < 3) t += d * L * C;             // h0
-    if (S.use_hmix) t += 2 * d * E + d * L * E;                 // dv dc h_edge
+    b.tracer = 10 * C + 8 * sum_eoc;                               // nEoC sgn area, c2e cOpp
+    b.tracer += d * L * T * (C + 2 * C + E) + d * L * C;           // tlo rfac anti, yh
+    b.tracer += d * L * T * C * touches + (S.stage > 0 && S.stage < 3 ? d * L * C : 0);
+    if (S.use_hmix) b.tracer += 2 * d * E + d * L * E + d * L * T * C;  // dv dc h_edge xtr
   }
-  b.tracer = t;
   b.vertex = 44 * V + 2 * d * V + d * E + d * L * (E + C + 2 * V);
   b.edge_diag = 16 * E + d * L * (C + V + 2 * E);
   return b;
 }
 
+// One RK stage, in order on the context's stream: momentum, high-order
+// flux (FCT), cells, tracer update (FCT), then the diagnostics of the
+// stage's output.  (Running the independent pairs on two streams at once was
+// measured slower on the level-8 workload: the pairs contend for L2 and HBM.)
 template <int NT, int W, int WE>
-int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
-                 const Bytes &b) {
-  if (int rc = prof_begin(ctx, "k_stage_edges", b.edges)) return rc;
-  hipLaunchKernelGGL((k_stage_edges<NT, WE, WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M,
-                     S);
-  if (int rc = after_launch("k_stage_edges")) return rc;
-  if (int rc = prof_end(ctx)) return rc;
-  if (int rc = prof_begin(ctx, "k_stage_cells", b.cells)) return rc;
-  hipLaunchKernelGGL((k_stage_cells<NT, W>), dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, S);
-  if (int rc = after_launch("k_stage_cells")) return rc;
-  if (int rc = prof_end(ctx)) return rc;
+int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
+                 double *vort_y, double *pv_vertex, double *h_edge_y, double *pv_edge_y) {
+  constexpr int NPL = NT > 0 ? NT / tracers_per_lane<NT>() : 1;  // lanes per entity
+  const bool fct = NT > 0 && S.scheme == MPAS_TRACER_FCT;
+  hipStream_t st = ctx->stream;
+#define MPAS_STEP(expr) \
+  do {                  \
+    if (int rc = (expr)) return rc; \
+  } while (0)
+  MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
+  hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S);
+  MPAS_STEP(after_launch("k_stage_edges"));
+  MPAS_STEP(prof_end(ctx));
   if constexpr (NT > 0) {
-    if (S.scheme == MPAS_TRACER_FCT) {
-      if (int rc = prof_begin(ctx, "k_stage_tracer", b.tracer)) return rc;
-      hipLaunchKernelGGL((k_stage_tracer<NT, W>), dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M,
-                         S);
-      if (int rc = after_launch("k_stage_tracer")) return rc;
-      if (int rc = prof_end(ctx)) return rc;
+    if (fct) {
+      MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
+      hipLaunchKernelGGL((k_stage_ho<NT, WE>), dim3(xcd_grid(int64_t(M.nE) * NPL, BS)), dim3(BS),
+                         0, st, M, S);
+      MPAS_STEP(after_launch("k_stage_ho"));
+      MPAS_STEP(prof_end(ctx));
     }
   }
+  MPAS_STEP(prof_begin(ctx, "k_stage_cells", b.cells));
+  hipLaunchKernelGGL((k_stage_cells<NT, W>), dim3(xcd_grid(int64_t(M.nC) * NPL, BS)), dim3(BS),
+                     0, st, M, S);
+  MPAS_STEP(after_launch("k_stage_cells"));
+  MPAS_STEP(prof_end(ctx));
+  if constexpr (NT > 0) {
+    if (fct) {
+      MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
+      hipLaunchKernelGGL((k_stage_tracer<NT, W>), dim3(xcd_grid(int64_t(M.nC) * NPL, BS)),
+                         dim3(BS), 0, st, M, S);
+      MPAS_STEP(after_launch("k_stage_tracer"));
+      MPAS_STEP(prof_end(ctx));
+    }
+  }
+  MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
+  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, S.yh, S.yu,
+                     vort_y, pv_vertex);
+  MPAS_STEP(after_launch("k_diag_vertex"));
+  MPAS_STEP(prof_end(ctx));
+  MPAS_STEP(prof_begin(ctx, "k_diag_edge", b.edge_diag));
+  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S.yh, pv_vertex,
+                     h_edge_y, pv_edge_y);
+  MPAS_STEP(after_launch("k_diag_edge"));
+  MPAS_STEP(prof_end(ctx));
+#undef MPAS_STEP
   return 0;
 }
 
-// Widths: cells of up to 6 edges with stencils of up to 10 (every
-// icosahedral SCVT), else up to 8 and 16 (the reference's maxEdges and
-// maxEdges2).
+#define MPAS_STAGE_ARGS ctx, M, S, b, vort_y, pv_vertex, h_edge_y, pv_edge_y
 template <int NT>
-int launch_stage_w(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
-                   const Bytes &b) {
-  if (M.narrow) return launch_stage<NT, 6, 10>(ctx, M, S, st, b);
-  return launch_stage<NT, 8, 16>(ctx, M, S, st, b);
+int launch_stage_w(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
+                   double *vort_y, double *pv_vertex, double *h_edge_y, double *pv_edge_y) {
+  if (M.narrow) return launch_stage<NT, 6, 10>(MPAS_STAGE_ARGS);
+  return launch_stage<NT, 8, 16>(MPAS_STAGE_ARGS);
 }
 
-int launch_stage_nt(mpas_ctx *ctx, const DMesh &M, const StageIO &S, hipStream_t st,
-                    const Bytes &b) {
+int launch_stage_nt(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
+                    double *vort_y, double *pv_vertex, double *h_edge_y, double *pv_edge_y) {
   switch (M.nT) {
-    case 0: return launch_stage_w<0>(ctx, M, S, st, b);
-    case 1: return launch_stage_w<1>(ctx, M, S, st, b);
-    case 2: return launch_stage_w<2>(ctx, M, S, st, b);
-    case 3: return launch_stage_w<3>(ctx, M, S, st, b);
-    case 4: return launch_stage_w<4>(ctx, M, S, st, b);
-    case 5: return launch_stage_w<5>(ctx, M, S, st, b);
-    case 6: return launch_stage_w<6>(ctx, M, S, st, b);
-    case 7: return launch_stage_w<7>(ctx, M, S, st, b);
-    case 8: return launch_stage_w<8>(ctx, M, S, st, b);
+    case 0: return launch_stage_w<0>(MPAS_STAGE_ARGS);
+    case 1: return launch_stage_w<1>(MPAS_STAGE_ARGS);
+    case 2: return launch_stage_w<2>(MPAS_STAGE_ARGS);
+    case 3: return launch_stage_w<3>(MPAS_STAGE_ARGS);
+    case 4: return launch_stage_w<4>(MPAS_STAGE_ARGS);
+    case 5: return launch_stage_w<5>(MPAS_STAGE_ARGS);
+    case 6: return launch_stage_w<6>(MPAS_STAGE_ARGS);
+    case 7: return launch_stage_w<7>(MPAS_STAGE_ARGS);
+    case 8: return launch_stage_w<8>(MPAS_STAGE_ARGS);
   }
   MPAS_FAIL(MPAS_EINVAL, "the fused step supports at most 8 tracers");
 }
@@ -606,7 +701,6 @@ int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot) {
 int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
   const DMesh &M = ctx->dm;
   SwState &W = *ctx->sw;
-  hipStream_t st = ctx->stream;
   const mpas_params &P = ctx->params;
   const double rk_weights[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
   const double rk_substep_weights[4] = {dt / 2, dt / 2, dt, 0};
@@ -652,17 +746,9 @@ int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
       S.rfac = W.rfac;
       S.tlo = W.tlo;
       const Bytes b = stage_bytes(M, S, sum_eoe, sum_adv, sum_eoc);
-      if (int rc = launch_stage_nt(ctx, M, S, st, b)) return rc;
-      if (int rc = prof_begin(ctx, "k_diag_vertex", b.vertex)) return rc;
-      hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, S.yh,
-                         S.yu, W.vort[nxt], W.pv_vertex);
-      if (int rc = after_launch("k_diag_vertex")) return rc;
-      if (int rc = prof_end(ctx)) return rc;
-      if (int rc = prof_begin(ctx, "k_diag_edge", b.edge_diag)) return rc;
-      hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S.yh,
-                         W.pv_vertex, W.h_edge[nxt], W.pv_edge[nxt]);
-      if (int rc = after_launch("k_diag_edge")) return rc;
-      if (int rc = prof_end(ctx)) return rc;
+      if (int rc = launch_stage_nt(ctx, M, S, b, W.vort[nxt], W.pv_vertex, W.h_edge[nxt],
+                                   W.pv_edge[nxt]))
+        return rc;
       W.cur = nxt;
     }
     W.elapsed += dt;
