@@ -296,6 +296,17 @@ int mpas_ctx_create(const mpas_mesh_desc *mesh, const mpas_params *params, mpas_
   auto ctx = std::make_unique<mpas_ctx>();
   if (int rc = prepare_host_mesh(mesh, params, ctx->host)) return rc;
   ctx->params = *params;
+  {
+    // The kernels form byte offsets in 32 bits (ld / st): every array they
+    // index must stay below 4 GiB.  The largest are the 16-wide edge lists
+    // of doubles, the limiter factors [C][L][T][2] and the [E][L][T] fluxes.
+    const HostMesh &h = ctx->host;
+    const double nT = std::max(1, params->nTracers), nL = h.nL;
+    const double worst = std::max({16.0 * 8 * h.nE, 16.0 * h.nC * nL * nT, 8.0 * h.nE * nL * nT,
+                                   8.0 * h.nV * nL * 3});
+    if (worst >= 4294967296.0)
+      MPAS_FAIL(MPAS_ESHAPE, "mesh too large for one context: an array would exceed 4 GiB");
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     MPAS_FAIL(MPAS_EHIP, "no HIP device is visible");
