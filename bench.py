@@ -65,6 +65,122 @@ def cpu_baseline(mesh, u, h, tr, h_s, dt, steps, n_tr):
                       f"{sec:.2f} s timed, {setup_s:.1f} s setup untimed"}
 
 
+def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3):
+    """Gradient of an observation loss with respect to the initial h, u and
+    tracer (level-6 mesh, 200 steps of dt = 240 s, 1 FCT tracer,
+    checkpoints every 20 steps).  Loss = sum over 1,000 seeded cells of the
+    squared differences of h and the tracer at the final step from
+    observations taken from a seeded-perturbed forward run.  Cost ratio =
+    (forward with checkpoints + backward sweep) / plain forward; accuracy by
+    the dot-product test <M dx, y> = <dx, M^T y> over the full 200 steps."""
+    from mpas_hip import TRACER_FCT, Context, Mesh, sw_workload_state
+    mesh = Mesh(level)
+    u, h, tr = sw_workload_state(mesh, n_tr, seed=3)
+    tr = tr.ravel()
+    ctx = Context(mesh, nTracers=n_tr, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    ctx.sw_upload(h, u, tr, np.full(mesh.nCells, 4500.0))
+    ctx.adj_init(k)
+    x0 = ctx.state_arrays()
+    for a, v in zip(x0, (u, h, tr)):
+        a.upload(v)
+    rng = np.random.default_rng(2024)
+    obs = rng.choice(mesh.nCells, n_obs, replace=False)
+    # observations from a forward run of a perturbed initial state
+    xp = ctx.state_arrays()
+    for a, v, sc in zip(xp, (u, h, tr), (0.5, 2.0, 0.02)):
+        a.upload(v + sc * rng.standard_normal(v.size))
+    xo = ctx.state_arrays()
+    ctx.adj_forward(dt, steps, xp, xo)
+    ctx.sync()
+    h_obs, tr_obs = xo[1].numpy()[obs], xo[2].numpy()[obs]
+
+    def timed(fn):
+        best = float("inf")
+        for _ in range(reps):
+            ctx.sync()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    # plain forward: the fused step from the same state
+    def plain():
+        ctx.sw_upload(h, u, tr, np.full(mesh.nCells, 4500.0))
+        ctx.sync()
+    t_fwd = float("inf")
+    for _ in range(reps):
+        plain()
+        t0 = time.perf_counter()
+        ctx.rk4_step(dt, steps)
+        ctx.sync()
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    x1 = ctx.state_arrays()
+    t_adj_fwd = timed(lambda: ctx.adj_forward(dt, steps, x0, x1))
+    hN, trN = x1[1].numpy(), x1[2].numpy()
+    J = float(np.sum((hN[obs] - h_obs) ** 2) + np.sum((trN[obs] - tr_obs) ** 2))
+    seed_u = np.zeros(mesh.nEdges)
+    seed_h = np.zeros(mesh.nCells)
+    seed_t = np.zeros(mesh.nCells * n_tr)
+    seed_h[obs] = 2 * (hN[obs] - h_obs)
+    seed_t[obs] = 2 * (trN[obs] - tr_obs)
+    yb = ctx.state_arrays()
+    for a, v in zip(yb, (seed_u, seed_h, seed_t)):
+        a.upload(v)
+    g = ctx.state_arrays()
+    t_bwd = timed(lambda: ctx.adj_backward(yb, g))
+    grad = [a.numpy() for a in g]
+    # dot-product test over the full run: random dx, y
+    dx_h = [rng.standard_normal(mesh.nEdges), rng.standard_normal(mesh.nCells),
+            0.01 * rng.standard_normal(mesh.nCells * n_tr)]
+    y_h = [rng.standard_normal(mesh.nEdges), rng.standard_normal(mesh.nCells),
+           rng.standard_normal(mesh.nCells * n_tr)]
+    dx, y, dxN, xb = (ctx.state_arrays() for _ in range(4))
+    for a, v in zip(dx, dx_h):
+        a.upload(v)
+    for a, v in zip(y, y_h):
+        a.upload(v)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.tlm_forward(dt, steps, x0, dx, dx1=dxN)
+    ctx.sync()
+    t_tlm = time.perf_counter() - t0
+    ctx.adj_forward(dt, steps, x0)
+    ctx.adj_backward(y, xb)
+    ctx.sync()
+    lhs = sum(float(np.dot(a.numpy(), v)) for a, v in zip(dxN, y_h))
+    rhs = sum(float(np.dot(v, a.numpy())) for a, v in zip(xb, dx_h))
+    dot_rel = abs(lhs - rhs) / abs(lhs)
+    # central differences of the loss (the plain forward, limiter not frozen)
+    eps = 1e-7
+    d_h = [0.01 * u * dx_h[0], dx_h[1], tr * dx_h[2]]
+    jd = []
+    for sgn in (1, -1):
+        xe = ctx.state_arrays()
+        for a, v, d in zip(xe, (u, h, tr), d_h):
+            a.upload(v + sgn * eps * d)
+        ctx.adj_forward(dt, steps, xe, x1)
+        ctx.sync()
+        he, te = x1[1].numpy(), x1[2].numpy()
+        jd.append(float(np.sum((he[obs] - h_obs) ** 2) + np.sum((te[obs] - tr_obs) ** 2)))
+    fd = (jd[0] - jd[1]) / (2 * eps)
+    ga = sum(float(np.dot(gv, d)) for gv, d in zip(grad, d_h))
+    ctx.close()
+    return {
+        "workload": f"icosahedral SCVT level {level} ({mesh.nCells} cells), FP64 SW RK4 + FCT, "
+                    f"{steps} steps dt={dt:g} s, {n_tr} tracer, checkpoint every {k} steps, "
+                    f"loss over {n_obs} seeded cells (h and tracer) vs perturbed-run observations",
+        "forward_ms": 1e3 * t_fwd, "forward_with_checkpoints_ms": 1e3 * t_adj_fwd,
+        "backward_ms": 1e3 * t_bwd, "tlm_ms": 1e3 * t_tlm,
+        "adjoint_forward_cost_ratio": (t_adj_fwd + t_bwd) / t_fwd,
+        "loss": J, "dot_product_rel_err": dot_rel,
+        "dot_product": {"tlm_side": lhs, "adjoint_side": rhs},
+        "fd_check": {"eps": eps, "adjoint": ga, "central_difference": fd,
+                     "rel_diff": abs(ga - fd) / abs(fd),
+                     "note": "plain forward: the limiter is not frozen in the differences"},
+    }
+
+
 def copy_calibration(nbytes):
     """k_copy over nbytes, 10 launches: reads and writes nbytes each."""
     from mpas_hip import Context, Mesh
@@ -95,6 +211,9 @@ def main():
                     help="steps of the per-kernel event-timed pass")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint measurement (level 6, 200 steps, 1 FCT tracer)")
+    ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel k_copy over BYTES, 10 times "
                          "(the known-size reference for HBM counters)")
@@ -109,6 +228,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_bench()), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -181,6 +303,11 @@ def main():
                      "avg_launch_us": avg_s * 1e6, "profiled_ms_per_step": prof_ms_step},
         "mass_drift": mass_drift,
     }
+    if not args.no_adjoint:
+        log("adjoint (level 6, 200 steps) ...")
+        out["adjoint"] = adjoint_bench()
+        log(f"  ratio {out['adjoint']['adjoint_forward_cost_ratio']:.3f}, "
+            f"dot-product rel err {out['adjoint']['dot_product_rel_err']:.2e}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.cpu_steps,

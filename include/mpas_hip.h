@@ -293,6 +293,45 @@ int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host,
    with the stage-fused kernels; the result equals the per-task sequence. */
 int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps);
 
+/* ---- tangent-linear model and discrete adjoint of mpas_rk4_step ----------
+   No counterpart in the reference (it has no derivative code); the forward
+   they differentiate is apps/sw/run.cc:13-70 with the FCT tracer tendency of
+   apps/ocean/tasks/solve.cc:40-366.  The FCT limiter factors are frozen at
+   the forward pass's values.  Every pointer below is a device array in the
+   state layouts of mpas_sw_state_upload (tracer pointers may be NULL when
+   nTracers == 0); mpas_sw_state_upload must have been called once (it sets
+   h_s).  Supports tracer_scheme FCT (or no tracers) with every level active.
+   Calls are ordered on the context's stream. */
+
+/* Allocates the tape: 4*checkpoint_every + 1 stage records. */
+int mpas_adj_init(mpas_ctx *ctx, int checkpoint_every);
+/* Runs nsteps fused steps from (u0, h0, tr0), keeping a checkpoint of the
+   state every checkpoint_every steps; the final state goes to (u1, h1, tr1)
+   (each may be NULL) and stays resident as the context's state. */
+int mpas_adj_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0,
+                     const double *h0, const double *tr0, double *u1, double *h1,
+                     double *tr1);
+/* Gradient of a scalar loss with respect to the initial state of the last
+   mpas_adj_forward, given its gradient with respect to the final state.
+   Replays each checkpoint segment into the tape, then sweeps it in reverse. */
+int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
+                      const double *tr1_bar, double *u0_bar, double *h0_bar,
+                      double *tr0_bar);
+/* Tangent-linear run: nsteps steps of the state from (u0, h0, tr0) and of the
+   perturbation (du0, dh0, dtr0) along it; outputs may be NULL.  Leaves the
+   context's resident state untouched. */
+int mpas_tlm_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0,
+                     const double *h0, const double *tr0, const double *du0,
+                     const double *dh0, const double *dtr0, double *u1, double *h1,
+                     double *tr1, double *du1, double *dh1, double *dtr1);
+/* Check hook: nsteps forward steps from (u0, h0, tr0) that use the limiter
+   factors held in the tape instead of forming them (the map the TLM
+   differentiates).  The tape holds steps 0 .. K-1 of the last forward run
+   after mpas_adj_backward; nsteps must not exceed them. */
+int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0,
+                            const double *h0, const double *tr0, double *u1,
+                            double *h1, double *tr1);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1036 @@
+// Tangent-linear model (TLM) and discrete adjoint of the stage-fused RK4
+// step with FCT tracers (the forward is apps/sw/run.cc:13-70 with the
+// tracer tasks of apps/ocean/tasks/solve.cc:40-366 as the tendency; the
+// reference has no derivative code).
+//
+// Linearisation.  One stage maps X = (u, h, tracers) to the tendencies
+// (tend_u, tend_h, h-weighted tracer tendency); the RK4 bookkeeping combines
+// them (u and h linearly, tracers as (h0*T0 + w*tend)/h).  The FCT limiter
+// factors R+/R- of every cell are frozen at the forward pass's values; every
+// branch (upwind side max/min(F, 0), the sign of F in the 3rd-order
+// coefficient, the sign of the antidiffusive flux that selects which factor
+// pair scales it) follows the forward state.  With these fixed the stage is
+// a smooth map, and TLM and adjoint are its derivative and the transpose of
+// that derivative.
+//
+// Gathers only.  Each forward scatter (a cell or vertex value fed by its
+// edges) becomes, in the adjoint, a gather over the transposed list: the
+// edges that hold an edge in their edgesOnEdge (with that slot's weight),
+// the edges whose advection stencil holds a cell (with the stencil
+// coefficients), and the vertices of a cell (with the kite area the vertex
+// pairs with that cell).  No atomics; every value is summed in one fixed
+// order, so both models are deterministic.
+//
+// Tape.  The forward keeps a checkpoint of the state every K steps.  The
+// backward sweep replays each segment with the same fused stage kernels,
+// writing every stage's input state, diagnostics, limiter factors and
+// antidiffusive fluxes into a tape of 4K + 1 stage records (the replay is
+// bit-identical to the first pass), then sweeps the segment in reverse.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+#include "stage.h"
+
+namespace mpasx {
+
+// Forward record of one RK stage: its input state X, the diagnostics of X,
+// and the limiter factors and antidiffusive fluxes the stage formed.
+struct Rec {
+  double *u, *h, *tr, *he, *ke, *pve, *vort, *div, *rfac, *anti;
+};
+
+// Transposed lists, slot-major ELL with run-time widths.
+struct AdjMesh {
+  // edges e2 whose edgesOnEdge holds e, and weightsOnEdge(e2, that slot)
+  const uint8_t *nEt;
+  const int *et;
+  const double *wt;
+  // edges whose advCellsForEdge holds c, with advCoef and advCoef_3rd of
+  // that slot
+  const uint8_t *nAt;
+  const int *at;
+  const double *ac1, *ac3;
+  // vertices of c with the kite area that h_vertex pairs with c (the i-th
+  // ascending cell of the vertex takes the i-th file-order kite area)
+  const uint8_t *nVc;
+  const int *vc;
+  const double *vk;
+};
+
+struct AdjState {
+  int K = 0;           // steps per checkpoint segment
+  int nsteps = 0;      // steps of the last mpas_adj_forward
+  double dt = 0;
+  bool forward_done = false;
+  int taped = 0;       // steps from step 0 whose records the tape holds
+  std::vector<Rec> rec;          // 4K + 1 stage records
+  std::vector<double *> ck;      // checkpoints: (u, h, tr) per segment
+  double *pv_vertex = nullptr, *nu = nullptr, *nh = nullptr, *ntr = nullptr, *tlo = nullptr;
+  // tangent-linear perturbations of the 5 stage states of a step, the
+  // RK4 accumulators, and per-stage intermediates
+  double *du[5] = {}, *dh[5] = {}, *dT[5] = {};
+  double *dnu = nullptr, *dnh = nullptr, *dQ = nullptr;
+  double *dvort = nullptr, *dpvv = nullptr, *dhe = nullptr, *dpve = nullptr;
+  double *dke = nullptr, *ddiv = nullptr, *dflux = nullptr;
+  // adjoint: state adjoints (ping-pong), the adjoints of the step's final
+  // state after the final division (aw_*), the current stage's tendency
+  // adjoints (tb_*), and per-stage intermediates
+  double *xb_u[2] = {}, *xb_h[2] = {}, *xb_tr[2] = {};
+  double *aw_u = nullptr, *aw_h = nullptr, *aw_tr = nullptr;
+  double *tb_u = nullptr, *tb_h = nullptr, *tb_tr = nullptr;
+  double *ubA = nullptr, *heb = nullptr, *pveb = nullptr, *r1 = nullptr, *r2 = nullptr,
+         *hbf = nullptr, *sg = nullptr, *vT = nullptr, *hvT = nullptr, *keT = nullptr,
+         *divT = nullptr;
+  AdjMesh T{};
+  std::vector<void *> bufs;
+};
+
+void adj_free(mpas_ctx *ctx) {
+  if (!ctx->adj) return;
+  for (void *p : ctx->adj->bufs) (void)hipFree(p);
+  delete ctx->adj;
+  ctx->adj = nullptr;
+}
+
+namespace {
+
+constexpr int BS = 256;
+
+// Per-stage constants of the linearised kernels.
+struct AdjParams {
+  int s, nT, freeze_u, use_visc, use_hmix, fct;
+  double w_rk, w_sub;      // weights of stage s
+  double w_rk_m, w_sub_m;  // weights of stage s - 1 (adjoint bookkeeping)
+  double visc_coef, kappa;
+};
+
+__device__ __forceinline__ double fsign(double F) { return (F < 0.0) ? -1.0 : double(F > 0.); }
+
+// Slope of the rescaled antidiffusive flux in the flux x with the limiter
+// factors frozen: max(0,x)*min(out1,in2) + min(0,x)*min(in1,out2)
+// (ocean/solve.cc:270-282).
+__device__ __forceinline__ double frozen_slope(const double *__restrict__ rfac, uint32_t i1,
+                                               uint32_t i2, double x) {
+  const double in1 = ld(rfac, 2 * i1), out1 = ld(rfac, 2 * i1 + 1);
+  const double in2 = ld(rfac, 2 * i2), out2 = ld(rfac, 2 * i2 + 1);
+  return x > 0.0 ? smin(out1, in2) : (x < 0.0 ? smin(in1, out2) : 0.0);
+}
+
+// h_vertex and pv_vertex of the recorded state, formed as the forward forms
+// them (ascending cells with file-order kite areas).
+__device__ __forceinline__ void vertex_state(const DMesh &M, int v, uint32_t k, const Rec &R,
+                                             const uint32_t (&c)[3], const double (&kt)[3],
+                                             double at, double &hv, double &pvv) {
+  const uint32_t L = M.nL;
+  double s = 0.0;
+  s += ld(R.h, c[0] * L + k) * kt[0];
+  s += ld(R.h, c[1] * L + k) * kt[1];
+  s += ld(R.h, c[2] * L + k) * kt[2];
+  hv = s / at;
+  pvv = (ld(M.fV, v) + ld(R.vort, v * L + k)) / hv;
+}
+
+// ---- tangent-linear model -----------------------------------------------------
+
+__global__ __launch_bounds__(BS) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
+                                                  const double *__restrict__ du,
+                                                  const double *__restrict__ dh, double *dvort,
+                                                  double *dpvv) {
+  const int v = xcd_item();
+  if (v >= M.nV) return;
+  const uint32_t L = M.nL, V = M.nV;
+  int x[3];
+  uint32_t c[3];
+  double kt[3];
+  for (int i = 0; i < 3; i++) {
+    x[i] = ld(M.v2e, i * V + v);
+    c[i] = ld(M.v2c, i * V + v);
+    kt[i] = ld(M.kite, i * V + v);
+  }
+  const double at = ld(M.areaTri, v);
+  for (uint32_t k = 0; k < L; k++) {
+    double dcirc = 0.0, dhv = 0.0;
+    for (int i = 0; i < 3; i++) {
+      const uint32_t e = v2e_id(x[i]);
+      dcirc += v2e_sign(x[i]) * (ld(M.dc, e) * ld(du, e * L + k));
+      dhv += ld(dh, c[i] * L + k) * kt[i];
+    }
+    const double dvo = dcirc / at;
+    dhv /= at;
+    double hv, pvv;
+    vertex_state(M, v, k, R, c, kt, at, hv, pvv);
+    st(dvort, v * L + k, dvo);
+    st(dpvv, v * L + k, (dvo - pvv * dhv) / hv);
+  }
+}
+
+// h_edge and pv_edge perturbations, and the perturbation of the edge's net
+// tracer flux into its cells: (1 - lambda) dLO + lambda dHO - dH (upwind,
+// 3rd-order, del2 mixing flux).
+__global__ __launch_bounds__(BS) void k_tl_edge_a(DMesh M, AdjParams P, Rec R,
+                                                  const double *__restrict__ du,
+                                                  const double *__restrict__ dh,
+                                                  const double *__restrict__ dT,
+                                                  const double *__restrict__ dpvv, double *dhe,
+                                                  double *dpve, double *dflux) {
+  const int e = xcd_item();
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE, NT = P.nT;
+  const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
+  const uint32_t c1 = cc.x, c2 = cc.y;
+  const double dv = ld(M.dv, e), dc = ld(M.dc, e);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    const double dhe_ = 0.5 * (ld(dh, c1 * L + k) + ld(dh, c2 * L + k));
+    st(dhe, o, dhe_);
+    st(dpve, o, 0.5 * ld(dpvv, uint32_t(vv.x) * L + k) + 0.5 * ld(dpvv, uint32_t(vv.y) * L + k));
+    if (!P.fct) continue;
+    const double ue = ld(R.u, o), hee = ld(R.he, o);
+    const double F = ue * hee, dF = ld(du, o) * hee + ue * dhe_;
+    const double sgn = fsign(F);
+    const double Fp = F > 0.0 ? F : 0.0, Fm = F < 0.0 ? F : 0.0;
+    const double dFp = F > 0.0 ? dF : 0.0, dFm = F < 0.0 ? dF : 0.0;
+    const int na = ld(M.nAdv, e);
+    const double rho = M.msd * P.kappa * dv / dc;
+    for (uint32_t t = 0; t < NT; t++) {
+      const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
+      const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
+      const double dT1 = ld(dT, i1), dT2 = ld(dT, i2);
+      const double dlo = dv * (dFp * T1 + Fp * dT1 + dFm * T2 + Fm * dT2);
+      double dho = 0.0;
+      for (int i = 0; i < na; i++) {
+        const uint32_t ci = ld(M.adv, i * E + e);
+        const double cf = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
+        const uint32_t ii = (ci * L + k) * NT + t;
+        dho += (ld(dT, ii) * F + ld(R.tr, ii) * dF) * cf;
+      }
+      const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
+      double dphi = (1.0 - lam) * dlo + lam * dho;
+      if (P.use_hmix) dphi -= rho * (dhe_ * (T2 - T1) + hee * (dT2 - dT1));
+      st(dflux, o * NT + t, dphi);
+    }
+  }
+}
+
+// Cell part: ke / divergence perturbations, tend_h and the tracer tendency,
+// then the RK4 bookkeeping of h and the tracers (solve.cc:297-305,
+// 332-340, 350-357 differentiated).
+__global__ __launch_bounds__(BS) void k_tl_cell(DMesh M, AdjParams P, Rec R, Rec Ry, Rec R0,
+                                                const double *__restrict__ du,
+                                                const double *__restrict__ dh0,
+                                                const double *__restrict__ dT0,
+                                                const double *__restrict__ dhe,
+                                                const double *__restrict__ dflux, double *dnh,
+                                                double *dQ, double *dke, double *ddiv,
+                                                double *dhy, double *dTy) {
+  const int c = xcd_item();
+  if (c >= M.nC) return;
+  const uint32_t L = M.nL, C = M.nC, NT = P.nT;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const double ar = ld(M.area, c);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t ol = c * L + k;
+    double dk = 0.0, dd = 0.0, dth = 0.0, dtT[8] = {};
+    for (int j = 0; j < n; j++) {
+      const uint32_t e = ld(M.c2e, j * C + c), oe = e * L + k;
+      const double sg = sign_on_cell(neg, j), dv = ld(M.dv, e), dc = ld(M.dc, e);
+      const double u = ld(R.u, oe), duj = ld(du, oe);
+      dk += 0.5 * dc * dv * u * duj;
+      dd += (-sg) * dv * duj;
+      dth += sg * dv * (duj * ld(R.he, oe) + u * ld(dhe, oe));
+      for (uint32_t t = 0; t < NT; t++) dtT[t] += sg * ld(dflux, oe * NT + t);
+    }
+    st(dke, ol, dk / ar);
+    if (P.use_visc) st(ddiv, ol, dd / ar);
+    dth /= ar;
+    const double d0 = ld(dh0, ol);
+    double dhy_;
+    if (P.s == 0) {
+      st(dnh, ol, d0 + P.w_rk * dth);
+      dhy_ = d0 + P.w_sub * dth;
+    } else if (P.s < 3) {
+      st(dnh, ol, ld(dnh, ol) + P.w_rk * dth);
+      dhy_ = d0 + P.w_sub * dth;
+    } else {
+      dhy_ = ld(dnh, ol) + P.w_rk * dth;
+    }
+    st(dhy, ol, dhy_);
+    if (!NT) continue;
+    const double hy = ld(Ry.h, ol), h0 = ld(R0.h, ol);
+    for (uint32_t t = 0; t < NT; t++) {
+      const uint32_t ot = ol * NT + t;
+      const double te = dtT[t] / ar;
+      const double Ty = ld(Ry.tr, ot), T0 = ld(R0.tr, ot), dt0 = ld(dT0, ot);
+      double y;
+      if (P.s == 0) {
+        st(dQ, ot, dt0 * h0 + T0 * d0 + P.w_rk * te);
+        y = (d0 * T0 + h0 * dt0 + P.w_sub * te - Ty * dhy_) / hy;
+      } else if (P.s < 3) {
+        st(dQ, ot, ld(dQ, ot) + P.w_rk * te);
+        y = (d0 * T0 + h0 * dt0 + P.w_sub * te - Ty * dhy_) / hy;
+      } else {
+        y = (ld(dQ, ot) + P.w_rk * te - Ty * dhy_) / hy;
+      }
+      st(dTy, ot, y);
+    }
+  }
+}
+
+// tend_u perturbation (PV flux, ke and height gradients, del2 viscosity) and
+// the RK4 bookkeeping of u.
+__global__ __launch_bounds__(BS) void k_tl_edge_b(DMesh M, AdjParams P, Rec R,
+                                                  const double *__restrict__ du,
+                                                  const double *__restrict__ du0,
+                                                  const double *__restrict__ dh,
+                                                  const double *__restrict__ dhe,
+                                                  const double *__restrict__ dpve,
+                                                  const double *__restrict__ dke,
+                                                  const double *__restrict__ ddiv,
+                                                  const double *__restrict__ dvort, double *dnu,
+                                                  double *duy) {
+  const int e = xcd_item();
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE;
+  if (P.freeze_u) {  // u stays the step's initial u (solve.cc:307-313, 360-366)
+    for (uint32_t k = 0; k < L; k++) st(duy, e * L + k, ld(du0, e * L + k));
+    return;
+  }
+  const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
+  const uint32_t c1 = cc.x, c2 = cc.y;
+  const int n = ld(M.nEoE, e);
+  const double dc = ld(M.dc, e), dv = ld(M.dv, e);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    const double pve = ld(R.pve, o), dpv = ld(dpve, o);
+    double dq = 0.0;
+    for (int j = 0; j < n; j++) {
+      const uint32_t oj = uint32_t(ld(M.eoe, j * E + e)) * L + k;
+      const double w = ld(M.woe, j * E + e);
+      const double uj = ld(R.u, oj), pvj = ld(R.pve, oj), hej = ld(R.he, oj);
+      dq += w * (ld(du, oj) * 0.5 * (pve + pvj) * hej + uj * 0.5 * (dpv + ld(dpve, oj)) * hej +
+                 uj * 0.5 * (pve + pvj) * ld(dhe, oj));
+    }
+    double dtu = dq - (ld(dke, c2 * L + k) - ld(dke, c1 * L + k) +
+                       M.gravity * (ld(dh, c2 * L + k) - ld(dh, c1 * L + k))) /
+                          dc;
+    if (P.use_visc)
+      dtu += P.visc_coef * ((ld(ddiv, c2 * L + k) - ld(ddiv, c1 * L + k)) / dc -
+                            (ld(dvort, uint32_t(vv.y) * L + k) -
+                             ld(dvort, uint32_t(vv.x) * L + k)) / dv);
+    const double d0 = ld(du0, o);
+    if (P.s == 0) {
+      st(dnu, o, d0 + P.w_rk * dtu);
+      st(duy, o, d0 + P.w_sub * dtu);
+    } else if (P.s < 3) {
+      st(dnu, o, ld(dnu, o) + P.w_rk * dtu);
+      st(duy, o, d0 + P.w_sub * dtu);
+    } else {
+      st(duy, o, ld(dnu, o) + P.w_rk * dtu);
+    }
+  }
+}
+
+// ---- adjoint ------------------------------------------------------------------
+
+// Start of a step's reverse sweep from the adjoint of its final state:
+// the final division T = Q / h reversed (aw_h = h adjoint after it, aw_tr =
+// Q adjoint), the adjoint of the step's initial state seeded with the
+// direct paths, and the stage-3 tendency adjoints.
+__global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Rec R0,
+                                                const double *__restrict__ ub_in,
+                                                const double *__restrict__ hb_in,
+                                                const double *__restrict__ trb_in,
+                                                double *ub_out, double *hb_out, double *trb_out,
+                                                double *aw_u, double *aw_h, double *aw_tr,
+                                                double *tb_u, double *tb_h, double *tb_tr) {
+  const int i = xcd_item();
+  const uint32_t L = M.nL, NT = P.nT;
+  if (i < M.nE) {
+    for (uint32_t k = 0; k < L; k++) {
+      const uint32_t o = i * L + k;
+      const double ub = ld(ub_in, o);
+      st(aw_u, o, ub);
+      st(ub_out, o, ub);
+      st(tb_u, o, P.freeze_u ? 0.0 : P.w_rk * ub);
+    }
+  }
+  if (i < M.nC) {
+    for (uint32_t k = 0; k < L; k++) {
+      const uint32_t ol = i * L + k;
+      const double hf = ld(Rf.h, ol), h0 = ld(R0.h, ol);
+      double htot = ld(hb_in, ol);
+      for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t ot = ol * NT + t;
+        htot -= ld(trb_in, ot) * ld(Rf.tr, ot) / hf;
+      }
+      double xh = htot;
+      for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t ot = ol * NT + t;
+        const double qb = ld(trb_in, ot) / hf;
+        st(aw_tr, ot, qb);
+        xh += qb * ld(R0.tr, ot);
+        st(trb_out, ot, qb * h0);
+        st(tb_tr, ot, P.w_rk * qb);
+      }
+      st(aw_h, ol, htot);
+      st(hb_out, ol, xh);
+      st(tb_h, ol, P.w_rk * htot);
+    }
+  }
+}
+
+// Edge part of the stage adjoint: from the tendency adjoints to the partial
+// u adjoint (thickness flux, PV flux, tracer flux paths), the h_edge and
+// pv_edge adjoints, and per (edge, tracer) what the cells gather: r1 / r2
+// (into cell1 / cell2: upwind flux and mixing) and hbf (the 3rd-order flux
+// adjoint times F, which each stencil cell scales by its coefficient).
+__global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
+                                                  const double *__restrict__ tb_u,
+                                                  const double *__restrict__ tb_h,
+                                                  const double *__restrict__ tb_tr, double *ubA,
+                                                  double *heb, double *pveb, double *r1,
+                                                  double *r2, double *hbf, double *sg) {
+  const int e = xcd_item();
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE, NT = P.nT;
+  const int2 cc = ld(M.e2c, e);
+  const uint32_t c1 = cc.x, c2 = cc.y;
+  const double A1 = ld(M.area, c1), A2 = ld(M.area, c2);
+  const double dv = ld(M.dv, e), dc = ld(M.dc, e);
+  const int n = ld(M.nEoE, e), nt = ld(A.nEt, e), na = P.fct ? ld(M.nAdv, e) : 0;
+  const double rho = M.msd * P.kappa * dv / dc;
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    const double ue = ld(R.u, o), hee = ld(R.he, o), pvee = ld(R.pve, o);
+    // tend_h: cell1 -= u dv h_edge / area, cell2 += (solve.cc:191-205)
+    const double fb = -ld(tb_h, c1 * L + k) / A1 + ld(tb_h, c2 * L + k) / A2;
+    double ub = fb * dv * hee, hb = fb * ue * dv, pb = 0.0;
+    if (!P.freeze_u) {
+      // q = sum_j w_j u_j 0.5 (pv_e + pv_j) h_j: own row for pv_e, the
+      // rows that hold e for u_e, h_edge_e and pv_edge_e
+      const double tue = ld(tb_u, o);
+      for (int j = 0; j < n; j++) {
+        const uint32_t oj = uint32_t(ld(M.eoe, j * E + e)) * L + k;
+        pb += 0.5 * ld(M.woe, j * E + e) * tue * ld(R.u, oj) * ld(R.he, oj);
+      }
+      double s1 = 0.0, s2 = 0.0;
+      for (int j = 0; j < nt; j++) {
+        const uint32_t o2 = uint32_t(ld(A.et, j * E + e)) * L + k;
+        const double t = ld(tb_u, o2) * ld(A.wt, j * E + e);
+        s1 += t * 0.5 * (ld(R.pve, o2) + pvee);
+        s2 += t;
+      }
+      ub += s1 * hee;
+      hb += s1 * ue;
+      pb += 0.5 * s2 * ue * hee;
+    }
+    if (P.fct) {
+      const double F = ue * hee, sgn = fsign(F);
+      const double Fp = F > 0.0 ? F : 0.0, Fm = F < 0.0 ? F : 0.0;
+      double Fb = 0.0;
+      for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
+        const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
+        const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
+        const double lob = (1.0 - lam) * phib, hob = lam * phib;
+        const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
+        Fb += lob * dv * ((F > 0.0 ? T1 : 0.0) + (F < 0.0 ? T2 : 0.0));
+        double hs = 0.0;
+        for (int i = 0; i < na; i++) {
+          const uint32_t ci = ld(M.adv, i * E + e);
+          const double cf = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
+          hs += ld(R.tr, (ci * L + k) * NT + t) * cf;
+        }
+        Fb += hob * hs;
+        double a1 = lob * dv * Fp, a2 = lob * dv * Fm;
+        if (P.use_hmix) {  // mixing flux h_edge (T2 - T1) rho enters with -sign/area
+          const double Hb = -phib;
+          hb += Hb * rho * (T2 - T1);
+          const double Db = Hb * hee * rho;
+          a1 -= Db;
+          a2 += Db;
+        }
+        st(r1, o * NT + t, a1);
+        st(r2, o * NT + t, a2);
+        st(hbf, o * NT + t, hob * F);
+      }
+      ub += Fb * hee;
+      hb += Fb * ue;
+      st(sg, o, sgn);
+    }
+    st(ubA, o, ub);
+    st(heb, o, hb);
+    st(pveb, o, pb);
+  }
+}
+
+// Vertex part: pv_vertex = (f + vorticity) / h_vertex and the viscosity's
+// vorticity term reversed; vT and hvT are the vorticity and h_vertex
+// adjoints divided by areaTriangle.
+__global__ __launch_bounds__(BS) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
+                                                  const double *__restrict__ pveb,
+                                                  const double *__restrict__ tb_u, double *vT,
+                                                  double *hvT) {
+  const int v = xcd_item();
+  if (v >= M.nV) return;
+  const uint32_t L = M.nL, V = M.nV;
+  int x[3];
+  uint32_t c[3];
+  double kt[3];
+  for (int i = 0; i < 3; i++) {
+    x[i] = ld(M.v2e, i * V + v);
+    c[i] = ld(M.v2c, i * V + v);
+    kt[i] = ld(M.kite, i * V + v);
+  }
+  const double at = ld(M.areaTri, v);
+  for (uint32_t k = 0; k < L; k++) {
+    double pvvb = 0.0;
+    for (int i = 0; i < 3; i++) pvvb += 0.5 * ld(pveb, v2e_id(x[i]) * L + k);
+    double hv, pvv;
+    vertex_state(M, v, k, R, c, kt, at, hv, pvv);
+    double vb = pvvb / hv;
+    if (P.use_visc && !P.freeze_u)
+      for (int i = 0; i < 3; i++) {
+        const uint32_t e = v2e_id(x[i]);
+        vb += (-v2e_sign(x[i])) * P.visc_coef * ld(tb_u, e * L + k) / ld(M.dv, e);
+      }
+    st(vT, v * L + k, vb / at);
+    st(hvT, v * L + k, -pvvb * pvv / hv / at);
+  }
+}
+
+// Cell part: h and tracer adjoints of the stage input, ke and divergence
+// adjoints for the edges, then the bookkeeping: X_s of stages 1-3 was
+// formed from the step's initial state and the stage s-1 tendencies.
+__global__ __launch_bounds__(BS) void k_ad_cell(
+    DMesh M, AdjMesh A, AdjParams P, Rec R, Rec R0, const double *__restrict__ tb_u,
+    const double *__restrict__ heb, const double *__restrict__ hvT,
+    const double *__restrict__ r1, const double *__restrict__ r2,
+    const double *__restrict__ hbf, const double *__restrict__ sg, double *keT, double *divT,
+    double *xb_h, double *xb_tr, const double *__restrict__ aw_h,
+    const double *__restrict__ aw_tr, double *tb_h, double *tb_tr) {
+  const int c = xcd_item();
+  if (c >= M.nC) return;
+  const uint32_t L = M.nL, C = M.nC, NT = P.nT;
+  const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const double ar = ld(M.area, c);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t ol = c * L + k;
+    double g1 = 0.0, hsum = 0.0, Tb[8] = {};
+    for (int j = 0; j < n; j++) {
+      const uint32_t e = ld(M.c2e, j * C + c), oe = e * L + k;
+      if (!P.freeze_u) g1 += sign_on_cell(neg, j) * ld(tb_u, oe) / ld(M.dc, e);
+      hsum += 0.5 * ld(heb, oe);
+      const bool first = (neg >> j) & 1;
+      for (uint32_t t = 0; t < NT; t++) Tb[t] += first ? ld(r1, oe * NT + t) : ld(r2, oe * NT + t);
+    }
+    for (int i = 0; i < nv; i++)
+      hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
+    for (int i = 0; i < na; i++) {
+      const uint32_t e = ld(A.at, i * C + c), oe = e * L + k;
+      const double cf = ld(A.ac1, i * C + c) + ld(A.ac3, i * C + c) * M.coef3rd * ld(sg, oe);
+      for (uint32_t t = 0; t < NT; t++) Tb[t] += ld(hbf, oe * NT + t) * cf;
+    }
+    const double hb = -M.gravity * g1 + hsum;
+    st(keT, ol, -g1 / ar);
+    if (P.use_visc) st(divT, ol, P.visc_coef * g1 / ar);
+    if (P.s >= 1) {
+      const double hs = ld(R.h, ol), h0 = ld(R0.h, ol);
+      double htot = hb;
+      for (uint32_t t = 0; t < NT; t++) htot -= Tb[t] * ld(R.tr, ol * NT + t) / hs;
+      double xh = ld(xb_h, ol) + htot;
+      for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t ot = ol * NT + t;
+        const double pb = Tb[t] / hs;
+        xh += pb * ld(R0.tr, ot);
+        st(xb_tr, ot, ld(xb_tr, ot) + pb * h0);
+        st(tb_tr, ot, P.w_rk_m * ld(aw_tr, ot) + P.w_sub_m * pb);
+      }
+      st(xb_h, ol, xh);
+      st(tb_h, ol, P.w_rk_m * ld(aw_h, ol) + P.w_sub_m * htot);
+    } else {
+      st(xb_h, ol, ld(xb_h, ol) + hb);
+      for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t ot = ol * NT + t;
+        st(xb_tr, ot, ld(xb_tr, ot) + Tb[t]);
+      }
+    }
+  }
+}
+
+// Edge part 2: the u adjoint of the stage input completed with the paths
+// through ke, the divergence and the vorticity, then the bookkeeping of u.
+__global__ __launch_bounds__(BS) void k_ad_edge_b(DMesh M, AdjParams P, Rec R,
+                                                  const double *__restrict__ ubA,
+                                                  const double *__restrict__ keT,
+                                                  const double *__restrict__ divT,
+                                                  const double *__restrict__ vT, double *xb_u,
+                                                  const double *__restrict__ aw_u, double *tb_u) {
+  const int e = xcd_item();
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL;
+  const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
+  const uint32_t c1 = cc.x, c2 = cc.y;
+  const double dc = ld(M.dc, e), dv = ld(M.dv, e);
+  for (uint32_t k = 0; k < L; k++) {
+    const uint32_t o = e * L + k;
+    double ub = ld(ubA, o) +
+                0.5 * dc * dv * ld(R.u, o) * (ld(keT, c1 * L + k) + ld(keT, c2 * L + k));
+    if (P.use_visc) ub += dv * (ld(divT, c1 * L + k) - ld(divT, c2 * L + k));
+    ub += dc * (ld(vT, uint32_t(vv.y) * L + k) - ld(vT, uint32_t(vv.x) * L + k));
+    st(xb_u, o, ld(xb_u, o) + ub);
+    if (P.s >= 1) st(tb_u, o, P.freeze_u ? 0.0 : P.w_rk_m * ld(aw_u, o) + P.w_sub_m * ub);
+  }
+}
+
+// ---- host side ----------------------------------------------------------------
+
+int dalloc(AdjState &A, size_t n, double **out) {
+  *out = nullptr;
+  if (!n) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * n) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(double) * n) +
+                               " bytes for the adjoint tape failed");
+  A.bufs.push_back(p);
+  MPAS_HIP(hipMemset(p, 0, sizeof(double) * n));
+  *out = static_cast<double *>(p);
+  return 0;
+}
+
+template <class T>
+int upload_vec(AdjState &A, const std::vector<T> &v, const T **out) {
+  *out = nullptr;
+  if (v.empty()) return 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * v.size()) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc for the transposed mesh failed");
+  A.bufs.push_back(p);
+  MPAS_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  *out = static_cast<const T *>(p);
+  return 0;
+}
+
+// A ragged list of (id, one or two values) per entity as slot-major ELL.
+struct Ragged {
+  std::vector<std::vector<int>> id;
+  std::vector<std::vector<double>> a, b;
+  explicit Ragged(int n) : id(n), a(n), b(n) {}
+  int width() const {
+    size_t w = 1;
+    for (auto &r : id) w = std::max(w, r.size());
+    return int(w);
+  }
+};
+
+int upload_ragged(AdjState &A, const Ragged &r, bool two, int *width, const uint8_t **cnt,
+                  const int **ids, const double **va, const double **vb) {
+  const int n = int(r.id.size()), w = r.width();
+  if (w > 255) MPAS_FAIL(MPAS_ESHAPE, "transposed list wider than 255 entries");
+  std::vector<uint8_t> c(n);
+  std::vector<int> ix(size_t(w) * n, 0);
+  std::vector<double> x(size_t(w) * n, 0.0), y(two ? size_t(w) * n : 0, 0.0);
+  for (int i = 0; i < n; i++) {
+    c[i] = uint8_t(r.id[i].size());
+    for (size_t j = 0; j < r.id[i].size(); j++) {
+      ix[j * n + i] = r.id[i][j];
+      x[j * n + i] = r.a[i][j];
+      if (two) y[j * n + i] = r.b[i][j];
+    }
+  }
+  *width = w;
+  int rc;
+  if ((rc = upload_vec(A, c, cnt)) || (rc = upload_vec(A, ix, ids)) || (rc = upload_vec(A, x, va)))
+    return rc;
+  if (two && (rc = upload_vec(A, y, vb))) return rc;
+  return 0;
+}
+
+int build_transposes(mpas_ctx *ctx, AdjState &A) {
+  const HostMesh &m = ctx->host;
+  const int C = m.nC, E = m.nE, V = m.nV;
+  Ragged eT(E), aT(C), vC(C);
+  for (int e2 = 0; e2 < E; e2++)
+    for (int j = 0; j < m.e2e.size(e2); j++) {
+      const int e = m.e2e.at(e2, j);
+      eT.id[e].push_back(e2);
+      eT.a[e].push_back(m.weightsOnEdge[size_t(e2) * 16 + j]);
+    }
+  if (m.fct)
+    for (int e = 0; e < E; e++)
+      for (int i = 0; i < m.nAdvCellsForEdge[e]; i++) {
+        const int c = m.advCellsForEdge[size_t(e) * 16 + i];
+        aT.id[c].push_back(e);
+        aT.a[c].push_back(m.advCoef[size_t(e) * 16 + i]);
+        aT.b[c].push_back(m.advCoef_3rd[size_t(e) * 16 + i]);
+      }
+  for (int v = 0; v < V; v++)
+    for (int i = 0; i < m.v2c.size(v); i++) {
+      const int c = m.v2c.at(v, i);
+      vC.id[c].push_back(v);
+      vC.a[c].push_back(m.kite[size_t(v) * 3 + i]);
+    }
+  AdjMesh &T = A.T;
+  int rc, w = 0;
+  if ((rc = upload_ragged(A, eT, false, &w, &T.nEt, &T.et, &T.wt, nullptr))) return rc;
+  if ((rc = upload_ragged(A, vC, false, &w, &T.nVc, &T.vc, &T.vk, nullptr))) return rc;
+  if (m.fct && (rc = upload_ragged(A, aT, true, &w, &T.nAt, &T.at, &T.ac1, &T.ac3))) return rc;
+  return 0;
+}
+
+int alloc_rec(AdjState &A, const DMesh &M, Rec &r) {
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
+  const size_t T = C * M.nT;
+  int rc;
+  if ((rc = dalloc(A, E, &r.u)) || (rc = dalloc(A, C, &r.h)) || (rc = dalloc(A, T, &r.tr)) ||
+      (rc = dalloc(A, E, &r.he)) || (rc = dalloc(A, C, &r.ke)) || (rc = dalloc(A, E, &r.pve)) ||
+      (rc = dalloc(A, V, &r.vort)) || (rc = dalloc(A, C, &r.div)) ||
+      (rc = dalloc(A, 2 * T, &r.rfac)) || (rc = dalloc(A, E * M.nT, &r.anti)))
+    return rc;
+  return 0;
+}
+
+int adj_alloc(mpas_ctx *ctx, int K) {
+  const DMesh &M = ctx->dm;
+  auto *A = new AdjState();
+  ctx->adj = A;
+  A->K = K;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
+  const size_t T = C * M.nT, ET = E * M.nT;
+  int rc;
+  if ((rc = build_transposes(ctx, *A))) return rc;
+  A->rec.resize(size_t(4) * K + 1);
+  for (Rec &r : A->rec)
+    if ((rc = alloc_rec(*A, M, r))) return rc;
+  double **e_bufs[] = {&A->nu, &A->dnu, &A->dhe, &A->dpve, &A->aw_u, &A->tb_u, &A->ubA,
+                       &A->heb, &A->pveb, &A->sg, &A->xb_u[0], &A->xb_u[1]};
+  double **c_bufs[] = {&A->nh, &A->dnh, &A->dke, &A->ddiv, &A->aw_h, &A->tb_h, &A->keT,
+                       &A->divT, &A->xb_h[0], &A->xb_h[1]};
+  double **t_bufs[] = {&A->ntr, &A->tlo, &A->dQ, &A->aw_tr, &A->tb_tr, &A->xb_tr[0],
+                       &A->xb_tr[1]};
+  double **et_bufs[] = {&A->dflux, &A->r1, &A->r2, &A->hbf};
+  double **v_bufs[] = {&A->pv_vertex, &A->dvort, &A->dpvv, &A->vT, &A->hvT};
+  for (auto b : e_bufs)
+    if ((rc = dalloc(*A, E, b))) return rc;
+  for (auto b : c_bufs)
+    if ((rc = dalloc(*A, C, b))) return rc;
+  for (auto b : t_bufs)
+    if ((rc = dalloc(*A, T, b))) return rc;
+  for (auto b : et_bufs)
+    if ((rc = dalloc(*A, ET, b))) return rc;
+  for (auto b : v_bufs)
+    if ((rc = dalloc(*A, V, b))) return rc;
+  for (int s = 0; s < 5; s++)
+    if ((rc = dalloc(*A, E, &A->du[s])) || (rc = dalloc(*A, C, &A->dh[s])) ||
+        (rc = dalloc(*A, T, &A->dT[s])))
+      return rc;
+  return 0;
+}
+
+int copy_d2d(mpas_ctx *ctx, double *dst, const double *src, size_t n) {
+  if (!n) return 0;
+  MPAS_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+
+// State (u, h, tr) into record r and the diagnostics of it.
+int restore(mpas_ctx *ctx, AdjState &A, Rec &r, const double *u, const double *h,
+            const double *tr) {
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL;
+  int rc;
+  if ((rc = copy_d2d(ctx, r.u, u, E)) || (rc = copy_d2d(ctx, r.h, h, C)) ||
+      (rc = copy_d2d(ctx, r.tr, tr, C * M.nT)))
+    return rc;
+  return diagnostics_into(ctx, r.u, r.h, r.ke, r.div, r.vort, A.pv_vertex, r.he, r.pve);
+}
+
+// One forward step from record 4i into records 4i+1 .. 4i+4.
+int record_step(mpas_ctx *ctx, AdjState &A, int i, double dt) {
+  const Rec &r0 = A.rec[4 * i];
+  for (int s = 0; s < 4; s++) {
+    const Rec &x = A.rec[4 * i + s], &y = A.rec[4 * i + s + 1];
+    StageIO S = stage_params(ctx, s, dt);
+    S.xu = x.u;
+    S.xh = x.h;
+    S.xtr = x.tr;
+    S.h_edge = x.he;
+    S.ke = x.ke;
+    S.pv_edge = x.pve;
+    S.vort = x.vort;
+    S.div = x.div;
+    S.h_s = ctx->sw->h_s;
+    S.u0 = r0.u;
+    S.h0 = r0.h;
+    S.tr0 = r0.tr;
+    S.nu = A.nu;
+    S.nh = A.nh;
+    S.ntr = A.ntr;
+    S.yu = y.u;
+    S.yh = y.h;
+    S.ytr = y.tr;
+    S.ke_y = y.ke;
+    S.div_y = y.div;
+    S.anti = x.anti;
+    S.rfac = x.rfac;
+    S.tlo = A.tlo;
+    if (int rc = launch_stage(ctx, S, y.vort, A.pv_vertex, y.he, y.pve)) return rc;
+  }
+  return 0;
+}
+
+AdjParams adj_params(const mpas_ctx *ctx, int s, double dt) {
+  const DMesh &M = ctx->dm;
+  const mpas_params &Pm = ctx->params;
+  AdjParams P{};
+  P.s = s;
+  P.nT = M.nT;
+  P.freeze_u = Pm.freeze_u;
+  P.use_visc = Pm.visc2 != 0.0;
+  P.use_hmix = M.nT && Pm.kappa2 != 0.0;
+  P.fct = M.nT && Pm.tracer_scheme == MPAS_TRACER_FCT;
+  P.w_rk = rk_weight(s, dt);
+  P.w_sub = rk_substep_weight(s, dt);
+  P.w_rk_m = s > 0 ? rk_weight(s - 1, dt) : 0.0;
+  P.w_sub_m = s > 0 ? rk_substep_weight(s - 1, dt) : 0.0;
+  P.visc_coef = M.msd * Pm.visc2;
+  P.kappa = Pm.kappa2;
+  return P;
+}
+
+#define ADJ_LAUNCH(kernel, n, ...)                                                         \
+  do {                                                                                     \
+    if (int rc_ = prof_begin(ctx, #kernel, 0.0)) return rc_;                               \
+    hipLaunchKernelGGL(kernel, dim3(xcd_grid((n), BS)), dim3(BS), 0, ctx->stream,          \
+                       __VA_ARGS__);                                                       \
+    if (int rc_ = after_launch(#kernel)) return rc_;                                       \
+    if (int rc_ = prof_end(ctx)) return rc_;                                               \
+  } while (0)
+
+// TLM of one step along records 0..4: dX[0] -> dX[4].
+int tlm_step(mpas_ctx *ctx, AdjState &A, double dt) {
+  const DMesh &M = ctx->dm;
+  for (int s = 0; s < 4; s++) {
+    const AdjParams P = adj_params(ctx, s, dt);
+    const Rec &R = A.rec[s], &Ry = A.rec[s + 1], &R0 = A.rec[0];
+    ADJ_LAUNCH(k_tl_vertex, M.nV, M, P, R, A.du[s], A.dh[s], A.dvort, A.dpvv);
+    ADJ_LAUNCH(k_tl_edge_a, M.nE, M, P, R, A.du[s], A.dh[s], A.dT[s], A.dpvv, A.dhe, A.dpve,
+               A.dflux);
+    ADJ_LAUNCH(k_tl_cell, M.nC, M, P, R, Ry, R0, A.du[s], A.dh[0], A.dT[0], A.dhe, A.dflux,
+               A.dnh, A.dQ, A.dke, A.ddiv, A.dh[s + 1], A.dT[s + 1]);
+    ADJ_LAUNCH(k_tl_edge_b, M.nE, M, P, R, A.du[s], A.du[0], A.dh[s], A.dhe, A.dpve, A.dke,
+               A.ddiv, A.dvort, A.dnu, A.du[s + 1]);
+  }
+  return 0;
+}
+
+// Adjoint of step i of the tape (records 4i .. 4i+4): the adjoint of its
+// final state in xb[src] -> the adjoint of its initial state in xb[1-src].
+int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
+  const DMesh &M = ctx->dm;
+  const int dst = 1 - src;
+  const Rec &R0 = A.rec[4 * i], &Rf = A.rec[4 * i + 4];
+  {
+    const AdjParams P = adj_params(ctx, 3, dt);
+    ADJ_LAUNCH(k_ad_prep, std::max(M.nE, M.nC), M, P, Rf, R0, A.xb_u[src], A.xb_h[src],
+               A.xb_tr[src], A.xb_u[dst], A.xb_h[dst], A.xb_tr[dst], A.aw_u, A.aw_h, A.aw_tr,
+               A.tb_u, A.tb_h, A.tb_tr);
+  }
+  for (int s = 3; s >= 0; s--) {
+    const AdjParams P = adj_params(ctx, s, dt);
+    const Rec &R = A.rec[4 * i + s];
+    ADJ_LAUNCH(k_ad_edge_a, M.nE, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr, A.ubA, A.heb, A.pveb,
+               A.r1, A.r2, A.hbf, A.sg);
+    ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
+    ADJ_LAUNCH(k_ad_cell, M.nC, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT, A.r1, A.r2, A.hbf, A.sg,
+               A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
+    ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
+               A.tb_u);
+  }
+  return 0;
+}
+
+int check_adj(mpas_ctx *ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->adj) MPAS_FAIL(MPAS_ESTATE, "mpas_adj_init has not been called");
+  if (!ctx->sw || !ctx->sw->ready)
+    MPAS_FAIL(MPAS_ESTATE, "upload a state first (mpas_sw_state_upload sets h_s)");
+  return 0;
+}
+
+int check_state_ptrs(const mpas_ctx *ctx, const void *u, const void *h, const void *tr,
+                     const char *what) {
+  if (!u || !h || (ctx->dm.nT && !tr))
+    MPAS_FAIL(MPAS_EINVAL, std::string(what) + ": NULL state pointer");
+  return 0;
+}
+
+}  // namespace
+}  // namespace mpasx
+
+using namespace mpasx;
+
+extern "C" {
+
+int mpas_adj_init(mpas_ctx *ctx, int checkpoint_every) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (checkpoint_every < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_adj_init: checkpoint_every must be >= 1");
+  const DMesh &M = ctx->dm;
+  const mpas_params &P = ctx->params;
+  if (M.nT && P.tracer_scheme != MPAS_TRACER_FCT)
+    MPAS_FAIL(MPAS_EINVAL, "the adjoint covers the FCT tracer scheme (or no tracers)");
+  if (M.nT > 8) MPAS_FAIL(MPAS_ESHAPE, "the adjoint supports at most 8 tracers");
+  if (M.nT && (M.mlc || M.hoMask || !ctx->aligned_c2c))
+    MPAS_FAIL(MPAS_EINVAL, "the adjoint needs every level active, a full high-order mask and "
+                           "cellsOnCell across edgesOnCell");
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  adj_free(ctx);
+  if (int rc = adj_alloc(ctx, checkpoint_every)) {
+    adj_free(ctx);
+    return rc;
+  }
+  MPAS_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int mpas_adj_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, const double *h0,
+                     const double *tr0, double *u1, double *h1, double *tr1) {
+  if (int rc = check_adj(ctx)) return rc;
+  if (int rc = check_state_ptrs(ctx, u0, h0, tr0, "mpas_adj_forward")) return rc;
+  if (nsteps < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_adj_forward: nsteps must be >= 1");
+  AdjState &A = *ctx->adj;
+  SwState &W = *ctx->sw;
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
+  const int nseg = (nsteps + A.K - 1) / A.K;
+  while (int(A.ck.size()) < 3 * nseg) {
+    double *u, *h, *tr;
+    int rc;
+    if ((rc = dalloc(A, E, &u)) || (rc = dalloc(A, C, &h)) || (rc = dalloc(A, std::max<size_t>(T, 1), &tr)))
+      return rc;
+    A.ck.push_back(u);
+    A.ck.push_back(h);
+    A.ck.push_back(tr);
+  }
+  int rc;
+  if ((rc = copy_d2d(ctx, W.u, u0, E)) || (rc = copy_d2d(ctx, W.h, h0, C)) ||
+      (rc = copy_d2d(ctx, W.tr, tr0, T)))
+    return rc;
+  W.cur = 0;
+  W.elapsed = 0;
+  if ((rc = sw_diagnostics(ctx, W.u, W.h, W.cur))) return rc;
+  for (int m = 0; m < nseg; m++) {
+    const int len = std::min(A.K, nsteps - m * A.K);
+    if ((rc = copy_d2d(ctx, A.ck[3 * m], W.u, E)) || (rc = copy_d2d(ctx, A.ck[3 * m + 1], W.h, C)) ||
+        (rc = copy_d2d(ctx, A.ck[3 * m + 2], W.tr, T)))
+      return rc;
+    if ((rc = sw_rk4_steps(ctx, dt, len))) return rc;
+  }
+  if ((u1 && (rc = copy_d2d(ctx, u1, W.u, E))) || (h1 && (rc = copy_d2d(ctx, h1, W.h, C))) ||
+      (tr1 && (rc = copy_d2d(ctx, tr1, W.tr, T))))
+    return rc;
+  A.nsteps = nsteps;
+  A.dt = dt;
+  A.forward_done = true;
+  A.taped = 0;
+  return 0;
+}
+
+int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
+                      const double *tr1_bar, double *u0_bar, double *h0_bar, double *tr0_bar) {
+  if (int rc = check_adj(ctx)) return rc;
+  if (int rc = check_state_ptrs(ctx, u1_bar, h1_bar, tr1_bar, "mpas_adj_backward")) return rc;
+  if (int rc = check_state_ptrs(ctx, u0_bar, h0_bar, tr0_bar, "mpas_adj_backward")) return rc;
+  AdjState &A = *ctx->adj;
+  if (!A.forward_done) MPAS_FAIL(MPAS_ESTATE, "mpas_adj_backward before mpas_adj_forward");
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
+  int rc, src = 0;
+  if ((rc = copy_d2d(ctx, A.xb_u[0], u1_bar, E)) || (rc = copy_d2d(ctx, A.xb_h[0], h1_bar, C)) ||
+      (rc = copy_d2d(ctx, A.xb_tr[0], tr1_bar, T)))
+    return rc;
+  const int nseg = (A.nsteps + A.K - 1) / A.K;
+  for (int m = nseg - 1; m >= 0; m--) {
+    const int len = std::min(A.K, A.nsteps - m * A.K);
+    if ((rc = restore(ctx, A, A.rec[0], A.ck[3 * m], A.ck[3 * m + 1], A.ck[3 * m + 2])))
+      return rc;
+    for (int i = 0; i < len; i++)
+      if ((rc = record_step(ctx, A, i, A.dt))) return rc;
+    for (int i = len - 1; i >= 0; i--) {
+      if ((rc = adj_step(ctx, A, i, A.dt, src))) return rc;
+      src = 1 - src;
+    }
+    A.taped = len;
+  }
+  if ((rc = copy_d2d(ctx, u0_bar, A.xb_u[src], E)) || (rc = copy_d2d(ctx, h0_bar, A.xb_h[src], C)) ||
+      (rc = copy_d2d(ctx, tr0_bar, A.xb_tr[src], T)))
+    return rc;
+  return 0;
+}
+
+int mpas_tlm_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, const double *h0,
+                     const double *tr0, const double *du0, const double *dh0, const double *dtr0,
+                     double *u1, double *h1, double *tr1, double *du1, double *dh1, double *dtr1) {
+  if (int rc = check_adj(ctx)) return rc;
+  if (int rc = check_state_ptrs(ctx, u0, h0, tr0, "mpas_tlm_forward")) return rc;
+  if (int rc = check_state_ptrs(ctx, du0, dh0, dtr0, "mpas_tlm_forward")) return rc;
+  if (nsteps < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_tlm_forward: nsteps must be >= 1");
+  AdjState &A = *ctx->adj;
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
+  int rc;
+  A.taped = 0;
+  if ((rc = restore(ctx, A, A.rec[0], u0, h0, tr0))) return rc;
+  if ((rc = copy_d2d(ctx, A.du[0], du0, E)) || (rc = copy_d2d(ctx, A.dh[0], dh0, C)) ||
+      (rc = copy_d2d(ctx, A.dT[0], dtr0, T)))
+    return rc;
+  for (int step = 0; step < nsteps; step++) {
+    if ((rc = record_step(ctx, A, 0, dt))) return rc;
+    if ((rc = tlm_step(ctx, A, dt))) return rc;
+    std::swap(A.rec[0], A.rec[4]);
+    std::swap(A.du[0], A.du[4]);
+    std::swap(A.dh[0], A.dh[4]);
+    std::swap(A.dT[0], A.dT[4]);
+  }
+  const Rec &r = A.rec[0];
+  if ((u1 && (rc = copy_d2d(ctx, u1, r.u, E))) || (h1 && (rc = copy_d2d(ctx, h1, r.h, C))) ||
+      (tr1 && (rc = copy_d2d(ctx, tr1, r.tr, T))) || (du1 && (rc = copy_d2d(ctx, du1, A.du[0], E))) ||
+      (dh1 && (rc = copy_d2d(ctx, dh1, A.dh[0], C))) ||
+      (dtr1 && (rc = copy_d2d(ctx, dtr1, A.dT[0], T))))
+    return rc;
+  return 0;
+}
+
+int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0, const double *h0,
+                            const double *tr0, double *u1, double *h1, double *tr1) {
+  if (int rc = check_adj(ctx)) return rc;
+  if (int rc = check_state_ptrs(ctx, u0, h0, tr0, "mpas_adj_frozen_forward")) return rc;
+  AdjState &A = *ctx->adj;
+  if (nsteps < 1 || nsteps > A.taped)
+    MPAS_FAIL(MPAS_ESTATE, "mpas_adj_frozen_forward: the tape holds " + std::to_string(A.taped) +
+                               " steps (run mpas_adj_backward on at most checkpoint_every steps)");
+  SwState &W = *ctx->sw;
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
+  int rc;
+  if ((rc = copy_d2d(ctx, W.u, u0, E)) || (rc = copy_d2d(ctx, W.h, h0, C)) ||
+      (rc = copy_d2d(ctx, W.tr, tr0, T)))
+    return rc;
+  W.cur = 0;
+  W.elapsed = 0;
+  if ((rc = sw_diagnostics(ctx, W.u, W.h, W.cur))) return rc;
+  std::vector<double *> fr(size_t(4) * nsteps);
+  for (int i = 0; i < nsteps; i++)
+    for (int s = 0; s < 4; s++) fr[4 * i + s] = A.rec[4 * i + s].rfac;
+  if ((rc = sw_rk4_steps(ctx, A.dt, nsteps, fr.data()))) return rc;
+  if ((u1 && (rc = copy_d2d(ctx, u1, W.u, E))) || (h1 && (rc = copy_d2d(ctx, h1, W.h, C))) ||
+      (tr1 && (rc = copy_d2d(ctx, tr1, W.tr, T))))
+    return rc;
+  return 0;
+}
+
+}  // extern "C"

@@ -8,6 +8,7 @@
 #include <numeric>
 
 #include "ctx.h"
+#include "stage.h"
 
 namespace mpasx {
 
@@ -28,9 +29,6 @@ int after_launch(const char *what) {
   if (e != hipSuccess) MPAS_FAIL(MPAS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
   return 0;
 }
-
-int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot);
-int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps);
 
 namespace {
 
@@ -203,6 +201,7 @@ int build_device_mesh(mpas_ctx *ctx) {
 }
 
 int release(mpas_ctx *ctx) {
+  adj_free(ctx);
   for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
   ctx->prof.pool.clear();
   for (void *p : ctx->mesh_bufs) (void)hipFree(p);

@@ -71,6 +71,8 @@ struct Prof {
   }
 };
 
+struct AdjState;  // tape and buffers of the tangent-linear and adjoint runs
+
 }  // namespace mpasx
 
 struct mpas_host_mesh : mpasx::HostMesh {};
@@ -85,6 +87,7 @@ struct mpas_ctx {
   std::vector<void *> mesh_bufs;  // static mesh arrays
   std::set<void *> user_bufs;     // mpas_buf_alloc
   std::unique_ptr<mpasx::SwState> sw;
+  mpasx::AdjState *adj = nullptr;  // mpas_adj_init; freed by adj_free
   bool aligned_c2c = false;       // cellsOnCell[j] is across edgesOnCell[j]
   mpasx::Prof prof;
 };
@@ -103,5 +106,8 @@ int after_launch(const char *what);
 // otherwise; st = null means the context's stream).
 int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st = nullptr);
 int prof_end(mpas_ctx *ctx, hipStream_t st = nullptr);
+
+// Frees the tangent-linear / adjoint tape of a context (no-op without one).
+void adj_free(mpas_ctx *ctx);
 
 }  // namespace mpasx

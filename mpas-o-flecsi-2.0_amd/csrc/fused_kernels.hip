@@ -17,26 +17,9 @@
 // (init_provis, setup_timestep, timeshift) become buffer roles.
 #include "ctx.h"
 #include "ops.h"
+#include "stage.h"
 
 namespace mpasx {
-
-struct StageIO {
-  int stage, scheme, freeze_u, use_visc, use_hmix;
-  double w_rk, w_sub, dt, visc_coef, kappa;
-  // state read by the stage (the reference's provis) and its diagnostics
-  const double *xu, *xh, *xtr;
-  const double *h_edge, *ke, *pv_edge, *vort, *div;
-  const double *h_s;
-  // state at the start of the step (the reference's prev)
-  const double *u0, *h0, *tr0;
-  // RK4 accumulators (the reference's next; tracers h-weighted)
-  double *nu, *nh, *ntr;
-  // state written by the stage: next provis, or the final state at stage 3
-  double *yu, *yh, *ytr;
-  double *ke_y, *div_y;
-  // FCT intermediates
-  double *anti, *rfac, *tlo;
-};
 
 namespace {
 
@@ -419,10 +402,12 @@ __global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
           rf[2 * g + 1] = smin(1.0, smax(0.0, scaleFactor));
           tl[g] = te;
         }
+        if (!S.keep_rfac) {
 #pragma unroll
-        for (int g = 0; g < TPL; g++) {
-          const double pair[2] = {rf[2 * g], rf[2 * g + 1]};
-          st_group<2>(S.rfac, (ot + g) * 2, pair);
+          for (int g = 0; g < TPL; g++) {
+            const double pair[2] = {rf[2 * g], rf[2 * g + 1]};
+            st_group<2>(S.rfac, (ot + g) * 2, pair);
+          }
         }
         st_group<TPL>(S.tlo, ot, tl);
       }
@@ -603,7 +588,7 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
 // stage's output.  (Running the independent pairs on two streams at once was
 // measured slower on the level-8 workload: the pairs contend for L2 and HBM.)
 template <int NT, int W, int WE>
-int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
+int launch_stage_t(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
                  double *vort_y, double *pv_vertex, double *h_edge_y, double *pv_edge_y) {
   constexpr int NPL = NT > 0 ? NT / tracers_per_lane<NT>() : 1;  // lanes per entity
   const bool fct = NT > 0 && S.scheme == MPAS_TRACER_FCT;
@@ -657,8 +642,8 @@ int launch_stage(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b
 template <int NT>
 int launch_stage_w(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
                    double *vort_y, double *pv_vertex, double *h_edge_y, double *pv_edge_y) {
-  if (M.narrow) return launch_stage<NT, 6, 10>(MPAS_STAGE_ARGS);
-  return launch_stage<NT, 8, 16>(MPAS_STAGE_ARGS);
+  if (M.narrow) return launch_stage_t<NT, 6, 10>(MPAS_STAGE_ARGS);
+  return launch_stage_t<NT, 8, 16>(MPAS_STAGE_ARGS);
 }
 
 int launch_stage_nt(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes &b,
@@ -679,49 +664,66 @@ int launch_stage_nt(mpas_ctx *ctx, const DMesh &M, const StageIO &S, const Bytes
 
 }  // namespace
 
-// Diagnostics of the state in (u, h) into ping-pong slot `slot`
-// (compute_solve_diagnostics, plus the divergence the viscosity reads).
-int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot) {
+StageIO stage_params(const mpas_ctx *ctx, int s, double dt) {
   const DMesh &M = ctx->dm;
-  SwState &S = *ctx->sw;
-  hipStream_t st = ctx->stream;
-  const int use_visc = ctx->params.visc2 != 0.0;
-  hipLaunchKernelGGL(k_diag_cell, dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, u,
-                     S.ke[slot], S.div[slot], use_visc);
-  if (int rc = after_launch("k_diag_cell")) return rc;
-  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, h, u,
-                     S.vort[slot], S.pv_vertex);
-  if (int rc = after_launch("k_diag_vertex")) return rc;
-  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, h,
-                     S.pv_vertex, S.h_edge[slot], S.pv_edge[slot]);
-  return after_launch("k_diag_edge");
+  const mpas_params &P = ctx->params;
+  StageIO S{};
+  S.stage = s;
+  S.scheme = M.nT ? P.tracer_scheme : MPAS_TRACER_NONE;
+  S.freeze_u = P.freeze_u;
+  S.use_visc = P.visc2 != 0.0;
+  S.use_hmix = M.nT && P.kappa2 != 0.0;
+  S.w_rk = rk_weight(s, dt);
+  S.w_sub = rk_substep_weight(s, dt);
+  S.dt = dt;
+  S.visc_coef = M.msd * P.visc2;
+  S.kappa = P.kappa2;
+  return S;
 }
 
-// nsteps of apps/sw/run.cc:13-70.
-int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
+int launch_stage(mpas_ctx *ctx, const StageIO &S, double *vort_y, double *pv_vertex,
+                 double *h_edge_y, double *pv_edge_y) {
   const DMesh &M = ctx->dm;
-  SwState &W = *ctx->sw;
-  const mpas_params &P = ctx->params;
-  const double rk_weights[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
-  const double rk_substep_weights[4] = {dt / 2, dt / 2, dt, 0};
   const HostMesh &hm = ctx->host;
   const double sum_eoe = double(hm.e2e.idx.size()), sum_eoc = double(hm.c2e.idx.size());
   double sum_adv = 0;
-  for (int x : hm.nAdvCellsForEdge) sum_adv += x;
+  if (S.scheme == MPAS_TRACER_FCT)
+    for (int x : hm.nAdvCellsForEdge) sum_adv += x;
+  const Bytes b = stage_bytes(M, S, sum_eoe, sum_adv, sum_eoc);
+  return launch_stage_nt(ctx, M, S, b, vort_y, pv_vertex, h_edge_y, pv_edge_y);
+}
+
+int diagnostics_into(mpas_ctx *ctx, const double *u, const double *h, double *ke, double *div,
+                     double *vort, double *pv_vertex, double *h_edge, double *pv_edge) {
+  const DMesh &M = ctx->dm;
+  hipStream_t st = ctx->stream;
+  const int use_visc = ctx->params.visc2 != 0.0;
+  hipLaunchKernelGGL(k_diag_cell, dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, u, ke, div,
+                     use_visc);
+  if (int rc = after_launch("k_diag_cell")) return rc;
+  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, h, u, vort,
+                     pv_vertex);
+  if (int rc = after_launch("k_diag_vertex")) return rc;
+  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, h, pv_vertex,
+                     h_edge, pv_edge);
+  return after_launch("k_diag_edge");
+}
+
+// Diagnostics of the state in (u, h) into ping-pong slot `slot`
+// (compute_solve_diagnostics, plus the divergence the viscosity reads).
+int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot) {
+  SwState &S = *ctx->sw;
+  return diagnostics_into(ctx, u, h, S.ke[slot], S.div[slot], S.vort[slot], S.pv_vertex,
+                          S.h_edge[slot], S.pv_edge[slot]);
+}
+
+// nsteps of apps/sw/run.cc:13-70.
+int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps, double *const *frozen_rfac) {
+  SwState &W = *ctx->sw;
   for (int step = 0; step < nsteps; step++) {
     for (int s = 0; s < 4; s++) {
       const int cur = W.cur, nxt = 1 - W.cur;
-      StageIO S{};
-      S.stage = s;
-      S.scheme = M.nT ? P.tracer_scheme : MPAS_TRACER_NONE;
-      S.freeze_u = P.freeze_u;
-      S.use_visc = P.visc2 != 0.0;
-      S.use_hmix = M.nT && P.kappa2 != 0.0;
-      S.w_rk = rk_weights[s];
-      S.w_sub = rk_substep_weights[s];
-      S.dt = dt;
-      S.visc_coef = M.msd * P.visc2;
-      S.kappa = P.kappa2;
+      StageIO S = stage_params(ctx, s, dt);
       S.xu = s == 0 ? W.u : W.pu[cur];
       S.xh = s == 0 ? W.h : W.ph[cur];
       S.xtr = s == 0 ? W.tr : W.ptr[cur];
@@ -743,11 +745,10 @@ int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
       S.ke_y = W.ke[nxt];
       S.div_y = W.div[nxt];
       S.anti = W.anti;
-      S.rfac = W.rfac;
+      S.rfac = frozen_rfac ? frozen_rfac[4 * step + s] : W.rfac;
+      S.keep_rfac = frozen_rfac != nullptr;
       S.tlo = W.tlo;
-      const Bytes b = stage_bytes(M, S, sum_eoe, sum_adv, sum_eoc);
-      if (int rc = launch_stage_nt(ctx, M, S, b, W.vort[nxt], W.pv_vertex, W.h_edge[nxt],
-                                   W.pv_edge[nxt]))
+      if (int rc = launch_stage(ctx, S, W.vort[nxt], W.pv_vertex, W.h_edge[nxt], W.pv_edge[nxt]))
         return rc;
       W.cur = nxt;
     }

@@ -345,3 +345,38 @@ class Context:
 
     def rk4_step(self, dt, nsteps=1):
         L.check(L.hip().mpas_rk4_step(self._h, float(dt), int(nsteps)), "mpas_rk4_step")
+
+    # ---- tangent-linear model and adjoint (device arrays in, device arrays out) --
+    def state_arrays(self):
+        """Three new device arrays shaped like the state (u, h, tracers)."""
+        return (self.alloc(self.nEdges * self.nLevels), self.alloc(self.nCells * self.nLevels),
+                self.alloc(max(1, self.nCells * self.nLevels * self.nTracers)))
+
+    def _trp(self, x):
+        return _ptr(x) if self.nTracers else None
+
+    def adj_init(self, checkpoint_every):
+        L.check(L.hip().mpas_adj_init(self._h, int(checkpoint_every)), "mpas_adj_init")
+
+    def adj_forward(self, dt, nsteps, x0, x1=(None, None, None)):
+        """x0 = (u0, h0, tr0) device arrays; the final state goes to x1."""
+        L.check(L.hip().mpas_adj_forward(self._h, float(dt), int(nsteps), _ptr(x0[0]),
+                                         _ptr(x0[1]), self._trp(x0[2]), _ptr(x1[0]),
+                                         _ptr(x1[1]), self._trp(x1[2])), "mpas_adj_forward")
+
+    def adj_backward(self, xbar1, xbar0):
+        L.check(L.hip().mpas_adj_backward(self._h, _ptr(xbar1[0]), _ptr(xbar1[1]),
+                                          self._trp(xbar1[2]), _ptr(xbar0[0]), _ptr(xbar0[1]),
+                                          self._trp(xbar0[2])), "mpas_adj_backward")
+
+    def tlm_forward(self, dt, nsteps, x0, dx0, x1=(None, None, None), dx1=(None, None, None)):
+        L.check(L.hip().mpas_tlm_forward(self._h, float(dt), int(nsteps), _ptr(x0[0]),
+                                         _ptr(x0[1]), self._trp(x0[2]), _ptr(dx0[0]),
+                                         _ptr(dx0[1]), self._trp(dx0[2]), _ptr(x1[0]),
+                                         _ptr(x1[1]), self._trp(x1[2]), _ptr(dx1[0]),
+                                         _ptr(dx1[1]), self._trp(dx1[2])), "mpas_tlm_forward")
+
+    def adj_frozen_forward(self, nsteps, x0, x1):
+        L.check(L.hip().mpas_adj_frozen_forward(self._h, int(nsteps), _ptr(x0[0]), _ptr(x0[1]),
+                                                self._trp(x0[2]), _ptr(x1[0]), _ptr(x1[1]),
+                                                self._trp(x1[2])), "mpas_adj_frozen_forward")

@@ -1,0 +1,195 @@
+"""Tangent-linear model (TLM) and discrete adjoint of the fused RK4 step
+(FCT limiter factors frozen at the forward pass), through the C ABI.
+
+How they are pinned (the reference has no derivative code):
+  * dot-product test: <M dx, y> = <dx, M^T y> for random dx, y, over several
+    checkpoint segments (tolerance 1e-10 relative);
+  * the TLM against central finite differences of the frozen-limiter
+    forward (the map it differentiates) and, with no tracers, of the plain
+    forward;
+  * the frozen-limiter replay at the unperturbed state reproduces the
+    forward bit for bit (the tape holds the forward's own limiter factors);
+  * the adjoint through a torch.autograd.Function wrapper (used only here):
+    the gradient of an observation loss against the TLM's directional
+    derivative of the same loss.
+"""
+import numpy as np
+import pytest
+
+from conftest import bitwise_equal, mesh
+
+pytestmark = pytest.mark.gpu
+
+DOT_TOL = 1e-10
+
+
+def _setup(level, nT, K, visc2=1e5, kappa2=1e3, seed=1):
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(level)
+    ctx = Context(m, nTracers=nT, tracer_scheme=TRACER_FCT if nT else 0, visc2=visc2,
+                  kappa2=kappa2)
+    u, h, tr = sw_workload_state(m, nT, seed=seed)
+    ctx.sw_upload(h, u, tr.ravel() if nT else None, np.full(m.nCells, 4500.0))
+    ctx.adj_init(K)
+    x0 = ctx.state_arrays()
+    x0[0].upload(u)
+    x0[1].upload(h)
+    if nT:
+        x0[2].upload(tr.ravel())
+    return m, ctx, x0, (u, h, tr.ravel())
+
+
+def _rand_state(ctx, rng, scale=(1.0, 1.0, 0.01)):
+    n = (ctx.nEdges, ctx.nCells, ctx.nCells * ctx.nTracers)
+    host = [scale[i] * rng.standard_normal(n[i]) for i in range(3)]
+    dev = ctx.state_arrays()
+    for i in range(3):
+        if n[i]:
+            dev[i].upload(host[i])
+    return host, dev
+
+
+def _get(ctx, x):
+    out = [x[0].numpy(), x[1].numpy()]
+    out.append(x[2].numpy() if ctx.nTracers else np.zeros(0))
+    return out
+
+
+def _dot(a, b):
+    return sum(float(np.dot(x, y)) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("nT,nsteps,K", [(0, 7, 3), (1, 10, 4), (2, 6, 6)])
+def test_dot_product(nT, nsteps, K):
+    m, ctx, x0, _ = _setup(4, nT, K)
+    rng = np.random.default_rng(7 + nT)
+    dt = 600.0
+    dx_h, dx = _rand_state(ctx, rng)
+    y_h, y = _rand_state(ctx, rng)
+    dxN = ctx.state_arrays()
+    ctx.tlm_forward(dt, nsteps, x0, dx, dx1=dxN)
+    ctx.adj_forward(dt, nsteps, x0)
+    xb = ctx.state_arrays()
+    ctx.adj_backward(y, xb)
+    ctx.sync()
+    lhs = _dot(_get(ctx, dxN), y_h)
+    rhs = _dot(dx_h, _get(ctx, xb))
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"dot test nT={nT} steps={nsteps} K={K}: <M dx, y> = {lhs:.16e}, "
+          f"<dx, M^T y> = {rhs:.16e}, rel {rel:.2e}")
+    assert rel < DOT_TOL
+    ctx.close()
+
+
+def test_tlm_final_state_equals_forward():
+    """The TLM's own forward trajectory is the fused forward's, bit for bit."""
+    m, ctx, x0, _ = _setup(3, 2, 3)
+    rng = np.random.default_rng(3)
+    _, dx = _rand_state(ctx, rng)
+    x1t, x1f = ctx.state_arrays(), ctx.state_arrays()
+    ctx.tlm_forward(600.0, 5, x0, dx, x1=x1t)
+    ctx.adj_forward(600.0, 5, x0, x1f)
+    ctx.sync()
+    for a, b in zip(_get(ctx, x1t), _get(ctx, x1f)):
+        assert bitwise_equal(a, b)
+    ctx.close()
+
+
+def _fd_check(level, nT, nsteps, eps, frozen):
+    """Central differences against the TLM.  With tracers the forward has
+    kinks where F = u h_edge or an antidiffusive flux crosses zero (the
+    upwind side, the 3rd-order sign, the limiter pair); the perturbation is
+    made relative to the state (u and tracers scaled by themselves) so that
+    no branch is crossed and the map is smooth over [x - eps dx, x + eps dx]."""
+    m, ctx, x0, host0 = _setup(level, nT, K=nsteps)
+    dt = 600.0
+    rng = np.random.default_rng(11)
+    dx_h, dx = _rand_state(ctx, rng)
+    if nT:
+        dx_h = [0.01 * host0[0] * dx_h[0], dx_h[1], host0[2] * dx_h[2]]
+        for i in range(3):
+            dx[i].upload(dx_h[i])
+    dxN, xN = ctx.state_arrays(), ctx.state_arrays()
+    ctx.tlm_forward(dt, nsteps, x0, dx, x1=xN, dx1=dxN)
+    ctx.adj_forward(dt, nsteps, x0)
+    if frozen:  # records the tape (segment 0 = the whole run)
+        yb, xb = ctx.state_arrays(), ctx.state_arrays()
+        ctx.adj_backward(yb, xb)
+    xp, xm, out_p, out_m = (ctx.state_arrays() for _ in range(4))
+    for i in range(3):
+        if dx_h[i].size:
+            xp[i].upload(host0[i] + eps * dx_h[i])
+            xm[i].upload(host0[i] - eps * dx_h[i])
+    if frozen:
+        ctx.adj_frozen_forward(nsteps, xp, out_p)
+        ctx.adj_frozen_forward(nsteps, xm, out_m)
+        same = ctx.state_arrays()
+        ctx.adj_frozen_forward(nsteps, x0, same)
+        ctx.sync()
+        for a, b in zip(_get(ctx, same), _get(ctx, xN)):
+            assert bitwise_equal(a, b), "frozen replay differs from the forward"
+    else:
+        ctx.adj_forward(dt, nsteps, xp, out_p)
+        ctx.adj_forward(dt, nsteps, xm, out_m)
+    ctx.sync()
+    tl = _get(ctx, dxN)
+    p, q = _get(ctx, out_p), _get(ctx, out_m)
+    rels = []
+    for name, a, b, t in zip(("u", "h", "tracers"), p, q, tl):
+        if not t.size:
+            continue
+        fd = (a - b) / (2 * eps)
+        rel = np.linalg.norm(fd - t) / np.linalg.norm(t)
+        print(f"FD vs TLM {name}: |TLM| {np.linalg.norm(t):.3e}, rel diff {rel:.2e}")
+        rels.append(rel)
+    ctx.close()
+    return rels
+
+
+def test_tlm_vs_finite_differences_sw():
+    # central differences: ~2e-10 at eps = 1e-4 (truncation and round-off)
+    assert max(_fd_check(4, 0, 6, 1e-4, frozen=False)) < 1e-6
+
+
+@pytest.mark.parametrize("eps", [1e-4, 1e-6])
+def test_tlm_vs_finite_differences_frozen_fct(eps):
+    # ~2e-9 at eps = 1e-4, ~2e-7 at 1e-6 (round-off of the differences)
+    assert max(_fd_check(4, 2, 6, eps, frozen=True)) < 1e-5
+
+
+def test_torch_autograd_observation_loss():
+    """The torch.autograd.Function wrapper (torch_adjoint_check.py) in a
+    process of its own: torch ships its own HIP runtime under the same
+    soname as ROCm's, so torch must load before the library does."""
+    import json
+    import os
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "torch_adjoint_check.py")
+    r = subprocess.run([sys.executable, script], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert res["grad_finite"] and res["rel_adjoint_vs_tlm"] < DOT_TOL
+
+
+def test_adjoint_error_paths():
+    from mpas_hip import TRACER_CENTRED, Context
+    from mpas_hip._lib import MpasError
+    m = mesh(3)
+    ctx = Context(m, nTracers=1, tracer_scheme=TRACER_CENTRED)
+    with pytest.raises(MpasError):
+        ctx.adj_init(4)  # the adjoint covers FCT tracers only
+    ctx.close()
+    m, ctx, x0, _ = _setup(3, 1, K=2)
+    xb = ctx.state_arrays()
+    with pytest.raises(MpasError):
+        ctx.adj_backward(x0, xb)  # no forward yet
+    ctx.adj_forward(600.0, 5, x0)
+    ctx.adj_backward(x0, xb)  # 3 segments: the tape ends on segment 0 (2 steps)
+    with pytest.raises(MpasError):
+        ctx.adj_frozen_forward(3, x0, xb)
+    with pytest.raises(MpasError):
+        ctx.adj_init(0)
+    ctx.close()
