@@ -181,6 +181,107 @@ def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3
     }
 
 
+def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
+                   groups=((5, 96, 480.0), (6, 128, 240.0), (7, 32, 120.0))):
+    """256-member ensemble on meshes of uneven size (96 x level 5 at dt 480 s,
+    128 x level 6 at dt 240 s, 32 x level 7 at dt 120 s), 2 FCT tracers, each
+    member with a seeded bump amplitude in [5, 20] m and u0 in [10, 30] m/s.
+    Batched: one launch per phase for all members (members of a mesh stored
+    member-minor), the step replayed from a hipGraph, the launch
+    configuration from the autotuner.  Baseline: each member run alone on its
+    mesh's context (mpas_rk4_step: the single-run launches), member after
+    member, timed over seq_steps steps per member (its cost per step is
+    constant).  The batched members equal the single runs bit for bit
+    (checked on one member per mesh)."""
+    from mpas_hip import BATCH_GRAPH, BATCH_LAUNCH, TRACER_FCT, Batch, Context, Mesh, sw_workload_state
+    t0 = time.perf_counter()
+    meshes = [Mesh(L) for L, _, _ in groups]
+    ctxs = [Context(m, nTracers=n_tr, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+            for m in meshes]
+    mesh_of, dts, states = [], [], []
+    for g, (L, n, dt) in enumerate(groups):
+        for _ in range(n):
+            i = len(mesh_of)
+            rng = np.random.default_rng(100 + i)
+            amp, u0 = rng.uniform(5.0, 20.0), rng.uniform(10.0, 30.0)
+            u, h, tr = sw_workload_state(meshes[g], n_tr, seed=1 + i, u0=u0, bump_amp=amp)
+            mesh_of.append(g)
+            dts.append(dt)
+            states.append((h, u, tr.ravel(), np.full(meshes[g].nCells, 4500.0)))
+    b = Batch(ctxs, mesh_of, dts)
+
+    def load():
+        for i, st in enumerate(states):
+            b.upload(i, *st)
+
+    load()
+    setup_s = time.perf_counter() - t0
+    log(f"  ensemble: {len(mesh_of)} members in {b.groups()} groups set up in {setup_s:.1f} s")
+    trials = b.autotune(trial_steps=tune_steps)
+    cfg = b.get_config()
+    log(f"  ensemble: autotuned {cfg} over {len(trials)} trials")
+
+    def run(nsteps, mode):
+        load()
+        b.sync()
+        t = time.perf_counter()
+        b.step(nsteps, mode)
+        b.sync()
+        return time.perf_counter() - t
+
+    run(2, BATCH_GRAPH)  # warm (graph capture)
+    t_graph = run(steps, BATCH_GRAPH)
+    log(f"  ensemble: {steps} batched steps (graph) {1e3 * t_graph / steps:.3f} ms/step")
+    t_launch = run(steps, BATCH_LAUNCH)
+    # baseline: every member alone, seq_steps steps each; bitwise check on
+    # one member per mesh against the batched run of the same length
+    probe = {mesh_of.index(g) for g in range(len(groups))}
+    run(seq_steps, BATCH_GRAPH)
+    got = {i: b.download(i) for i in probe}
+    t_seq, same = 0.0, True
+    for i, st in enumerate(states):
+        c = ctxs[mesh_of[i]]
+        c.sw_upload(*st)
+        c.sync()
+        t = time.perf_counter()
+        c.rk4_step(dts[i], seq_steps)
+        c.sync()
+        t_seq += time.perf_counter() - t
+        if i in probe:
+            same = same and all(np.array_equal(x.view(np.int64), y.view(np.int64))
+                                for x, y in zip(got[i], c.sw_download()))
+    log(f"  ensemble: members alone {1e3 * t_seq / seq_steps:.3f} ms/step")
+    cells = sum(meshes[g].nCells for g in mesh_of)
+    # counted bytes: static mesh arrays once per mesh per stage, state per member
+    model = 0.0
+    for g, (L, n, _) in enumerate(groups):
+        per = model_bytes_per_step(ctxs[g], meshes[g], n_tr)
+        state = 18 * 8 * (meshes[g].nCells * (1 + n_tr) + meshes[g].nEdges)
+        model += (per - state) + n * state
+    b.close()
+    for c in ctxs:
+        c.close()
+    ms_graph = 1e3 * t_graph / steps
+    ms_seq = 1e3 * t_seq / seq_steps
+    return {
+        "workload": "ensemble of 256 members: 96 x level 5 (10,242 cells, dt 480 s), 128 x "
+                    "level 6 (40,962 cells, dt 240 s), 32 x level 7 (163,842 cells, dt 120 s); "
+                    f"FP64 SW RK4 + FCT, {n_tr} tracers, seeded bump 5-20 m and u0 10-30 m/s",
+        "members": len(mesh_of), "member_cells": cells, "steps": steps,
+        "ms_per_step_batched_graph": ms_graph,
+        "ms_per_step_batched_launch": 1e3 * t_launch / steps,
+        "ms_per_step_sequential": ms_seq, "sequential_steps_timed": seq_steps,
+        "speedup_vs_sequential": ms_seq / ms_graph,
+        "member_cell_steps_per_s": cells * steps / t_graph,
+        "hbm_frac_model": {"bytes_per_step": model,
+                           "achieved_GBs": model / (ms_graph * 1e-3) / 1e9,
+                           "frac": model / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "bitwise_equal_to_single_runs": bool(same),
+        "autotuned_config": cfg, "autotune_trials": len(trials),
+        "setup_s": setup_s,
+    }
+
+
 def copy_calibration(nbytes):
     """k_copy over nbytes, 10 launches: reads and writes nbytes each."""
     from mpas_hip import Context, Mesh
@@ -214,6 +315,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (level 6, 200 steps, 1 FCT tracer)")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 256-member ensemble measurement")
+    ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel k_copy over BYTES, 10 times "
                          "(the known-size reference for HBM counters)")
@@ -231,6 +335,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_bench()), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench()), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -308,6 +415,12 @@ def main():
         out["adjoint"] = adjoint_bench()
         log(f"  ratio {out['adjoint']['adjoint_forward_cost_ratio']:.3f}, "
             f"dot-product rel err {out['adjoint']['dot_product_rel_err']:.2e}")
+    if not args.no_ensemble:
+        log("ensemble (256 members, 500 steps) ...")
+        out["ensemble"] = ensemble_bench()
+        log(f"  {out['ensemble']['ms_per_step_batched_graph']:.3f} ms/step batched, "
+            f"{out['ensemble']['ms_per_step_sequential']:.3f} sequential, speedup "
+            f"{out['ensemble']['speedup_vs_sequential']:.1f}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.cpu_steps,

@@ -332,6 +332,47 @@ int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0,
                             const double *h0, const double *tr0, double *u1,
                             double *h1, double *tr1);
 
+/* ---- ensembles -------------------------------------------------------------
+   No counterpart in the reference (one mesh per run there): many members,
+   on the meshes of several contexts, advanced together by the fused step.
+   Member i runs on meshes[mesh_of[i]] with time step dt[i]; every context
+   must use one device, one tracer count and scheme, and one level.  Members
+   sharing a mesh and a time step form a group stored member-minor (member
+   g is level g of one state; they must share h_s); each member's state
+   evolves exactly as mpas_rk4_step would evolve it alone. */
+typedef struct mpas_batch mpas_batch;
+enum {
+  MPAS_BATCH_GRAPH = 0, /* one launch per phase for all groups, the step
+                           captured in a hipGraph and replayed */
+  MPAS_BATCH_LAUNCH = 1 /* the same launches, issued directly */
+};
+int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers,
+                      const int32_t *mesh_of, const double *dt, mpas_batch **out);
+int mpas_batch_destroy(mpas_batch *batch);
+/* Number of member groups (distinct mesh and time step pairs). */
+int mpas_batch_groups(const mpas_batch *batch);
+/* Host arrays as mpas_sw_state_upload; runs the group's diagnostics. */
+int mpas_batch_upload(mpas_batch *batch, int member, const double *h, const double *u,
+                      const double *tracers, const double *h_s);
+int mpas_batch_download(mpas_batch *batch, int member, double *h, double *u,
+                        double *tracers);
+int mpas_batch_step(mpas_batch *batch, int nsteps, int mode);
+int mpas_batch_sync(mpas_batch *batch);
+/* Launch configuration: workgroup size of the 6 phase kernels (momentum,
+   high-order flux, cells, tracer update, vertex and edge diagnostics;
+   multiples of 64 up to 512), members per lane (consecutive members of one
+   entity a lane advances), workgroup order (0 group after group, 1 the
+   groups' workgroups interleaved) and XCD-contiguous placement (0/1). */
+int mpas_batch_config(mpas_batch *batch, const int32_t *block, int members_per_lane,
+                      int order, int xcd);
+int mpas_batch_get_config(const mpas_batch *batch, int32_t *block, int *members_per_lane,
+                          int *order, int *xcd);
+/* Times trial_steps graph replays per candidate (members per lane,
+   placement and order, then each kernel's workgroup size), keeps the
+   fastest configuration and restores the members' state; writes the
+   trials as JSON into report. */
+int mpas_batch_autotune(mpas_batch *batch, int trial_steps, char *report, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

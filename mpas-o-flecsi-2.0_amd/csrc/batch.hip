@@ -1,0 +1,699 @@
+// Ensembles: many members, on meshes of uneven size, advanced together.
+// (The reference runs one mesh per process; this has no counterpart there.)
+//
+// Members that share a mesh and a time step form a group, stored
+// member-minor: member g of a group is level g of one multi-level state
+// ([entity][member], tracers [cell][member][tracer]).  Every kernel of the
+// path treats levels independently, so a member evolves bit for bit as a
+// single run of it would.  The kernels run with levels on lanes
+// (stage_kernels.h): neighbouring lanes take neighbouring members of one
+// entity, so state accesses and neighbour gathers are contiguous runs
+// across members and each stencil load is shared by the members.
+//
+// Every phase of an RK stage (momentum, high-order flux, cells, tracer
+// update, vertex and edge diagnostics) is ONE launch over all groups: a
+// table maps each logical workgroup to its group and block, and the
+// group's mesh view and stage descriptors are read from device memory.  A
+// step is 24 launches, captured once in a hipGraph and replayed.
+//
+// Tunables (mpas_batch_autotune picks them by timing graph replays): the
+// workgroup size of each of the 6 kernels, members per lane, the order of
+// the groups' workgroups in the launch space, and XCD-contiguous placement.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "stage_kernels.h"
+
+namespace mpasx {
+
+// Device-side descriptor of one group.
+struct BatchGroup {
+  DMesh M;       // the mesh context's view with nL = members of the group
+  StageIO S[4];  // stage s of a step (the ping-pong roles repeat every step)
+  double *vort_y[4], *h_edge_y[4], *pv_edge_y[4];
+  double *pv_vertex;
+};
+
+// Buffers of one group (the SwState fields, nL = members).
+struct GroupBufs {
+  double *u, *h, *tr, *nu, *nh, *ntr, *pu[2], *ph[2], *ptr[2];
+  double *h_edge[2], *ke[2], *pv_edge[2], *vort[2], *div[2], *pv_vertex, *h_s;
+  double *anti, *rfac, *tlo;
+};
+
+struct GroupInfo {
+  int ctx = 0;   // mesh context
+  double dt = 0;
+  std::vector<int> members;
+  std::vector<double> h_s;  // bottom topography shared by the group (set by the first upload)
+  GroupBufs b{};
+};
+
+constexpr int kBatchKernels = 6;  // edges, ho, cells, tracer, vertex, edge_diag
+constexpr int kBatchMaxBlock = 512;
+
+struct BatchArgs {
+  const BatchGroup *__restrict__ grp;
+  const int2 *__restrict__ tab;  // logical workgroup -> (group, its block); group -1: padding
+  int xcd;                       // XCD-contiguous placement
+};
+
+}  // namespace mpasx
+
+struct mpas_batch {
+  std::vector<mpas_ctx *> meshes;
+  int nmem = 0, nT = 0, narrow = 1, device = 0;
+  std::vector<int> grp_of, idx_in_grp;  // per member
+  std::vector<mpasx::GroupInfo> groups;
+  std::vector<mpasx::BatchGroup> desc;
+  std::vector<int> ready;
+  hipStream_t stream = nullptr;
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  double *staging = nullptr;  // one member's largest field (uploads, downloads)
+  mpasx::BatchGroup *d_grp = nullptr;
+  // launch configuration and its tables
+  int block[mpasx::kBatchKernels] = {256, 256, 256, 256, 256, 256};
+  int lpl = 1;    // members per lane
+  int order = 0;  // 0: group after group, 1: the groups' workgroups interleaved
+  int xcd = 1;
+  int2 *d_tab[mpasx::kBatchKernels] = {};
+  int grid[mpasx::kBatchKernels] = {};
+  bool tables_ok = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace mpasx {
+namespace {
+
+__device__ __forceinline__ int batch_block(const BatchArgs &A) {
+  const int b = blockIdx.x;
+  if (!A.xcd) return b;
+  const int per = gridDim.x / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
+#define BATCH_PROLOGUE                                                    \
+  const int2 t = A.tab[batch_block(A)];                                   \
+  if (t.x < 0) return;                                                    \
+  const BatchGroup &g = A.grp[t.x];                                       \
+  const int item = t.y * int(blockDim.x) + int(threadIdx.x)
+
+template <int WE>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  stage_edges<WE, true>(g.M, g.S[s], item);
+}
+template <int NT, int WA>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  stage_ho<NT, WA, true>(g.M, g.S[s], item);
+}
+template <int NT, int W>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  stage_cells<NT, W, true>(g.M, g.S[s], item);
+}
+template <int NT, int W>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  stage_tracer<NT, W, true>(g.M, g.S[s], item);
+}
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_vertex(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  int sub;
+  uint32_t k0, k1;
+  const int v = split_levels<true, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
+  if (v < g.M.nV)
+    diag_vertex(g.M, v, g.S[s].yh, g.S[s].yu, nullptr, nullptr, g.vort_y[s], g.pv_vertex, k0, k1);
+}
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_edge(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  int sub;
+  uint32_t k0, k1;
+  const int e = split_levels<true, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
+  if (e < g.M.nE)
+    diag_edge(g.M, e, g.S[s].yh, g.pv_vertex, g.h_edge_y[s], g.pv_edge_y[s], k0, k1);
+}
+#undef BATCH_PROLOGUE
+
+// One member's field between its contiguous copy (n entities x w doubles)
+// and the group layout (n entities x members x w doubles).
+__global__ __launch_bounds__(256) void kb_member_scatter(double *group, const double *src,
+                                                         int64_t n, int w, int members, int idx) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * w) return;
+  const int64_t e = i / w, j = i - e * w;
+  group[(e * members + idx) * w + j] = src[i];
+}
+__global__ __launch_bounds__(256) void kb_member_gather(double *dst, const double *group,
+                                                        int64_t n, int w, int members, int idx) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * w) return;
+  const int64_t e = i / w, j = i - e * w;
+  dst[i] = group[(e * members + idx) * w + j];
+}
+
+int lanes_per_entity(int nT) { return nT > 0 ? nT / (nT % 2 == 0 ? 2 : 1) : 1; }
+
+// Work items of kernel k for group q.
+int64_t kernel_items(const mpas_batch *B, int k, int q) {
+  const GroupInfo &G = B->groups[q];
+  const DMesh &M = B->meshes[G.ctx]->dm;
+  const int64_t npl = lanes_per_entity(B->nT);
+  const int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
+  switch (k) {
+    case 0: return M.nE * nch;
+    case 1: return B->nT ? M.nE * nch * npl : 0;
+    case 2: return M.nC * nch * npl;
+    case 3: return B->nT ? M.nC * nch * npl : 0;
+    case 4: return M.nV * nch;
+    default: return M.nE * nch;
+  }
+}
+
+void free_tables(mpas_batch *B) {
+  for (int k = 0; k < kBatchKernels; k++) {
+    if (B->d_tab[k]) (void)hipFree(B->d_tab[k]);
+    B->d_tab[k] = nullptr;
+  }
+  B->tables_ok = false;
+}
+
+void drop_graph(mpas_batch *B) {
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+}
+
+// The stage descriptors of a group: the roles sw_rk4_steps gives the
+// context's buffers, with the ping-pong slot of stage s = s % 2.
+void describe_group(mpas_batch *B, int q) {
+  const GroupInfo &G = B->groups[q];
+  const GroupBufs &W = G.b;
+  mpas_ctx *ctx = B->meshes[G.ctx];
+  BatchGroup &d = B->desc[q];
+  d.M = ctx->dm;
+  d.M.nL = int(G.members.size());
+  d.pv_vertex = W.pv_vertex;
+  for (int s = 0; s < 4; s++) {
+    const int cur = s % 2, nxt = 1 - cur;
+    StageIO S = stage_params(ctx, s, G.dt);
+    S.lpl = B->lpl;
+    S.nchunk = (d.M.nL + B->lpl - 1) / B->lpl;
+    S.xu = s == 0 ? W.u : W.pu[cur];
+    S.xh = s == 0 ? W.h : W.ph[cur];
+    S.xtr = s == 0 ? W.tr : W.ptr[cur];
+    S.h_edge = W.h_edge[cur];
+    S.ke = W.ke[cur];
+    S.pv_edge = W.pv_edge[cur];
+    S.vort = W.vort[cur];
+    S.div = W.div[cur];
+    S.h_s = W.h_s;
+    S.u0 = W.u;
+    S.h0 = W.h;
+    S.tr0 = W.tr;
+    S.nu = W.nu;
+    S.nh = W.nh;
+    S.ntr = W.ntr;
+    S.yu = s < 3 ? W.pu[nxt] : W.u;
+    S.yh = s < 3 ? W.ph[nxt] : W.h;
+    S.ytr = s < 3 ? W.ptr[nxt] : W.tr;
+    S.ke_y = W.ke[nxt];
+    S.div_y = W.div[nxt];
+    S.anti = W.anti;
+    S.rfac = W.rfac;
+    S.keep_rfac = 0;
+    S.tlo = W.tlo;
+    d.S[s] = S;
+    d.vort_y[s] = W.vort[nxt];
+    d.h_edge_y[s] = W.h_edge[nxt];
+    d.pv_edge_y[s] = W.pv_edge[nxt];
+  }
+}
+
+// Descriptors (they hold members per lane) and workgroup tables.
+int build_tables(mpas_batch *B) {
+  drop_graph(B);
+  free_tables(B);
+  for (size_t q = 0; q < B->groups.size(); q++) describe_group(B, int(q));
+  MPAS_HIP(hipMemcpy(B->d_grp, B->desc.data(), sizeof(BatchGroup) * B->desc.size(),
+                     hipMemcpyHostToDevice));
+  const int ng = int(B->groups.size());
+  for (int k = 0; k < kBatchKernels; k++) {
+    const int bs = B->block[k];
+    std::vector<int> nb(ng);
+    for (int q = 0; q < ng; q++) nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
+    std::vector<int2> tab;
+    if (B->order == 1) {  // round-robin over the groups' blocks
+      for (int j = 0, more = 1; more; j++) {
+        more = 0;
+        for (int q = 0; q < ng; q++)
+          if (j < nb[q]) tab.push_back(make_int2(q, j)), more = 1;
+      }
+    } else {
+      for (int q = 0; q < ng; q++)
+        for (int j = 0; j < nb[q]; j++) tab.push_back(make_int2(q, j));
+    }
+    while (tab.empty() || tab.size() % kXcds) tab.push_back(make_int2(-1, 0));
+    B->grid[k] = int(tab.size());
+    MPAS_HIP(hipMalloc(&B->d_tab[k], sizeof(int2) * tab.size()));
+    MPAS_HIP(hipMemcpy(B->d_tab[k], tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
+  }
+  B->tables_ok = true;
+  return 0;
+}
+
+BatchArgs args_of(const mpas_batch *B, int k) { return BatchArgs{B->d_grp, B->d_tab[k], B->xcd}; }
+
+template <int NT, int W, int WE>
+int launch_batch_stage_t(mpas_batch *B, int s) {
+  hipStream_t st = B->stream;
+  const bool fct = NT > 0 && B->desc[0].S[0].scheme == MPAS_TRACER_FCT;
+#define BL(k, kern)                                                                     \
+  do {                                                                                  \
+    hipLaunchKernelGGL(kern, dim3(B->grid[k]), dim3(B->block[k]), 0, st, args_of(B, k), s); \
+    if (int rc = after_launch(#kern)) return rc;                                        \
+  } while (0)
+  BL(0, (kb_stage_edges<WE>));
+  if constexpr (NT > 0) {
+    if (fct) BL(1, (kb_stage_ho<NT, WE>));
+  }
+  BL(2, (kb_stage_cells<NT, W>));
+  if constexpr (NT > 0) {
+    if (fct) BL(3, (kb_stage_tracer<NT, W>));
+  }
+  BL(4, kb_diag_vertex);
+  BL(5, kb_diag_edge);
+#undef BL
+  return 0;
+}
+
+template <int NT>
+int launch_batch_stage_w(mpas_batch *B, int s) {
+  if (B->narrow) return launch_batch_stage_t<NT, 6, 10>(B, s);
+  return launch_batch_stage_t<NT, 8, 16>(B, s);
+}
+
+int launch_batch_stage(mpas_batch *B, int s) {
+  switch (B->nT) {
+    case 0: return launch_batch_stage_w<0>(B, s);
+    case 1: return launch_batch_stage_w<1>(B, s);
+    case 2: return launch_batch_stage_w<2>(B, s);
+    case 3: return launch_batch_stage_w<3>(B, s);
+    case 4: return launch_batch_stage_w<4>(B, s);
+    case 5: return launch_batch_stage_w<5>(B, s);
+    case 6: return launch_batch_stage_w<6>(B, s);
+    case 7: return launch_batch_stage_w<7>(B, s);
+    case 8: return launch_batch_stage_w<8>(B, s);
+  }
+  MPAS_FAIL(MPAS_EINVAL, "ensembles support at most 8 tracers");
+}
+
+int launch_batch_step(mpas_batch *B) {
+  for (int s = 0; s < 4; s++)
+    if (int rc = launch_batch_stage(B, s)) return rc;
+  return 0;
+}
+
+int capture_graph(mpas_batch *B) {
+  drop_graph(B);
+  MPAS_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = launch_batch_step(B);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(B->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess)
+    MPAS_FAIL(MPAS_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  B->graph = g;
+  MPAS_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+// Carves one group's buffers out of the arena (256-byte aligned).
+struct Carver {
+  char *base;
+  size_t off = 0;
+  double *take(size_t n) {
+    double *p = base ? reinterpret_cast<double *>(base + off) : nullptr;
+    off += (n * sizeof(double) + 255) / 256 * 256;
+    return p;
+  }
+};
+
+void carve_group(Carver &c, const DMesh &M, int members, int nT, GroupBufs &b) {
+  const size_t E = size_t(M.nE) * members, C = size_t(M.nC) * members;
+  const size_t V = size_t(M.nV) * members, T = C * nT;
+  b.u = c.take(E), b.nu = c.take(E), b.pu[0] = c.take(E), b.pu[1] = c.take(E);
+  b.h_edge[0] = c.take(E), b.h_edge[1] = c.take(E), b.pv_edge[0] = c.take(E), b.pv_edge[1] = c.take(E);
+  b.h = c.take(C), b.nh = c.take(C), b.ph[0] = c.take(C), b.ph[1] = c.take(C);
+  b.ke[0] = c.take(C), b.ke[1] = c.take(C), b.div[0] = c.take(C), b.div[1] = c.take(C);
+  b.h_s = c.take(size_t(M.nC));
+  b.tr = c.take(T), b.ntr = c.take(T), b.ptr[0] = c.take(T), b.ptr[1] = c.take(T), b.tlo = c.take(T);
+  b.vort[0] = c.take(V), b.vort[1] = c.take(V), b.pv_vertex = c.take(V);
+  b.anti = c.take(E * nT), b.rfac = c.take(2 * T);
+}
+
+int check_batch(const mpas_batch *B) {
+  if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
+  MPAS_HIP(hipSetDevice(B->device));
+  return 0;
+}
+
+void release_batch(mpas_batch *B) {
+  drop_graph(B);
+  free_tables(B);
+  if (B->d_grp) (void)hipFree(B->d_grp);
+  if (B->staging) (void)hipFree(B->staging);
+  if (B->arena) (void)hipFree(B->arena);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+}
+
+int run_steps(mpas_batch *B, int nsteps, int mode) {
+  if (!B->tables_ok)
+    if (int rc = build_tables(B)) return rc;
+  if (mode == MPAS_BATCH_LAUNCH) {
+    for (int step = 0; step < nsteps; step++)
+      if (int rc = launch_batch_step(B)) return rc;
+    return 0;
+  }
+  if (!B->exec)
+    if (int rc = capture_graph(B)) return rc;
+  for (int step = 0; step < nsteps; step++) MPAS_HIP(hipGraphLaunch(B->exec, B->stream));
+  return 0;
+}
+
+double time_steps(mpas_batch *B, int nsteps, int *rc) {
+  *rc = run_steps(B, 1, MPAS_BATCH_GRAPH);  // capture + warm
+  if (!*rc && hipStreamSynchronize(B->stream) != hipSuccess) *rc = MPAS_EHIP;
+  if (*rc) return 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  *rc = run_steps(B, nsteps, MPAS_BATCH_GRAPH);
+  if (!*rc && hipStreamSynchronize(B->stream) != hipSuccess) *rc = MPAS_EHIP;
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// One member's field between its host array (n entities x w doubles) and
+// the group layout, through the staging buffer: one contiguous copy and a
+// scatter / gather kernel.
+int member_copy(mpas_batch *B, int m, double *group_field, const double *host, double *host_out,
+                size_t n, size_t w) {
+  const GroupInfo &G = B->groups[B->grp_of[m]];
+  const int members = int(G.members.size()), i = B->idx_in_grp[m];
+  const int64_t cnt = int64_t(n * w);
+  const int grid = int((cnt + 255) / 256);
+  if (host) {
+    MPAS_HIP(hipMemcpyAsync(B->staging, host, sizeof(double) * cnt, hipMemcpyHostToDevice,
+                            B->stream));
+    hipLaunchKernelGGL(kb_member_scatter, dim3(grid), dim3(256), 0, B->stream, group_field,
+                       B->staging, int64_t(n), int(w), members, i);
+    return after_launch("kb_member_scatter");
+  }
+  hipLaunchKernelGGL(kb_member_gather, dim3(grid), dim3(256), 0, B->stream, B->staging,
+                     group_field, int64_t(n), int(w), members, i);
+  if (int rc = after_launch("kb_member_gather")) return rc;
+  MPAS_HIP(hipMemcpyAsync(host_out, B->staging, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                          B->stream));
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+}  // namespace
+}  // namespace mpasx
+
+using namespace mpasx;
+
+extern "C" {
+
+int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers, const int32_t *mesh_of,
+                      const double *dt, mpas_batch **out) {
+  if (!out) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_create: out is NULL");
+  *out = nullptr;
+  if (!meshes || nmeshes < 1 || nmembers < 1 || !mesh_of || !dt)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_batch_create: bad arguments");
+  auto B = std::make_unique<mpas_batch>();
+  for (int q = 0; q < nmeshes; q++) {
+    if (int rc = check_ctx(meshes[q])) return rc;
+    B->meshes.push_back(meshes[q]);
+  }
+  const mpas_ctx *c0 = meshes[0];
+  B->device = c0->device;
+  B->nT = c0->dm.nT;
+  B->narrow = c0->dm.narrow;
+  for (mpas_ctx *c : B->meshes) {
+    if (c->device != B->device) MPAS_FAIL(MPAS_EINVAL, "every mesh context must use one device");
+    if (c->dm.nT != B->nT || c->params.tracer_scheme != c0->params.tracer_scheme)
+      MPAS_FAIL(MPAS_EINVAL, "every mesh context must have the same tracers and scheme");
+    if (c->dm.narrow != B->narrow)
+      MPAS_FAIL(MPAS_EINVAL, "mesh contexts mix narrow and wide stencil widths");
+    if (c->dm.nL != 1)
+      MPAS_FAIL(MPAS_EINVAL, "ensemble mesh contexts have one level (members take the levels)");
+    if (B->nT && c->params.tracer_scheme == MPAS_TRACER_FCT &&
+        (c->dm.mlc || c->dm.hoMask || !c->aligned_c2c))
+      MPAS_FAIL(MPAS_EINVAL, "the fused FCT step needs every level active and aligned cellsOnCell");
+  }
+  B->nmem = nmembers;
+  B->grp_of.resize(nmembers);
+  B->idx_in_grp.resize(nmembers);
+  for (int m = 0; m < nmembers; m++) {
+    if (mesh_of[m] < 0 || mesh_of[m] >= nmeshes)
+      MPAS_FAIL(MPAS_EINVAL, "mpas_batch_create: mesh_of out of range");
+    if (!(dt[m] > 0)) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_create: dt must be positive");
+    int q = 0;
+    for (; q < int(B->groups.size()); q++)
+      if (B->groups[q].ctx == mesh_of[m] && B->groups[q].dt == dt[m]) break;
+    if (q == int(B->groups.size())) {
+      B->groups.emplace_back();
+      B->groups[q].ctx = mesh_of[m];
+      B->groups[q].dt = dt[m];
+    }
+    B->grp_of[m] = q;
+    B->idx_in_grp[m] = int(B->groups[q].members.size());
+    B->groups[q].members.push_back(m);
+  }
+  Carver sizer{nullptr};
+  for (GroupInfo &G : B->groups) {
+    GroupBufs dummy;
+    carve_group(sizer, B->meshes[G.ctx]->dm, int(G.members.size()), B->nT, dummy);
+  }
+  B->arena_bytes = sizer.off;
+  if (hipMalloc(&B->arena, B->arena_bytes) != hipSuccess) {
+    release_batch(B.get());
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(B->arena_bytes) +
+                               " bytes for the ensemble failed");
+  }
+  if (hipMemset(B->arena, 0, B->arena_bytes) != hipSuccess ||
+      hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&B->d_grp, sizeof(BatchGroup) * B->groups.size()) != hipSuccess) {
+    release_batch(B.get());
+    MPAS_FAIL(MPAS_EHIP, "ensemble set-up failed");
+  }
+  size_t stage_n = 1;
+  for (const GroupInfo &G : B->groups) {
+    const DMesh &M = B->meshes[G.ctx]->dm;
+    stage_n = std::max({stage_n, size_t(M.nE), size_t(M.nC) * std::max(1, B->nT)});
+  }
+  if (hipMalloc(&B->staging, sizeof(double) * stage_n) != hipSuccess) {
+    release_batch(B.get());
+    MPAS_FAIL(MPAS_EHIP, "ensemble set-up failed");
+  }
+  Carver c{static_cast<char *>(B->arena)};
+  for (GroupInfo &G : B->groups)
+    carve_group(c, B->meshes[G.ctx]->dm, int(G.members.size()), B->nT, G.b);
+  B->desc.resize(B->groups.size());
+  B->ready.assign(nmembers, 0);
+  if (int rc = build_tables(B.get())) {
+    release_batch(B.get());
+    return rc;
+  }
+  *out = B.release();
+  return 0;
+}
+
+int mpas_batch_destroy(mpas_batch *B) {
+  if (!B) return 0;
+  (void)hipSetDevice(B->device);
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  release_batch(B);
+  delete B;
+  return 0;
+}
+
+int mpas_batch_upload(mpas_batch *B, int member, const double *h, const double *u,
+                      const double *tracers, const double *h_s) {
+  if (int rc = check_batch(B)) return rc;
+  if (member < 0 || member >= B->nmem) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_upload: bad member");
+  if (!h || !u || !h_s || (B->nT && !tracers))
+    MPAS_FAIL(MPAS_EINVAL, "mpas_batch_upload: NULL array");
+  GroupInfo &G = B->groups[B->grp_of[member]];
+  const mpas_ctx *ctx = B->meshes[G.ctx];
+  const DMesh &M = ctx->dm;
+  if (G.h_s.empty()) {
+    G.h_s.assign(h_s, h_s + M.nC);
+    MPAS_HIP(hipMemcpyAsync(G.b.h_s, h_s, sizeof(double) * M.nC, hipMemcpyHostToDevice,
+                            B->stream));
+  } else if (std::memcmp(G.h_s.data(), h_s, sizeof(double) * M.nC) != 0) {
+    MPAS_FAIL(MPAS_EINVAL, "members of one mesh and time step share h_s (bottom topography)");
+  }
+  int rc;
+  if ((rc = member_copy(B, member, G.b.h, h, nullptr, M.nC, 1)) ||
+      (rc = member_copy(B, member, G.b.u, u, nullptr, M.nE, 1)) ||
+      (B->nT && (rc = member_copy(B, member, G.b.tr, tracers, nullptr, M.nC, B->nT))))
+    return rc;
+  // diagnostics of the group's state (every member: cheap, and the members
+  // not yet uploaded are recomputed when they are)
+  DMesh Mg = M;
+  Mg.nL = int(G.members.size());
+  if ((rc = diagnostics_dm(Mg, B->stream, ctx->params.visc2 != 0.0, G.b.u, G.b.h, G.b.ke[0],
+                           G.b.div[0], G.b.vort[0], G.b.pv_vertex, G.b.h_edge[0],
+                           G.b.pv_edge[0])))
+    return rc;
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  B->ready[member] = 1;
+  return 0;
+}
+
+int mpas_batch_download(mpas_batch *B, int member, double *h, double *u, double *tracers) {
+  if (int rc = check_batch(B)) return rc;
+  if (member < 0 || member >= B->nmem) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_download: bad member");
+  GroupInfo &G = B->groups[B->grp_of[member]];
+  const DMesh &M = B->meshes[G.ctx]->dm;
+  int rc;
+  if ((h && (rc = member_copy(B, member, G.b.h, nullptr, h, M.nC, 1))) ||
+      (u && (rc = member_copy(B, member, G.b.u, nullptr, u, M.nE, 1))) ||
+      (tracers && B->nT && (rc = member_copy(B, member, G.b.tr, nullptr, tracers, M.nC, B->nT))))
+    return rc;
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+int mpas_batch_step(mpas_batch *B, int nsteps, int mode) {
+  if (int rc = check_batch(B)) return rc;
+  if (mode != MPAS_BATCH_GRAPH && mode != MPAS_BATCH_LAUNCH)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_batch_step: unknown mode");
+  for (int m = 0; m < B->nmem; m++)
+    if (!B->ready[m]) MPAS_FAIL(MPAS_ESTATE, "member " + std::to_string(m) + " has no state");
+  if (nsteps < 0) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_step: nsteps < 0");
+  return run_steps(B, nsteps, mode);
+}
+
+int mpas_batch_sync(mpas_batch *B) {
+  if (int rc = check_batch(B)) return rc;
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+int mpas_batch_groups(const mpas_batch *B) { return B ? int(B->groups.size()) : MPAS_EINVAL; }
+
+int mpas_batch_config(mpas_batch *B, const int32_t *block, int members_per_lane, int order,
+                      int xcd) {
+  if (int rc = check_batch(B)) return rc;
+  if (order < 0 || order > 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: order is 0 or 1");
+  if (members_per_lane < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: members_per_lane < 1");
+  if (block)
+    for (int k = 0; k < kBatchKernels; k++)
+      if (block[k] < 64 || block[k] > kBatchMaxBlock || block[k] % 64)
+        MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: block sizes are multiples of 64 up to 512");
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  if (block) std::copy(block, block + kBatchKernels, B->block);
+  B->lpl = members_per_lane;
+  B->order = order;
+  B->xcd = xcd != 0;
+  return build_tables(B);
+}
+
+int mpas_batch_get_config(const mpas_batch *B, int32_t *block, int *members_per_lane, int *order,
+                          int *xcd) {
+  if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
+  if (block) std::copy(B->block, B->block + kBatchKernels, block);
+  if (members_per_lane) *members_per_lane = B->lpl;
+  if (order) *order = B->order;
+  if (xcd) *xcd = B->xcd;
+  return 0;
+}
+
+int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap) {
+  if (int rc = check_batch(B)) return rc;
+  if (trial_steps < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_autotune: trial_steps < 1");
+  for (int m = 0; m < B->nmem; m++)
+    if (!B->ready[m]) MPAS_FAIL(MPAS_ESTATE, "member " + std::to_string(m) + " has no state");
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  // the trials advance the members: keep the state and restore it after
+  void *save = nullptr;
+  if (hipMalloc(&save, B->arena_bytes) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "mpas_batch_autotune: no memory for the state copy");
+  MPAS_HIP(hipMemcpyAsync(save, B->arena, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream));
+  std::string log = "[";
+  int rc = 0;
+  auto trial = [&](const char *what) -> double {
+    if (rc) return 1e30;
+    if ((rc = build_tables(B))) return 1e30;
+    const double t = time_steps(B, trial_steps, &rc);
+    char line[256];
+    std::snprintf(line, sizeof line,
+                  "%s{\"trial\": \"%s\", \"block\": [%d, %d, %d, %d, %d, %d], "
+                  "\"members_per_lane\": %d, \"order\": %d, \"xcd\": %d, \"ms_per_step\": %.4f}",
+                  log.size() > 1 ? ", " : "", what, B->block[0], B->block[1], B->block[2],
+                  B->block[3], B->block[4], B->block[5], B->lpl, B->order, B->xcd,
+                  1e3 * t / trial_steps);
+    log += line;
+    return rc ? 1e30 : t;
+  };
+  double best = trial("start");
+  size_t gmax = 1;
+  for (const GroupInfo &G : B->groups) gmax = std::max(gmax, G.members.size());
+  // members per lane, placement and order, then each kernel's workgroup size
+  for (int lpl : {1, 2, 4, 8, 16}) {
+    if (lpl == B->lpl || size_t(lpl) > gmax) continue;
+    const int keep = B->lpl;
+    B->lpl = lpl;
+    const double t = trial("members_per_lane");
+    if (t < best) best = t;
+    else B->lpl = keep;
+  }
+  for (int x = 0; x < 2; x++)
+    for (int o = 0; o < 2; o++) {
+      if (x == B->xcd && o == B->order) continue;
+      const int px = B->xcd, po = B->order;
+      B->xcd = x;
+      B->order = o;
+      const double t = trial("placement");
+      if (t < best) best = t;
+      else B->xcd = px, B->order = po;
+    }
+  const int sizes[] = {64, 128, 256, 512};
+  for (int k = 0; k < kBatchKernels; k++) {
+    if (kernel_items(B, k, 0) == 0) continue;
+    const int keep = B->block[k];
+    int pick = keep;
+    for (int bs : sizes) {
+      if (bs == keep) continue;
+      B->block[k] = bs;
+      const double t = trial("block");
+      if (t < best) best = t, pick = bs;
+    }
+    B->block[k] = pick;
+  }
+  if (!rc) rc = build_tables(B);
+  (void)hipMemcpyAsync(B->arena, save, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream);
+  (void)hipStreamSynchronize(B->stream);
+  (void)hipFree(save);
+  if (rc) return rc;
+  log += "]";
+  if (report && cap) {
+    if (log.size() + 1 > cap) MPAS_FAIL(MPAS_ESHAPE, "mpas_batch_autotune: report buffer too small");
+    std::memcpy(report, log.c_str(), log.size() + 1);
+  }
+  return 0;
+}
+
+}  // extern "C"

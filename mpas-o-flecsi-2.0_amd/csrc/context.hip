@@ -312,6 +312,12 @@ int mpas_ctx_create(const mpas_mesh_desc *mesh, const mpas_params *params, mpas_
   MPAS_HIP(hipGetDevice(&ctx->device));
   MPAS_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
   ctx->stream = ctx->own_stream;
+  {
+    const HostMesh &h = ctx->host;
+    ctx->sum_eoe = double(h.e2e.idx.size());
+    ctx->sum_eoc = double(h.c2e.idx.size());
+    for (int x : h.nAdvCellsForEdge) ctx->sum_adv += x;
+  }
   int rc = build_device_mesh(ctx.get());
   if (rc) {
     std::string keep = g_err;
@@ -534,6 +540,7 @@ static int prof_event(mpas_ctx *ctx, hipEvent_t *ev) {
 }
 
 int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st) {
+  if (!ctx) return 0;
   Prof &P = ctx->prof;
   if (!P.on) return 0;
   if (!st) st = ctx->stream;
@@ -546,6 +553,7 @@ int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st) 
 }
 
 int prof_end(mpas_ctx *ctx, hipStream_t st) {
+  if (!ctx) return 0;
   Prof &P = ctx->prof;
   if (!P.on) return 0;
   if (!st) st = ctx->stream;

@@ -89,6 +89,9 @@ struct mpas_ctx {
   std::unique_ptr<mpasx::SwState> sw;
   mpasx::AdjState *adj = nullptr;  // mpas_adj_init; freed by adj_free
   bool aligned_c2c = false;       // cellsOnCell[j] is across edgesOnCell[j]
+  // list lengths summed over the mesh (edgesOnEdge, edgesOnCell,
+  // advCellsForEdge): the per-launch byte counts of the fused step
+  double sum_eoe = 0, sum_eoc = 0, sum_adv = 0;
   mpasx::Prof prof;
 };
 
@@ -103,7 +106,7 @@ int check_ctx(const mpas_ctx *ctx);
 int after_launch(const char *what);
 
 // Event bracket of one launch on stream st while profiling is on (no-ops
-// otherwise; st = null means the context's stream).
+// otherwise, and for a null context; st = null means the context's stream).
 int prof_begin(mpas_ctx *ctx, const char *kernel, double bytes, hipStream_t st = nullptr);
 int prof_end(mpas_ctx *ctx, hipStream_t st = nullptr);
 

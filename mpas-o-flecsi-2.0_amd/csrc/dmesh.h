@@ -61,12 +61,7 @@ struct DMesh {
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 
-__device__ __forceinline__ int lvl_cell(const DMesh &M, int c) {
-  return M.mlc ? M.mlc[c] : M.nL;
-}
-__device__ __forceinline__ int lvl_edge(const DMesh &M, int e) {
-  return M.mlet ? M.mlet[e] : M.nL;
-}
+
 __device__ __forceinline__ double sign_on_cell(uint8_t neg, int j) {
   return ((neg >> j) & 1) ? -1.0 : 1.0;
 }
@@ -77,15 +72,33 @@ __device__ __forceinline__ double v2e_sign(int x) { return x < 0 ? -1.0 : 1.0; }
 // registers: the byte offset is formed in 32 bits, which lets the compiler
 // use the scalar-base form of global memory instructions (one VGPR per
 // address instead of two).  Every array handed to the kernels is below
-// 4 GiB (checked when the context is created).
+// 4 GiB (checked when the context is created).  The access is made in the
+// global address space explicitly: pointers that reach a kernel through a
+// descriptor in memory (the ensemble kernels) are otherwise generic, and
+// generic (flat) accesses are slower and wait on LDS traffic as well.
+// (The host compilation pass only parses these bodies; it keeps the
+// generic form, which its vector types require.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define MPAS_GLOBAL __attribute__((address_space(1)))
+#else
+#define MPAS_GLOBAL
+#endif
 template <class T>
 __device__ __forceinline__ T ld(const T *__restrict__ p, uint32_t i) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) +
-                                      i * uint32_t(sizeof(T)));
+  const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);
+  return *(const MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T)));
 }
 template <class T>
 __device__ __forceinline__ void st(T *p, uint32_t i, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + i * uint32_t(sizeof(T))) = v;
+  MPAS_GLOBAL char *b = (MPAS_GLOBAL char *)(p);
+  *(MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T))) = v;
+}
+
+__device__ __forceinline__ int lvl_cell(const DMesh &M, int c) {
+  return M.mlc ? ld(M.mlc, uint32_t(c)) : M.nL;
+}
+__device__ __forceinline__ int lvl_edge(const DMesh &M, int e) {
+  return M.mlet ? ld(M.mlet, uint32_t(e)) : M.nL;
 }
 
 // Grid-stride helpers: one work item per lane.

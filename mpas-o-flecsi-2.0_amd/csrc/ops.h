@@ -9,11 +9,12 @@ namespace mpasx {
 // 81-92): circulation gathers v1 -= dc*u / v2 += dc*u over the vertex's
 // edges in ascending id; h_vertex pairs the ascending m.cells(v) with the
 // file-order kiteAreasOnVertex (line 86).  circulation / h_vertex may be
-// null (the fused step does not keep them).
+// null (the fused step does not keep them).  Levels [lv0, lv1) (all by default).
 __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double *__restrict__ h,
                                             const double *__restrict__ u, double *h_vertex,
                                             double *circulation, double *vorticity,
-                                            double *pv_vertex) {
+                                            double *pv_vertex, uint32_t lv0 = 0,
+                                            uint32_t lv1 = ~0u) {
   const uint32_t L = M.nL, V = M.nV;
   const int x0 = ld(M.v2e, v), x1 = ld(M.v2e, V + v), x2 = ld(M.v2e, 2 * V + v);
   const uint32_t e0 = v2e_id(x0), e1 = v2e_id(x1), e2 = v2e_id(x2);
@@ -21,7 +22,7 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
   const double k0 = ld(M.kite, v), k1 = ld(M.kite, V + v), k2 = ld(M.kite, 2 * V + v);
   const double d0 = ld(M.dc, e0), d1 = ld(M.dc, e1), d2 = ld(M.dc, e2);
   const double at = ld(M.areaTri, v), f = ld(M.fV, v);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = lv0; k < min(lv1, L); k++) {
     double circ = 0.0;
     circ = circ + v2e_sign(x0) * (d0 * ld(u, e0 * L + k));
     circ = circ + v2e_sign(x1) * (d1 * ld(u, e1 * L + k));
@@ -44,12 +45,13 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
 // starts at zero and takes half of each vertex value, lower vertex id first.
 __device__ __forceinline__ void diag_edge(const DMesh &M, int e, const double *__restrict__ h,
                                           const double *__restrict__ pv_vertex,
-                                          double *h_edge, double *pv_edge) {
+                                          double *h_edge, double *pv_edge, uint32_t lv0 = 0,
+                                          uint32_t lv1 = ~0u) {
   const uint32_t L = M.nL;
   const int2 cc = ld(M.e2c, e);
   const int2 vv = ld(M.e2v, e);
   const uint32_t va = min(vv.x, vv.y), vb = max(vv.x, vv.y);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = lv0; k < min(lv1, L); k++) {
     const uint32_t o = e * L + k;
     st(h_edge, o, 0.5 * (ld(h, uint32_t(cc.x) * L + k) + ld(h, uint32_t(cc.y) * L + k)));
     double p = 0.0;

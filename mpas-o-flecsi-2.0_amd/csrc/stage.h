@@ -11,6 +11,9 @@ struct StageIO {
   // 1: the limiter factors in `rfac` are given (a frozen-limiter replay)
   // and k_stage_cells leaves them untouched
   int keep_rfac;
+  // levels per lane and level chunks per entity of the levels-on-lanes
+  // kernels (ensembles); unused by the single-run kernels
+  int lpl, nchunk;
   double w_rk, w_sub, dt, visc_coef, kappa;
   // state read by the stage (the reference's provis) and its diagnostics
   const double *xu, *xh, *xtr;
@@ -45,10 +48,18 @@ StageIO stage_params(const mpas_ctx *ctx, int s, double dt);
 int launch_stage(mpas_ctx *ctx, const StageIO &S, double *vort_y, double *pv_vertex,
                  double *h_edge_y, double *pv_edge_y);
 
+// The same on an explicit mesh and stream (no context, no profiling).
+int launch_stage_dm(const DMesh &M, hipStream_t st, const StageIO &S, double *vort_y,
+                    double *pv_vertex, double *h_edge_y, double *pv_edge_y);
+
 // compute_solve_diagnostics of (u, h) into explicit buffers (ke and the
 // divergence by cells, vorticity and pv by vertices, h_edge and pv_edge).
 int diagnostics_into(mpas_ctx *ctx, const double *u, const double *h, double *ke, double *div,
                      double *vort, double *pv_vertex, double *h_edge, double *pv_edge);
+
+int diagnostics_dm(const DMesh &M, hipStream_t st, int use_visc, const double *u,
+                   const double *h, double *ke, double *div, double *vort, double *pv_vertex,
+                   double *h_edge, double *pv_edge);
 
 // diagnostics_into the context's ping-pong slot `slot`.
 int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot);

@@ -1,0 +1,521 @@
+// Device bodies of the stage-fused RK4 kernels, one work item each: the
+// single-instance kernels (fused_kernels.hip) and the ensemble kernels
+// (batch.hip) wrap the same code, so every member of an ensemble is
+// advanced with exactly the operations of a single run.
+#pragma once
+#include "ops.h"
+#include "stage.h"
+
+namespace mpasx {
+
+template <int NT>
+__device__ __forceinline__ void load_tr(const double *__restrict__ p, uint32_t i0,
+                                        double (&v)[NT]) {
+#pragma unroll
+  for (int t = 0; t < NT; t++) v[t] = ld(p, i0 + t);
+}
+
+// TPL consecutive tracers of one entity: one 16-byte load / store per pair
+// (the offset i is even whenever TPL == 2: NT is even then).
+template <int TPL>
+__device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t i,
+                                         double (&v)[TPL]) {
+  if constexpr (TPL == 2) {
+    const double2 x = ld(reinterpret_cast<const double2 *>(p), i / 2);
+    v[0] = x.x;
+    v[1] = x.y;
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) v[g] = ld(p, i + g);
+  }
+}
+template <int TPL>
+__device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v)[TPL]) {
+  if constexpr (TPL == 2) {
+    st(reinterpret_cast<double2 *>(p), i / 2, make_double2(v[0], v[1]));
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) st(p, i + g, v[g]);
+  }
+}
+
+// Tracers handled by one lane: pairs when the count is even.
+template <int NT>
+constexpr int tracers_per_lane() {
+  return NT % 2 == 0 ? 2 : 1;
+}
+
+// Adds the live slots' values in ascending edge id, the order of the
+// reference's scatter loops: at step r the slot of rank r is selected with
+// compile-time indices only (a run-time index into a register array would
+// send the array to scratch memory).  acc -= v where the cell is the edge's
+// first cell (negative edge sign), acc += v otherwise.
+template <int W>
+__device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W],
+                                                uint32_t rank, uint8_t neg, int n) {
+#pragma unroll
+  for (int r = 0; r < W; r++) {
+    double f = 0.0;
+    bool ng = false;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const bool hit = ((rank >> (4 * j)) & 15) == uint32_t(r);
+      f = hit ? v[j] : f;
+      ng = hit ? bool((neg >> j) & 1) : ng;
+    }
+    const double x = ng ? acc - f : acc + f;
+    acc = (r < n) ? x : acc;
+  }
+  return acc;
+}
+
+// The stage kernels read every slot of a ragged list up front, with widths
+// fixed at compile time (WE edgesOnEdge, WA advection cells, W edges of a
+// cell): all index loads issue together, then all gathers, so a lane waits
+// for two memory round trips instead of two per slot.  Slots past a row's
+// count read the ELL padding (valid ids) and are masked out of every sum,
+// which still runs over the live slots in the reference's order.
+//
+// Tracer work runs NPL = NT / TPL lanes per entity, each lane handling TPL
+// consecutive tracers (two when NT is even: one 16-byte access of the
+// [cell][level][tracer] record per pair); the lanes of an entity are
+// adjacent and the first of them writes the entity's non-tracer outputs.
+
+// Levels of a work item.  The single-run kernels loop over every level in
+// each lane (LOL = false: the static stencil data of an entity is loaded once
+// and reused for all levels).  With levels on lanes (LOL = true) a lane takes
+// S.lpl consecutive levels of one entity, and the lanes of an entity cover
+// its S.nchunk chunks of levels side by side: when the levels are ensemble
+// members (every path kernel treats levels independently), neighbouring
+// lanes then read neighbouring members of the same entity, so every state
+// access and gather is one contiguous run and the stencil loads are shared.
+// Item order: entity, then level chunk, then tracer lane group.
+template <bool LOL, int NPL>
+__device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int item, int &sub,
+                                            uint32_t &k0, uint32_t &k1) {
+  const int q = item / NPL;
+  sub = item - q * NPL;
+  if constexpr (!LOL) {
+    k0 = 0;
+    k1 = L;
+    return q;
+  } else {
+    const int ent = q / S.nchunk, ch = q - ent * S.nchunk;
+    k0 = uint32_t(ch) * uint32_t(S.lpl);
+    k1 = min(k0 + uint32_t(S.lpl), L);
+    return ent;
+  }
+}
+
+// ---- edges: momentum ---------------------------------------------------------
+// compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
+// last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
+// 359-366)
+template <int WE, bool LOL = false>
+__device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
+  int sub;
+  uint32_t k0, k1;
+  const int e = split_levels<LOL, 1>(S, M.nL, item, sub, k0, k1);
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE;
+  if (S.freeze_u) {
+    if (S.stage < 3)
+      for (uint32_t k = k0; k < k1; k++) st(S.yu, e * L + k, ld(S.u0, e * L + k));
+    return;
+  }
+  const int2 cc = ld(M.e2c, e);
+  const int n = ld(M.nEoE, e);
+  uint32_t id[WE];
+  double w[WE];
+#pragma unroll
+  for (int j = 0; j < WE; j++) {
+    id[j] = ld(M.eoe, j * E + e);
+    w[j] = ld(M.woe, j * E + e);
+  }
+  const double dc = ld(M.dc, e);
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t o = e * L + k;
+    double uu[WE], pv[WE], he[WE];
+#pragma unroll
+    for (int j = 0; j < WE; j++) {
+      const uint32_t oj = id[j] * L + k;
+      uu[j] = ld(S.xu, oj);
+      pv[j] = ld(S.pv_edge, oj);
+      he[j] = ld(S.h_edge, oj);
+    }
+    const double pve = ld(S.pv_edge, o);
+    double q = 0;
+#pragma unroll
+    for (int j = 0; j < WE; j++) {
+      const double workpv = 0.5 * (pve + pv[j]);
+      const double qn = q + w[j] * uu[j] * workpv * he[j];
+      q = (j < n) ? qn : q;
+    }
+    const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
+    double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
+                    M.gravity * (ld(S.xh, o2) + ld(S.h_s, cc.y) - ld(S.xh, o1) -
+                                 ld(S.h_s, cc.x))) /
+                       dc;
+    if (S.use_visc) t += visc_at(M, e, k, cc, ld(M.e2v, e), S.div, S.vort, S.visc_coef);
+    if (S.stage == 0) {
+      const double u0 = ld(S.u0, o);
+      st(S.yu, o, u0 + S.w_sub * t);
+      st(S.nu, o, u0 + S.w_rk * t);
+    } else if (S.stage < 3) {
+      st(S.yu, o, ld(S.u0, o) + S.w_sub * t);
+      st(S.nu, o, ld(S.nu, o) + S.w_rk * t);
+    } else {
+      st(S.yu, o, ld(S.nu, o) + S.w_rk * t);
+    }
+  }
+}
+
+// ---- edges x tracers: FCT high-order minus low-order flux -------------------
+// calc_normal_thickness_flux, the high-order flux and its correction by the
+// low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
+// lane.
+template <int NT, int WA, bool LOL = false>
+__device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
+  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  int sub;
+  uint32_t k0, k1;
+  const int e = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int t0 = sub * TPL;
+  if (e >= M.nE) return;
+  const uint32_t L = M.nL, E = M.nE;
+  const int2 cc = ld(M.e2c, e);
+  const double dv = ld(M.dv, e);
+  const int na = ld(M.nAdv, e);
+  uint32_t ic[WA];
+  double c1[WA], c3[WA];
+#pragma unroll
+  for (int i = 0; i < WA; i++) {
+    ic[i] = ld(M.adv, i * E + e);
+    c1[i] = ld(M.advCoef, i * E + e);
+    c3[i] = ld(M.advCoef3, i * E + e) * M.coef3rd;
+  }
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t o = e * L + k;
+    const double F = ld(S.xu, o) * ld(S.h_edge, o);
+    const int mask = M.hoMask ? ld(M.hoMask, o) : 1;
+    const double wgt = F * double(mask);
+    const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
+    double tv[WA][TPL];
+#pragma unroll
+    for (int i = 0; i < WA; i++) ld_group<TPL>(S.xtr, (ic[i] * L + k) * NT + t0, tv[i]);
+    double ho[TPL];
+#pragma unroll
+    for (int g = 0; g < TPL; g++) ho[g] = 0.0;
+#pragma unroll
+    for (int i = 0; i < WA; i++) {
+      const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
+      const bool live = i < na && int(k) < lvl_cell(M, ic[i]);
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        const double x = ho[g] + tv[i][g] * weightFactor;
+        ho[g] = live ? x : ho[g];
+      }
+    }
+    if (int(k) < lvl_edge(M, e)) {
+      double t1[TPL], t2[TPL];
+      ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k) * NT + t0, t1);
+      ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k) * NT + t0, t2);
+      const double tracerWeight = double((mask + 1) & 1) * (dv * 0.5) * F;
+      const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        const double lo = dv * (ntf1 * t1[g] + ntf2 * t2[g]);
+        ho[g] += tracerWeight * (t1[g] + t2[g]);
+        ho[g] -= lo;
+      }
+    }
+    st_group<TPL>(S.anti, o * NT + t0, ho);
+  }
+}
+
+// compute_substep / accumulate_update / finalize_timestep for one tracer of
+// one cell (solve.cc:297-305, 332-340, 350-357)
+__device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uint32_t ot,
+                                              double te, double hy) {
+  if (S.stage == 0) {
+    const double h0 = ld(S.h0, ol), t0 = ld(S.tr0, ot);
+    st(S.ytr, ot, (h0 * t0 + S.w_sub * te) / hy);
+    st(S.ntr, ot, t0 * h0 + S.w_rk * te);
+  } else if (S.stage < 3) {
+    st(S.ytr, ot, (ld(S.h0, ol) * ld(S.tr0, ot) + S.w_sub * te) / hy);
+    st(S.ntr, ot, ld(S.ntr, ot) + S.w_rk * te);
+  } else {
+    st(S.ytr, ot, (ld(S.ntr, ot) + S.w_rk * te) / hy);
+  }
+}
+
+// ---- cells (x tracers) -------------------------------------------------------
+// Thickness tendency and update, ke and divergence of the new state (the
+// cell's first lane stores them), and per tracer either the centred
+// tendency with its update or the FCT bounds, upwind tendency and limiter
+// factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
+template <int NT, int W, bool LOL = false>
+__device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
+  constexpr int TPL = NT > 0 ? tracers_per_lane<NT>() : 1;
+  constexpr int NPL = NT > 0 ? NT / TPL : 1;
+  int sub;
+  uint32_t k0, k1;
+  const int c = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int t0 = sub * TPL;
+  if (c >= M.nC) return;
+  const uint32_t L = M.nL, C = M.nC;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const uint32_t rank = ld(M.ascRank, c);
+  const double ar = ld(M.area, c);
+  const bool lead = t0 == 0;
+  uint32_t e[W];
+  double dv[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) e[j] = ld(M.c2e, j * C + c);
+#pragma unroll
+  for (int j = 0; j < W; j++) dv[j] = ld(M.dv, e[j]);
+
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t ol = c * L + k;
+    if (lead) {
+      // ke (solve.cc:63-77) and divergence of the new state
+      double yu[W], dc[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        yu[j] = ld(S.yu, e[j] * L + k);
+        dc[j] = ld(M.dc, e[j]);
+      }
+      double ke = 0.0, dvg = 0.0;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const double kx = ke + 0.25 * dc[j] * dv[j] * (yu[j] * yu[j]);
+        const double dx = dvg + (-sign_on_cell(neg, j)) * dv[j] * yu[j];
+        ke = (j < n) ? kx : ke;
+        dvg = (j < n) ? dx : dvg;
+      }
+      st(S.ke_y, ol, ke / ar);
+      if (S.use_visc) st(S.div_y, ol, dvg / ar);
+    }
+    double xu[W], he[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      xu[j] = ld(S.xu, e[j] * L + k);
+      he[j] = ld(S.h_edge, e[j] * L + k);
+    }
+    // compute_tend, thickness part: edges in ascending id (solve.cc:191-205)
+    double th;
+    {
+      double fl[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) fl[j] = xu[j] * dv[j] * he[j];
+      th = sum_ascending<W>(0.0, fl, rank, neg, n);
+    }
+    th = th / ar;
+    double hy;
+    if (S.stage == 0) {
+      const double h0 = ld(S.h0, ol);
+      hy = h0 + S.w_sub * th;
+      if (lead) st(S.nh, ol, h0 + S.w_rk * th);
+    } else if (S.stage < 3) {
+      hy = ld(S.h0, ol) + S.w_sub * th;
+      if (lead) st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
+    } else {
+      hy = ld(S.nh, ol) + S.w_rk * th;
+    }
+    if (lead) st(S.yh, ol, hy);
+
+    if constexpr (NT > 0) {
+      const double invArea = 1.0 / ar;
+      const uint32_t ot = ol * NT + t0;
+      double tc[TPL];
+      ld_group<TPL>(S.xtr, ot, tc);
+      double tn[W][TPL];
+      {
+        uint32_t op[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) op[j] = ld(M.cOpp, j * C + c);
+#pragma unroll
+        for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+      }
+      if (S.scheme == MPAS_TRACER_CENTRED) {
+        // compute_scalar_tend (solve.cc:237-270), ascending edge id, then
+        // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          double term[W];
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const bool first = (neg >> j) & 1;
+            const double tracer_edge =
+                first ? 0.5 * (tc[g] + tn[j][g]) : 0.5 * (tn[j][g] + tc[g]);
+            term[j] = (xu[j] * dv[j] * he[j] * tracer_edge) / ar;
+          }
+          double te = sum_ascending<W>(0.0, term, rank, neg, n);
+          if (S.use_hmix) {
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+              const bool first = (neg >> j) & 1;
+              const double rho_scaling = M.msd * S.kappa * dv[j] / ld(M.dc, e[j]);
+              const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
+              const double flux = he[j] * turb * rho_scaling;
+              const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
+              te = (j < n) ? x : te;
+            }
+          }
+          tracer_update(S, ol, ot + g, te, hy);
+        }
+      } else if (S.scheme == MPAS_TRACER_FCT) {
+        // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
+        const double dtInvAreaCell = S.dt / ar;
+        const double lt = ld(S.xh, ol);
+        double hp = lt;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const double signedFactor = dtInvAreaCell * dv[j] * sign_on_cell(neg, j);
+          const double x = hp + signedFactor * (xu[j] * he[j]);
+          hp = (j < n) ? x : hp;
+        }
+        const double hProvInv = 1.0 / hp;
+        double an[W][TPL];
+#pragma unroll
+        for (int j = 0; j < W; j++) ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
+        double rf[2 * TPL], tl[TPL];
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          // bounds over cellsOnCell (195-214); upwind tendency and
+          // antidiffusive in/out sums, file order (232-249)
+          double tmax = tc[g], tmin = tc[g], te = 0.0, fin = 0.0, fout = 0.0;
+#pragma unroll
+          for (int j = 0; j < W; j++) {
+            const bool live = j < n;
+            tmax = live ? smax(tmax, tn[j][g]) : tmax;
+            tmin = live ? smin(tmin, tn[j][g]) : tmin;
+            const double signedFactor = sign_on_cell(neg, j) * invArea;
+            const double F = xu[j] * he[j];
+            const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
+            const bool first = (neg >> j) & 1;
+            const double t1 = first ? tc[g] : tn[j][g];
+            const double t2 = first ? tn[j][g] : tc[g];
+            const double lo = dv[j] * (ntf1 * t1 + ntf2 * t2);
+            const double xe = te + signedFactor * lo;
+            const double xi = fin + smax(0.0, signedFactor * an[j][g]);
+            const double xo = fout + smin(0.0, signedFactor * an[j][g]);
+            te = live ? xe : te;
+            fin = live ? xi : fin;
+            fout = live ? xo : fout;
+          }
+          // limiter factors (253-266), stored (in, out) per tracer
+          const double eps = 1.e-10;
+          const double tracerUpwindNew = (tc[g] * lt + S.dt * te) * hProvInv;
+          const double tracerMaxNew = tracerUpwindNew + S.dt * fin * hProvInv;
+          const double tracerMinNew = tracerUpwindNew + S.dt * fout * hProvInv;
+          double scaleFactor =
+              (tmax - tracerUpwindNew) / (tracerMaxNew - tracerUpwindNew + eps);
+          rf[2 * g] = smin(1.0, smax(0.0, scaleFactor));
+          scaleFactor = (tracerUpwindNew - tmin) / (tracerUpwindNew - tracerMinNew + eps);
+          rf[2 * g + 1] = smin(1.0, smax(0.0, scaleFactor));
+          tl[g] = te;
+        }
+        if (!S.keep_rfac) {
+#pragma unroll
+          for (int g = 0; g < TPL; g++) {
+            const double pair[2] = {rf[2 * g], rf[2 * g + 1]};
+            st_group<2>(S.rfac, (ot + g) * 2, pair);
+          }
+        }
+        st_group<TPL>(S.tlo, ot, tl);
+      }
+    }
+  }
+}
+
+// ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
+template <int NT, int W, bool LOL = false>
+__device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item) {
+  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  int sub;
+  uint32_t k0, k1;
+  const int c = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int t0 = sub * TPL;
+  if (c >= M.nC) return;
+  const uint32_t L = M.nL, C = M.nC;
+  const int n = ld(M.nEoC, c);
+  const uint8_t neg = ld(M.sgnNeg, c);
+  const double invAreaCell1 = 1.0 / ld(M.area, c);
+  uint32_t e[W], op[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    e[j] = ld(M.c2e, j * C + c);
+    op[j] = ld(M.cOpp, j * C + c);
+  }
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t ol = c * L + k, ot = ol * NT + t0;
+    double te[TPL], rs[2 * TPL];
+    ld_group<TPL>(S.tlo, ot, te);
+#pragma unroll
+    for (int g = 0; g < TPL; g++) {
+      double pair[2];
+      ld_group<2>(S.rfac, (ot + g) * 2, pair);
+      rs[2 * g] = pair[0];
+      rs[2 * g + 1] = pair[1];
+    }
+    double an[W][TPL], ro[W][2 * TPL];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
+      const uint32_t oo = (op[j] * L + k) * NT + t0;
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        double pair[2];
+        ld_group<2>(S.rfac, (oo + g) * 2, pair);
+        ro[j][2 * g] = pair[0];
+        ro[j][2 * g + 1] = pair[1];
+      }
+    }
+    // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const bool first = (neg >> j) & 1;  // c == m.cells(e)[0]
+      const double signedFactor = (first ? -1.0 : 1.0) * invAreaCell1;
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        const double rin = rs[2 * g], rout = rs[2 * g + 1];
+        const double oin = ro[j][2 * g], oout = ro[j][2 * g + 1];
+        const double in1 = first ? rin : oin, out1 = first ? rout : oout;
+        const double in2 = first ? oin : rin, out2 = first ? oout : rout;
+        const double x = an[j][g];
+        const double hof = smax(0.0, x) * smin(out1, in2) + smin(0.0, x) * smin(in1, out2);
+        const double y = te[g] + signedFactor * hof;
+        te[g] = (j < n) ? y : te[g];
+      }
+    }
+    if (S.use_hmix) {
+      // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
+      double tc[TPL], tn[W][TPL], he[W], rho[W];
+      ld_group<TPL>(S.xtr, ot, tc);
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+        he[j] = ld(S.h_edge, e[j] * L + k);
+        rho[j] = M.msd * S.kappa * ld(M.dv, e[j]) / ld(M.dc, e[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const bool first = (neg >> j) & 1;
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
+          const double flux = he[j] * turb * rho[j];
+          const double y = te[g] - (first ? -1.0 : 1.0) * flux * invAreaCell1;
+          te[g] = (j < n) ? y : te[g];
+        }
+      }
+    }
+    const double hy = ld(S.yh, ol);
+#pragma unroll
+    for (int g = 0; g < TPL; g++) tracer_update(S, ol, ot + g, te[g], hy);
+  }
+}
+
+}  // namespace mpasx

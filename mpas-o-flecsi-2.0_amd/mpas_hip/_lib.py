@@ -19,6 +19,7 @@ P_f64 = ctypes.POINTER(ctypes.c_double)
 P_i64 = ctypes.POINTER(ctypes.c_int64)
 
 TRACER_NONE, TRACER_CENTRED, TRACER_FCT = 0, 1, 2
+BATCH_GRAPH, BATCH_LAUNCH = 0, 1
 STATUS = {0: "MPAS_OK", -1: "MPAS_EINVAL", -2: "MPAS_EHIP", -3: "MPAS_ESHAPE",
           -4: "MPAS_ESTATE", -5: "MPAS_ENOMEM"}
 
@@ -68,6 +69,13 @@ def hip():
 
 def check(rc, where):
     if rc != 0:
+        raise MpasError(rc, where, hip().mpas_last_error().decode())
+    return rc
+
+
+def check_count(rc, where):
+    """A call that returns a count (>= 0) or a negative status."""
+    if rc < 0:
         raise MpasError(rc, where, hip().mpas_last_error().decode())
     return rc
 
@@ -127,6 +135,17 @@ SIGNATURES = {
     "mpas_adj_backward": (c_int, [c_void_p] + [_D] * 6),
     "mpas_tlm_forward": (c_int, [c_void_p, c_double, c_int] + [_D] * 12),
     "mpas_adj_frozen_forward": (c_int, [c_void_p, c_int] + [_D] * 6),
+    "mpas_batch_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, P_i32, P_f64,
+                                   ctypes.POINTER(c_void_p)]),
+    "mpas_batch_destroy": (c_int, [c_void_p]),
+    "mpas_batch_upload": (c_int, [c_void_p, c_int, P_f64, P_f64, P_f64, P_f64]),
+    "mpas_batch_download": (c_int, [c_void_p, c_int, P_f64, P_f64, P_f64]),
+    "mpas_batch_step": (c_int, [c_void_p, c_int, c_int]),
+    "mpas_batch_sync": (c_int, [c_void_p]),
+    "mpas_batch_groups": (c_int, [c_void_p]),
+    "mpas_batch_config": (c_int, [c_void_p, P_i32, c_int, c_int, c_int]),
+    "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 3),
+    "mpas_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
 }
 
 

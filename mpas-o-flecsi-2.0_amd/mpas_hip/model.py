@@ -380,3 +380,91 @@ class Context:
         L.check(L.hip().mpas_adj_frozen_forward(self._h, int(nsteps), _ptr(x0[0]), _ptr(x0[1]),
                                                 self._trp(x0[2]), _ptr(x1[0]), _ptr(x1[1]),
                                                 self._trp(x1[2])), "mpas_adj_frozen_forward")
+
+
+class Batch:
+    """An ensemble: member i runs on contexts[mesh_of[i]] with time step dt[i]
+    (mpas_batch_*).  The contexts must outlive the batch."""
+
+    def __init__(self, contexts, mesh_of, dt):
+        self._b = None
+        self.contexts = list(contexts)
+        self.mesh_of = np.ascontiguousarray(mesh_of, dtype=np.int32)
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        if dt.size != self.mesh_of.size:
+            raise ValueError("mesh_of and dt differ in length")
+        hs = (ctypes.c_void_p * len(self.contexts))(*[c._h for c in self.contexts])
+        b = ctypes.c_void_p()
+        L.check(L.hip().mpas_batch_create(hs, len(self.contexts), int(self.mesh_of.size),
+                                          self.mesh_of.ctypes.data_as(L.P_i32),
+                                          dt.ctypes.data_as(L.P_f64), ctypes.byref(b)),
+                "mpas_batch_create")
+        self._b = b.value
+        self.n = int(self.mesh_of.size)
+
+    def close(self):
+        if self._b:
+            L.check(L.hip().mpas_batch_destroy(self._b), "mpas_batch_destroy")
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, member, h, u, tracers=None, h_s=None):
+        c = self.contexts[self.mesh_of[member]]
+        h = np.ascontiguousarray(h, dtype=np.float64).ravel()
+        u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+        h_s = np.ascontiguousarray(np.zeros(c.nCells) if h_s is None else h_s,
+                                   dtype=np.float64).ravel()
+        tr = np.ascontiguousarray(tracers, dtype=np.float64).ravel() if c.nTracers else None
+        L.check(L.hip().mpas_batch_upload(self._b, int(member), h.ctypes.data_as(L.P_f64),
+                                          u.ctypes.data_as(L.P_f64),
+                                          tr.ctypes.data_as(L.P_f64) if tr is not None else None,
+                                          h_s.ctypes.data_as(L.P_f64)), "mpas_batch_upload")
+
+    def download(self, member):
+        c = self.contexts[self.mesh_of[member]]
+        h = np.empty(c.nCells * c.nLevels)
+        u = np.empty(c.nEdges * c.nLevels)
+        tr = np.empty(c.nCells * c.nLevels * c.nTracers)
+        L.check(L.hip().mpas_batch_download(self._b, int(member), h.ctypes.data_as(L.P_f64),
+                                            u.ctypes.data_as(L.P_f64),
+                                            tr.ctypes.data_as(L.P_f64) if tr.size else None),
+                "mpas_batch_download")
+        return h, u, tr
+
+    def step(self, nsteps, mode=L.BATCH_GRAPH):
+        L.check(L.hip().mpas_batch_step(self._b, int(nsteps), int(mode)), "mpas_batch_step")
+
+    def sync(self):
+        L.check(L.hip().mpas_batch_sync(self._b), "mpas_batch_sync")
+
+    def groups(self):
+        return L.check_count(L.hip().mpas_batch_groups(self._b), "mpas_batch_groups")
+
+    def config(self, block=None, members_per_lane=1, order=0, xcd=1):
+        blk = None
+        if block is not None:
+            blk = np.ascontiguousarray(block, dtype=np.int32)
+        L.check(L.hip().mpas_batch_config(self._b, blk.ctypes.data_as(L.P_i32) if blk is not None
+                                          else None, int(members_per_lane), int(order), int(xcd)),
+                "mpas_batch_config")
+
+    def get_config(self):
+        blk = np.zeros(6, dtype=np.int32)
+        lpl, o, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.check(L.hip().mpas_batch_get_config(self._b, blk.ctypes.data_as(L.P_i32),
+                                              ctypes.byref(lpl), ctypes.byref(o),
+                                              ctypes.byref(x)), "mpas_batch_get_config")
+        return {"block": blk.tolist(), "members_per_lane": lpl.value, "order": o.value,
+                "xcd": x.value}
+
+    def autotune(self, trial_steps=3):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        L.check(L.hip().mpas_batch_autotune(self._b, int(trial_steps), buf, len(buf)),
+                "mpas_batch_autotune")
+        return json.loads(buf.value.decode())

@@ -1,0 +1,136 @@
+"""Ensembles (mpas_batch_*): members on meshes of uneven size advanced by
+one launch per phase for all member groups (direct or replayed from a
+hipGraph), members of one mesh and time step stored member-minor.
+
+Design claim: a member evolves bit for bit as it does alone (every kernel
+treats levels, here the members, independently).  Pinned against a plain
+context run of each member, across launch modes and configurations."""
+import numpy as np
+import pytest
+
+from conftest import bitwise_equal, mesh
+
+pytestmark = pytest.mark.gpu
+
+LEVELS = (3, 4, 3, 4, 3, 3, 4)
+DTS = (600.0, 300.0, 450.0, 300.0, 600.0, 600.0, 300.0)
+
+
+def _ensemble(levels=LEVELS, nT=2, dts=DTS):
+    from mpas_hip import TRACER_FCT, Batch, Context, sw_workload_state
+    lv = sorted(set(levels))
+    ctxs = [Context(mesh(L), nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+            for L in lv]
+    mesh_of = [lv.index(L) for L in levels]
+    states = []
+    for i, L in enumerate(levels):
+        u, h, tr = sw_workload_state(mesh(L), nT, seed=10 + i, u0=10.0 + 3 * i,
+                                     bump_amp=5.0 + 2 * i)
+        states.append((h, u, tr.ravel(), np.full(mesh(L).nCells, 4500.0)))
+    b = Batch(ctxs, mesh_of, dts)
+    return ctxs, b, states, mesh_of
+
+
+def _load(b, states):
+    for i, (h, u, tr, hs) in enumerate(states):
+        b.upload(i, h, u, tr, hs)
+
+
+def _singles(ctxs, states, mesh_of, nsteps, dts=DTS):
+    out = []
+    for i, (h, u, tr, hs) in enumerate(states):
+        c = ctxs[mesh_of[i]]
+        c.sw_upload(h, u, tr, hs)
+        c.rk4_step(dts[i], nsteps)
+        out.append(c.sw_download())
+    return out
+
+
+def _close(ctxs, b):
+    b.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_groups_follow_mesh_and_dt():
+    ctxs, b, _, _ = _ensemble()
+    # (L3, 600), (L4, 300), (L3, 450)
+    assert b.groups() == 3
+    _close(ctxs, b)
+
+
+def test_members_equal_single_runs_both_modes():
+    from mpas_hip import BATCH_GRAPH, BATCH_LAUNCH
+    ctxs, b, states, mesh_of = _ensemble()
+    ref = _singles(ctxs, states, mesh_of, 5)
+    for mode in (BATCH_GRAPH, BATCH_LAUNCH):
+        _load(b, states)
+        b.step(5, mode)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (mode, i)
+    _close(ctxs, b)
+
+
+def test_configurations_keep_results():
+    ctxs, b, states, mesh_of = _ensemble()
+    ref = _singles(ctxs, states, mesh_of, 4)
+    for blk, lpl, order, xcd in (([64] * 6, 1, 1, 0), ([512] * 6, 2, 0, 1),
+                                 ([128, 256, 64, 512, 128, 64], 3, 1, 1), ([256] * 6, 8, 0, 0)):
+        b.config(blk, lpl, order, xcd)
+        _load(b, states)
+        b.step(4)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (blk, lpl, order, xcd, i)
+    _close(ctxs, b)
+
+
+def test_autotune_keeps_results_and_state():
+    ctxs, b, states, mesh_of = _ensemble()
+    ref = _singles(ctxs, states, mesh_of, 3)
+    _load(b, states)
+    before = [b.download(i) for i in range(b.n)]
+    trials = b.autotune(trial_steps=2)
+    assert len(trials) >= 5 and all(t["ms_per_step"] > 0 for t in trials)
+    print("tuned:", b.get_config())
+    for x, y in zip(before, [b.download(i) for i in range(b.n)]):  # the trials are undone
+        for a, c in zip(x, y):
+            assert bitwise_equal(a, c)
+    b.step(3)
+    b.sync()
+    for i in range(b.n):
+        for a, r in zip(b.download(i), ref[i]):
+            assert bitwise_equal(a, r)
+    _close(ctxs, b)
+
+
+def test_batch_error_paths():
+    from mpas_hip import TRACER_CENTRED, TRACER_FCT, Batch, Context, MpasError
+    a = Context(mesh(3), nTracers=2, tracer_scheme=TRACER_FCT)
+    c = Context(mesh(3), nTracers=1, tracer_scheme=TRACER_FCT)
+    with pytest.raises(MpasError):
+        Batch([a, c], [0, 1], [600.0, 600.0])  # tracer counts differ
+    with pytest.raises(MpasError):
+        Batch([a], [1], [600.0])  # mesh index out of range
+    b = Batch([a], [0, 0], [600.0, 600.0])
+    with pytest.raises(MpasError):
+        b.step(1)  # no member state yet
+    with pytest.raises(MpasError):
+        b.config([96] * 6)
+    m = mesh(3)
+    u = np.zeros(m.nEdges)
+    h = np.full(m.nCells, 5000.0)
+    tr = np.zeros(2 * m.nCells)
+    b.upload(0, h, u, tr, np.full(m.nCells, 4500.0))
+    with pytest.raises(MpasError):  # a group shares its bottom topography
+        b.upload(1, h, u, tr, np.full(m.nCells, 4400.0))
+    b.close()
+    a.close()
+    c.close()
+    d = Context(mesh(3), nTracers=1, tracer_scheme=TRACER_CENTRED)
+    b = Batch([d], [0], [600.0])  # centred tracers batch as well
+    b.close()
+    d.close()
