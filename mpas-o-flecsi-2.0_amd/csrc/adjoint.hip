@@ -47,11 +47,10 @@ struct AdjMesh {
   const uint8_t *nEt;
   const int *et;
   const double *wt;
-  // edges whose advCellsForEdge holds c, with advCoef and advCoef_3rd of
-  // that slot
+  // (edge, slot) pairs whose advCellsForEdge entry is c, as the flat slot
+  // index slot * nEdges + edge
   const uint8_t *nAt;
   const int *at;
-  const double *ac1, *ac3;
   // vertices of c with the kite area that h_vertex pairs with c (the i-th
   // ascending cell of the vertex takes the i-th file-order kite area)
   const uint8_t *nVc;
@@ -65,6 +64,7 @@ struct AdjState {
   double dt = 0;
   bool forward_done = false;
   int taped = 0;       // steps from step 0 whose records the tape holds
+  int tape_seg = -1;   // checkpoint segment whose records the tape holds
   std::vector<Rec> rec;          // 4K + 1 stage records
   std::vector<double *> ck;      // checkpoints: (u, h, tr) per segment
   double *pv_vertex = nullptr, *nu = nullptr, *nh = nullptr, *ntr = nullptr, *tlo = nullptr;
@@ -81,8 +81,7 @@ struct AdjState {
   double *aw_u = nullptr, *aw_h = nullptr, *aw_tr = nullptr;
   double *tb_u = nullptr, *tb_h = nullptr, *tb_tr = nullptr;
   double *ubA = nullptr, *heb = nullptr, *pveb = nullptr, *r1 = nullptr, *r2 = nullptr,
-         *hbf = nullptr, *sg = nullptr, *vT = nullptr, *hvT = nullptr, *keT = nullptr,
-         *divT = nullptr;
+         *hbf = nullptr, *vT = nullptr, *hvT = nullptr, *keT = nullptr, *divT = nullptr;
   AdjMesh T{};
   std::vector<void *> bufs;
 };
@@ -382,88 +381,136 @@ __global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Re
   }
 }
 
+// Ragged lists are read in chunks of kChunk slots: the slot indices of a
+// chunk are clamped to the row (the tail repeats the last slot, masked out
+// of the sums), so every load of a chunk is in bounds and issues before the
+// first value is needed, instead of one dependent round trip per slot.
+constexpr int kChunk = 16;
+
 // Edge part of the stage adjoint: from the tendency adjoints to the partial
 // u adjoint (thickness flux, PV flux, tracer flux paths), the h_edge and
 // pv_edge adjoints, and per (edge, tracer) what the cells gather: r1 / r2
-// (into cell1 / cell2: upwind flux and mixing) and hbf (the 3rd-order flux
-// adjoint times F, which each stencil cell scales by its coefficient).
+// (into cell1 / cell2: upwind flux and mixing) and, per (stencil slot,
+// edge, tracer), hbf = the 3rd-order flux adjoint times F times the slot's
+// coefficient: the adjoint the slot's cell receives.
+// Two lanes per edge (adjacent, one wave) run the same code on alternate
+// slots of every ragged list (edgesOnEdge, its transpose, the advection
+// stencil); one lane exchange sums the partials and lane 0 finishes.
+constexpr int kEdgeLanes = 2;
 __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
                                                   const double *__restrict__ tb_u,
                                                   const double *__restrict__ tb_h,
                                                   const double *__restrict__ tb_tr, double *ubA,
                                                   double *heb, double *pveb, double *r1,
-                                                  double *r2, double *hbf, double *sg) {
-  const int e = xcd_item();
+                                                  double *r2, double *hbf) {
+  const int item = xcd_item();
+  const int e = item / kEdgeLanes, lane = item % kEdgeLanes;
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE, NT = P.nT;
+  constexpr int Q = kChunk / kEdgeLanes;  // slots per lane and round
   const int2 cc = ld(M.e2c, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
   const double A1 = ld(M.area, c1), A2 = ld(M.area, c2);
   const double dv = ld(M.dv, e), dc = ld(M.dc, e);
-  const int n = ld(M.nEoE, e), nt = ld(A.nEt, e), na = P.fct ? ld(M.nAdv, e) : 0;
-  const double rho = M.msd * P.kappa * dv / dc;
+  const int na = P.fct ? ld(M.nAdv, e) : 0;
   for (uint32_t k = 0; k < L; k++) {
     const uint32_t o = e * L + k;
     const double ue = ld(R.u, o), hee = ld(R.he, o), pvee = ld(R.pve, o);
-    // tend_h: cell1 -= u dv h_edge / area, cell2 += (solve.cc:191-205)
-    const double fb = -ld(tb_h, c1 * L + k) / A1 + ld(tb_h, c2 * L + k) / A2;
-    double ub = fb * dv * hee, hb = fb * ue * dv, pb = 0.0;
+    double pb = 0.0, s1 = 0.0, s2 = 0.0;
     if (!P.freeze_u) {
-      // q = sum_j w_j u_j 0.5 (pv_e + pv_j) h_j: own row for pv_e, the
-      // rows that hold e for u_e, h_edge_e and pv_edge_e
+      const int n = ld(M.nEoE, e), nt = ld(A.nEt, e);
       const double tue = ld(tb_u, o);
-      for (int j = 0; j < n; j++) {
-        const uint32_t oj = uint32_t(ld(M.eoe, j * E + e)) * L + k;
-        pb += 0.5 * ld(M.woe, j * E + e) * tue * ld(R.u, oj) * ld(R.he, oj);
+      for (int j0 = lane; j0 < n; j0 += kChunk) {
+        uint32_t oj[Q];
+        double w[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const int jj = min(j0 + j * kEdgeLanes, n - 1);
+          oj[j] = uint32_t(ld(M.eoe, jj * E + e)) * L + k;
+          w[j] = ld(M.woe, jj * E + e);
+        }
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const double x = 0.5 * w[j] * tue * ld(R.u, oj[j]) * ld(R.he, oj[j]);
+          pb += (j0 + j * kEdgeLanes < n) ? x : 0.0;
+        }
       }
-      double s1 = 0.0, s2 = 0.0;
-      for (int j = 0; j < nt; j++) {
-        const uint32_t o2 = uint32_t(ld(A.et, j * E + e)) * L + k;
-        const double t = ld(tb_u, o2) * ld(A.wt, j * E + e);
-        s1 += t * 0.5 * (ld(R.pve, o2) + pvee);
-        s2 += t;
+      for (int j0 = lane; j0 < nt; j0 += kChunk) {
+        uint32_t o2[Q];
+        double w[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const int jj = min(j0 + j * kEdgeLanes, nt - 1);
+          o2[j] = uint32_t(ld(A.et, jj * E + e)) * L + k;
+          w[j] = ld(A.wt, jj * E + e);
+        }
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const double t = ld(tb_u, o2[j]) * w[j];
+          const bool live = j0 + j * kEdgeLanes < nt;
+          s1 += live ? t * 0.5 * (ld(R.pve, o2[j]) + pvee) : 0.0;
+          s2 += live ? t : 0.0;
+        }
       }
-      ub += s1 * hee;
-      hb += s1 * ue;
-      pb += 0.5 * s2 * ue * hee;
+      pb += __shfl_xor(pb, 1);
+      s1 += __shfl_xor(s1, 1);
+      s2 += __shfl_xor(s2, 1);
     }
+    // tracer paths: the stencil sums split over the lanes, the rest on both
+    double Fb = 0.0, hbt = 0.0;
     if (P.fct) {
+      const double rho = M.msd * P.kappa * dv / dc;
       const double F = ue * hee, sgn = fsign(F);
       const double Fp = F > 0.0 ? F : 0.0, Fm = F < 0.0 ? F : 0.0;
-      double Fb = 0.0;
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
         const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
         const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
         const double lob = (1.0 - lam) * phib, hob = lam * phib;
-        const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
-        Fb += lob * dv * ((F > 0.0 ? T1 : 0.0) + (F < 0.0 ? T2 : 0.0));
+        const double hf = hob * F;
         double hs = 0.0;
-        for (int i = 0; i < na; i++) {
-          const uint32_t ci = ld(M.adv, i * E + e);
-          const double cf = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
-          hs += ld(R.tr, (ci * L + k) * NT + t) * cf;
+        for (int i0 = lane; i0 < na; i0 += kChunk) {
+          uint32_t ii[Q];
+          double cf[Q];
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const int i = min(i0 + j * kEdgeLanes, na - 1);
+            ii[j] = (uint32_t(ld(M.adv, i * E + e)) * L + k) * NT + t;
+            cf[j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
+          }
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const int i = i0 + j * kEdgeLanes;
+            if (i < na) {
+              hs += ld(R.tr, ii[j]) * cf[j];
+              st(hbf, ((uint32_t(i) * E + e) * L + k) * NT + t, hf * cf[j]);
+            }
+          }
         }
-        Fb += hob * hs;
+        hs += __shfl_xor(hs, 1);
+        if (lane) continue;
+        const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
+        Fb += lob * dv * ((F > 0.0 ? T1 : 0.0) + (F < 0.0 ? T2 : 0.0)) + hob * hs;
         double a1 = lob * dv * Fp, a2 = lob * dv * Fm;
         if (P.use_hmix) {  // mixing flux h_edge (T2 - T1) rho enters with -sign/area
           const double Hb = -phib;
-          hb += Hb * rho * (T2 - T1);
+          hbt += Hb * rho * (T2 - T1);
           const double Db = Hb * hee * rho;
           a1 -= Db;
           a2 += Db;
         }
         st(r1, o * NT + t, a1);
         st(r2, o * NT + t, a2);
-        st(hbf, o * NT + t, hob * F);
       }
-      ub += Fb * hee;
-      hb += Fb * ue;
-      st(sg, o, sgn);
     }
+    if (lane) continue;
+    // tend_h: cell1 -= u dv h_edge / area, cell2 += (solve.cc:191-205)
+    const double fb = -ld(tb_h, c1 * L + k) / A1 + ld(tb_h, c2 * L + k) / A2;
+    const double ub = fb * dv * hee + s1 * hee + Fb * hee;
+    const double hb = fb * ue * dv + s1 * ue + hbt + Fb * ue;
     st(ubA, o, ub);
     st(heb, o, hb);
-    st(pveb, o, pb);
+    st(pveb, o, pb + 0.5 * s2 * ue * hee);
   }
 }
 
@@ -505,36 +552,56 @@ __global__ __launch_bounds__(BS) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
 // Cell part: h and tracer adjoints of the stage input, ke and divergence
 // adjoints for the edges, then the bookkeeping: X_s of stages 1-3 was
 // formed from the step's initial state and the stage s-1 tendencies.
+// Four lanes per cell (adjacent, one wave): lane r takes every 4th entry
+// of the cell's edges, vertices and stencil slots; a fixed exchange tree
+// sums the partials and lane 0 finishes.
+constexpr int kCellLanes = 4;
+__device__ __forceinline__ double cell_lanes_sum(double x) {
+  x += __shfl_xor(x, 1);
+  x += __shfl_xor(x, 2);
+  return x;
+}
 __global__ __launch_bounds__(BS) void k_ad_cell(
     DMesh M, AdjMesh A, AdjParams P, Rec R, Rec R0, const double *__restrict__ tb_u,
     const double *__restrict__ heb, const double *__restrict__ hvT,
     const double *__restrict__ r1, const double *__restrict__ r2,
-    const double *__restrict__ hbf, const double *__restrict__ sg, double *keT, double *divT,
+    const double *__restrict__ hbf, double *keT, double *divT,
     double *xb_h, double *xb_tr, const double *__restrict__ aw_h,
     const double *__restrict__ aw_tr, double *tb_h, double *tb_tr) {
-  const int c = xcd_item();
+  const int item = xcd_item();
+  const int c = item / kCellLanes, lane = item % kCellLanes;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
   const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
   const uint8_t neg = ld(M.sgnNeg, c);
   const double ar = ld(M.area, c);
+  constexpr int Q = kChunk / kCellLanes;  // entries per lane and round
   for (uint32_t k = 0; k < L; k++) {
     const uint32_t ol = c * L + k;
     double g1 = 0.0, hsum = 0.0, Tb[8] = {};
-    for (int j = 0; j < n; j++) {
+    for (int j = lane; j < n; j += kCellLanes) {
       const uint32_t e = ld(M.c2e, j * C + c), oe = e * L + k;
       if (!P.freeze_u) g1 += sign_on_cell(neg, j) * ld(tb_u, oe) / ld(M.dc, e);
       hsum += 0.5 * ld(heb, oe);
       const bool first = (neg >> j) & 1;
       for (uint32_t t = 0; t < NT; t++) Tb[t] += first ? ld(r1, oe * NT + t) : ld(r2, oe * NT + t);
     }
-    for (int i = 0; i < nv; i++)
+    for (int i = lane; i < nv; i += kCellLanes)
       hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
-    for (int i = 0; i < na; i++) {
-      const uint32_t e = ld(A.at, i * C + c), oe = e * L + k;
-      const double cf = ld(A.ac1, i * C + c) + ld(A.ac3, i * C + c) * M.coef3rd * ld(sg, oe);
-      for (uint32_t t = 0; t < NT; t++) Tb[t] += ld(hbf, oe * NT + t) * cf;
+    for (int i0 = lane; i0 < na; i0 += kChunk) {
+      uint32_t os[Q];
+#pragma unroll
+      for (int j = 0; j < Q; j++)
+        os[j] = (uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * L + k) * NT;
+      for (uint32_t t = 0; t < NT; t++) {
+#pragma unroll
+        for (int j = 0; j < Q; j++) Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, os[j] + t) : 0.0;
+      }
     }
+    g1 = cell_lanes_sum(g1);
+    hsum = cell_lanes_sum(hsum);
+    for (uint32_t t = 0; t < NT; t++) Tb[t] = cell_lanes_sum(Tb[t]);
+    if (lane) continue;
     const double hb = -M.gravity * g1 + hsum;
     st(keT, ol, -g1 / ar);
     if (P.use_visc) st(divT, ol, P.visc_coef * g1 / ar);
@@ -664,9 +731,8 @@ int build_transposes(mpas_ctx *ctx, AdjState &A) {
     for (int e = 0; e < E; e++)
       for (int i = 0; i < m.nAdvCellsForEdge[e]; i++) {
         const int c = m.advCellsForEdge[size_t(e) * 16 + i];
-        aT.id[c].push_back(e);
-        aT.a[c].push_back(m.advCoef[size_t(e) * 16 + i]);
-        aT.b[c].push_back(m.advCoef_3rd[size_t(e) * 16 + i]);
+        aT.id[c].push_back(i * E + e);
+        aT.a[c].push_back(0.0);
       }
   for (int v = 0; v < V; v++)
     for (int i = 0; i < m.v2c.size(v); i++) {
@@ -678,7 +744,8 @@ int build_transposes(mpas_ctx *ctx, AdjState &A) {
   int rc, w = 0;
   if ((rc = upload_ragged(A, eT, false, &w, &T.nEt, &T.et, &T.wt, nullptr))) return rc;
   if ((rc = upload_ragged(A, vC, false, &w, &T.nVc, &T.vc, &T.vk, nullptr))) return rc;
-  if (m.fct && (rc = upload_ragged(A, aT, true, &w, &T.nAt, &T.at, &T.ac1, &T.ac3))) return rc;
+  const double *unused = nullptr;
+  if (m.fct && (rc = upload_ragged(A, aT, false, &w, &T.nAt, &T.at, &unused, nullptr))) return rc;
   return 0;
 }
 
@@ -707,12 +774,12 @@ int adj_alloc(mpas_ctx *ctx, int K) {
   for (Rec &r : A->rec)
     if ((rc = alloc_rec(*A, M, r))) return rc;
   double **e_bufs[] = {&A->nu, &A->dnu, &A->dhe, &A->dpve, &A->aw_u, &A->tb_u, &A->ubA,
-                       &A->heb, &A->pveb, &A->sg, &A->xb_u[0], &A->xb_u[1]};
+                       &A->heb, &A->pveb, &A->xb_u[0], &A->xb_u[1]};
   double **c_bufs[] = {&A->nh, &A->dnh, &A->dke, &A->ddiv, &A->aw_h, &A->tb_h, &A->keT,
                        &A->divT, &A->xb_h[0], &A->xb_h[1]};
   double **t_bufs[] = {&A->ntr, &A->tlo, &A->dQ, &A->aw_tr, &A->tb_tr, &A->xb_tr[0],
                        &A->xb_tr[1]};
-  double **et_bufs[] = {&A->dflux, &A->r1, &A->r2, &A->hbf};
+  double **et_bufs[] = {&A->dflux, &A->r1, &A->r2};
   double **v_bufs[] = {&A->pv_vertex, &A->dvort, &A->dpvv, &A->vT, &A->hvT};
   for (auto b : e_bufs)
     if ((rc = dalloc(*A, E, b))) return rc;
@@ -722,6 +789,7 @@ int adj_alloc(mpas_ctx *ctx, int K) {
     if ((rc = dalloc(*A, T, b))) return rc;
   for (auto b : et_bufs)
     if ((rc = dalloc(*A, ET, b))) return rc;
+  if (M.nT && (rc = dalloc(*A, ET * std::max(1, M.wAdv), &A->hbf))) return rc;
   for (auto b : v_bufs)
     if ((rc = dalloc(*A, V, b))) return rc;
   for (int s = 0; s < 5; s++)
@@ -843,11 +911,12 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
   for (int s = 3; s >= 0; s--) {
     const AdjParams P = adj_params(ctx, s, dt);
     const Rec &R = A.rec[4 * i + s];
-    ADJ_LAUNCH(k_ad_edge_a, M.nE, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr, A.ubA, A.heb, A.pveb,
-               A.r1, A.r2, A.hbf, A.sg);
+    ADJ_LAUNCH(k_ad_edge_a, int64_t(M.nE) * kEdgeLanes, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr,
+               A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
     ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
-    ADJ_LAUNCH(k_ad_cell, M.nC, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT, A.r1, A.r2, A.hbf, A.sg,
-               A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
+    ADJ_LAUNCH(k_ad_cell, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT,
+               A.r1, A.r2, A.hbf, A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr,
+               A.tb_h, A.tb_tr);
     ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
                A.tb_u);
   }
@@ -928,7 +997,20 @@ int mpas_adj_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, con
     if ((rc = copy_d2d(ctx, A.ck[3 * m], W.u, E)) || (rc = copy_d2d(ctx, A.ck[3 * m + 1], W.h, C)) ||
         (rc = copy_d2d(ctx, A.ck[3 * m + 2], W.tr, T)))
       return rc;
-    if ((rc = sw_rk4_steps(ctx, dt, len))) return rc;
+    if (m < nseg - 1) {
+      if ((rc = sw_rk4_steps(ctx, dt, len))) return rc;
+      continue;
+    }
+    // the last segment runs into the tape (the backward sweep starts on it
+    // without a replay); the same kernels, so the same state bit for bit
+    if ((rc = restore(ctx, A, A.rec[0], W.u, W.h, W.tr))) return rc;
+    for (int i = 0; i < len; i++)
+      if ((rc = record_step(ctx, A, i, dt))) return rc;
+    const Rec &f = A.rec[4 * len];
+    if ((rc = copy_d2d(ctx, W.u, f.u, E)) || (rc = copy_d2d(ctx, W.h, f.h, C)) ||
+        (rc = copy_d2d(ctx, W.tr, f.tr, T)) || (rc = sw_diagnostics(ctx, W.u, W.h, W.cur)))
+      return rc;
+    W.elapsed += dt * len;
   }
   if ((u1 && (rc = copy_d2d(ctx, u1, W.u, E))) || (h1 && (rc = copy_d2d(ctx, h1, W.h, C))) ||
       (tr1 && (rc = copy_d2d(ctx, tr1, W.tr, T))))
@@ -937,6 +1019,7 @@ int mpas_adj_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, con
   A.dt = dt;
   A.forward_done = true;
   A.taped = 0;
+  A.tape_seg = nseg - 1;
   return 0;
 }
 
@@ -956,10 +1039,13 @@ int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
   const int nseg = (A.nsteps + A.K - 1) / A.K;
   for (int m = nseg - 1; m >= 0; m--) {
     const int len = std::min(A.K, A.nsteps - m * A.K);
-    if ((rc = restore(ctx, A, A.rec[0], A.ck[3 * m], A.ck[3 * m + 1], A.ck[3 * m + 2])))
-      return rc;
-    for (int i = 0; i < len; i++)
-      if ((rc = record_step(ctx, A, i, A.dt))) return rc;
+    if (m != A.tape_seg) {  // replay the segment into the tape
+      if ((rc = restore(ctx, A, A.rec[0], A.ck[3 * m], A.ck[3 * m + 1], A.ck[3 * m + 2])))
+        return rc;
+      for (int i = 0; i < len; i++)
+        if ((rc = record_step(ctx, A, i, A.dt))) return rc;
+      A.tape_seg = m;
+    }
     for (int i = len - 1; i >= 0; i--) {
       if ((rc = adj_step(ctx, A, i, A.dt, src))) return rc;
       src = 1 - src;
@@ -984,6 +1070,7 @@ int mpas_tlm_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, con
   const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
   int rc;
   A.taped = 0;
+  A.tape_seg = -1;
   if ((rc = restore(ctx, A, A.rec[0], u0, h0, tr0))) return rc;
   if ((rc = copy_d2d(ctx, A.du[0], du0, E)) || (rc = copy_d2d(ctx, A.dh[0], dh0, C)) ||
       (rc = copy_d2d(ctx, A.dT[0], dtr0, T)))
