@@ -320,9 +320,15 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (level 6, 200 steps, 1 FCT tracer)")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=200,
+                    help="steps of the adjoint measurement (its workload has 200)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-member ensemble measurement")
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=500,
+                    help="batched steps of the ensemble measurement (its workload has 500)")
+    ap.add_argument("--ens-seq-steps", type=int, default=20,
+                    help="steps per member of the member-after-member baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel k_copy over BYTES, 10 times "
                          "(the known-size reference for HBM counters)")
@@ -339,10 +345,11 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.adjoint_only:
-        print(json.dumps(adjoint_bench()), flush=True)
+        print(json.dumps(adjoint_bench(steps=args.adj_steps)), flush=True)
         return
     if args.ensemble_only:
-        print(json.dumps(ensemble_bench()), flush=True)
+        print(json.dumps(ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps)),
+              flush=True)
         return
 
     t0 = time.perf_counter()
@@ -417,13 +424,13 @@ def main():
     }
     if not args.no_adjoint:
         log("adjoint (level 6, 200 steps) ...")
-        out["adjoint"] = adjoint_bench()
+        out["adjoint"] = adjoint_bench(steps=args.adj_steps)
         log(f"  adjoint/forward {out['adjoint']['adjoint_forward_cost_ratio']:.3f}, "
             f"gradient/forward {out['adjoint']['gradient_total_cost_ratio']:.3f}, "
             f"dot-product rel err {out['adjoint']['dot_product_rel_err']:.2e}")
     if not args.no_ensemble:
         log("ensemble (256 members, 500 steps) ...")
-        out["ensemble"] = ensemble_bench()
+        out["ensemble"] = ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps)
         log(f"  {out['ensemble']['ms_per_step_batched_graph']:.3f} ms/step batched, "
             f"{out['ensemble']['ms_per_step_sequential']:.3f} sequential, speedup "
             f"{out['ensemble']['speedup_vs_sequential']:.1f}")
