@@ -362,13 +362,16 @@ int mpas_batch_sync(mpas_batch *batch);
    high-order flux, cells, tracer update, vertex and edge diagnostics;
    multiples of 64 up to 512), members per lane (consecutive members of one
    entity a lane advances), workgroup order (0 group after group, 1 the
-   groups' workgroups interleaved) and XCD-contiguous placement (0/1). */
+   groups' workgroups interleaved), XCD-contiguous placement (0/1) and one
+   entity per wave (each entity's lanes padded to whole waves, so the static
+   mesh loads are scalar: 0 for no group, 1 for the groups whose padded
+   lanes stay at least 3/4 busy, 2 for every group). */
 int mpas_batch_config(mpas_batch *batch, const int32_t *block, int members_per_lane,
-                      int order, int xcd);
+                      int order, int xcd, int wave_entity);
 int mpas_batch_get_config(const mpas_batch *batch, int32_t *block, int *members_per_lane,
-                          int *order, int *xcd);
-/* Times trial_steps graph replays per candidate (members per lane,
-   placement and order, then each kernel's workgroup size), keeps the
+                          int *order, int *xcd, int *wave_entity);
+/* Times trial_steps graph replays per candidate (entity per wave and
+   members per lane, placement and order, then each kernel's workgroup size), keeps the
    fastest configuration and restores the members' state; writes the
    trials as JSON into report. */
 int mpas_batch_autotune(mpas_batch *batch, int trial_steps, char *report, size_t cap);

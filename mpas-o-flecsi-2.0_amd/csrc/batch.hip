@@ -32,7 +32,7 @@ namespace mpasx {
 
 // Device-side descriptor of one group.
 struct BatchGroup {
-  DMesh M;       // the mesh context's view with nL = members of the group
+  DMesh M;  // the mesh context's view with nL = members of the group
   StageIO S[4];  // stage s of a step (the ping-pong roles repeat every step)
   double *vort_y[4], *h_edge_y[4], *pv_edge_y[4];
   double *pv_vertex;
@@ -79,10 +79,16 @@ struct mpas_batch {
   // launch configuration and its tables
   int block[mpasx::kBatchKernels] = {256, 256, 256, 256, 256, 256};
   int lpl = 1;    // members per lane
+  // one entity per wave (members padded to whole waves): 0 for no group,
+  // 1 for the groups that keep at least 3/4 of the padded lanes busy, 2 for
+  // every group
+  int wave_entity = 1;
   int order = 0;  // 0: group after group, 1: the groups' workgroups interleaved
   int xcd = 1;
-  int2 *d_tab[mpasx::kBatchKernels] = {};
-  int grid[mpasx::kBatchKernels] = {};
+  // workgroup tables per kernel and layout (index mode - 1); grid 0: no
+  // group of that layout
+  int2 *d_tab[mpasx::kBatchKernels][2] = {};
+  int grid[mpasx::kBatchKernels][2] = {};
   bool tables_ok = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -104,41 +110,53 @@ __device__ __forceinline__ int batch_block(const BatchArgs &A) {
   const BatchGroup &g = A.grp[t.x];                                       \
   const int item = t.y * int(blockDim.x) + int(threadIdx.x)
 
-template <int WE>
+// MODE: the groups' layout (1: levels on lanes, 2: the same with one
+// entity per wave); the groups of each layout have their own launch.
+template <int WE, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_edges<WE, true>(g.M, g.S[s], item);
+  stage_edges<WE, MODE>(g.M, g.S[s], item);
 }
-template <int NT, int WA>
+template <int NT, int WA, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_ho<NT, WA, true>(g.M, g.S[s], item);
+  stage_ho<NT, WA, MODE>(g.M, g.S[s], item);
 }
-template <int NT, int W>
+template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_cells<NT, W, true>(g.M, g.S[s], item);
+  stage_cells<NT, W, MODE>(g.M, g.S[s], item);
 }
-template <int NT, int W>
+template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_tracer<NT, W, true>(g.M, g.S[s], item);
+  stage_tracer<NT, W, MODE>(g.M, g.S[s], item);
 }
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_vertex(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
+template <int MODE>
+__device__ __forceinline__ void diag_vertex_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
-  const int v = split_levels<true, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
+  const int v = split_levels<MODE, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
   if (v < g.M.nV)
     diag_vertex(g.M, v, g.S[s].yh, g.S[s].yu, nullptr, nullptr, g.vort_y[s], g.pv_vertex, k0, k1);
 }
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_edge(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
+template <int MODE>
+__device__ __forceinline__ void diag_edge_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
-  const int e = split_levels<true, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
+  const int e = split_levels<MODE, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
   if (e < g.M.nE)
     diag_edge(g.M, e, g.S[s].yh, g.pv_vertex, g.h_edge_y[s], g.pv_edge_y[s], k0, k1);
+}
+template <int MODE>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_vertex(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  diag_vertex_lanes<MODE>(g, s, item);
+}
+template <int MODE>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_edge(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  diag_edge_lanes<MODE>(g, s, item);
 }
 #undef BATCH_PROLOGUE
 
@@ -161,27 +179,45 @@ __global__ __launch_bounds__(256) void kb_member_gather(double *dst, const doubl
 
 int lanes_per_entity(int nT) { return nT > 0 ? nT / (nT % 2 == 0 ? 2 : 1) : 1; }
 
+// Whether group q runs one entity per wave.
+bool group_wave_entity(const mpas_batch *B, int q) {
+  if (B->wave_entity != 1) return B->wave_entity == 2;
+  const int64_t nch = (int64_t(B->groups[q].members.size()) + B->lpl - 1) / B->lpl;
+  const int64_t n = nch * lanes_per_entity(B->nT);
+  return 4 * n >= 3 * ((n + 63) / 64 * 64);
+}
+
+// Lanes per entity of kernel k (tracer lane groups npl when it has them):
+// nchunk * npl, padded to a multiple of the wave size with one entity per
+// wave.
+int64_t entity_lanes(const mpas_batch *B, int q, bool tracer_lanes) {
+  const GroupInfo &G = B->groups[q];
+  const int64_t npl = tracer_lanes ? lanes_per_entity(B->nT) : 1;
+  const int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
+  const int64_t n = nch * npl;
+  return group_wave_entity(B, q) ? (n + 63) / 64 * 64 : n;
+}
+
 // Work items of kernel k for group q.
 int64_t kernel_items(const mpas_batch *B, int k, int q) {
-  const GroupInfo &G = B->groups[q];
-  const DMesh &M = B->meshes[G.ctx]->dm;
-  const int64_t npl = lanes_per_entity(B->nT);
-  const int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
+  const DMesh &M = B->meshes[B->groups[q].ctx]->dm;
   switch (k) {
-    case 0: return M.nE * nch;
-    case 1: return B->nT ? M.nE * nch * npl : 0;
-    case 2: return M.nC * nch * npl;
-    case 3: return B->nT ? M.nC * nch * npl : 0;
-    case 4: return M.nV * nch;
-    default: return M.nE * nch;
+    case 0: return M.nE * entity_lanes(B, q, false);
+    case 1: return B->nT ? M.nE * entity_lanes(B, q, true) : 0;
+    case 2: return M.nC * entity_lanes(B, q, true);
+    case 3: return B->nT ? M.nC * entity_lanes(B, q, true) : 0;
+    case 4: return M.nV * entity_lanes(B, q, false);
+    default: return M.nE * entity_lanes(B, q, false);
   }
 }
 
 void free_tables(mpas_batch *B) {
-  for (int k = 0; k < kBatchKernels; k++) {
-    if (B->d_tab[k]) (void)hipFree(B->d_tab[k]);
-    B->d_tab[k] = nullptr;
-  }
+  for (int k = 0; k < kBatchKernels; k++)
+    for (int m = 0; m < 2; m++) {
+      if (B->d_tab[k][m]) (void)hipFree(B->d_tab[k][m]);
+      B->d_tab[k][m] = nullptr;
+      B->grid[k][m] = 0;
+    }
   B->tables_ok = false;
 }
 
@@ -246,54 +282,73 @@ int build_tables(mpas_batch *B) {
   MPAS_HIP(hipMemcpy(B->d_grp, B->desc.data(), sizeof(BatchGroup) * B->desc.size(),
                      hipMemcpyHostToDevice));
   const int ng = int(B->groups.size());
-  for (int k = 0; k < kBatchKernels; k++) {
-    const int bs = B->block[k];
-    std::vector<int> nb(ng);
-    for (int q = 0; q < ng; q++) nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
-    std::vector<int2> tab;
-    if (B->order == 1) {  // round-robin over the groups' blocks
-      for (int j = 0, more = 1; more; j++) {
-        more = 0;
-        for (int q = 0; q < ng; q++)
-          if (j < nb[q]) tab.push_back(make_int2(q, j)), more = 1;
-      }
-    } else {
+  for (int k = 0; k < kBatchKernels; k++)
+    for (int mode = 1; mode <= 2; mode++) {
+      const int bs = B->block[k];
+      std::vector<int> nb(ng, 0);
       for (int q = 0; q < ng; q++)
-        for (int j = 0; j < nb[q]; j++) tab.push_back(make_int2(q, j));
+        if (int(group_wave_entity(B, q)) + 1 == mode)
+          nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
+      std::vector<int2> tab;
+      if (B->order == 1) {  // round-robin over the groups' blocks
+        for (int j = 0, more = 1; more; j++) {
+          more = 0;
+          for (int q = 0; q < ng; q++)
+            if (j < nb[q]) tab.push_back(make_int2(q, j)), more = 1;
+        }
+      } else {
+        for (int q = 0; q < ng; q++)
+          for (int j = 0; j < nb[q]; j++) tab.push_back(make_int2(q, j));
+      }
+      if (tab.empty()) continue;
+      while (tab.size() % kXcds) tab.push_back(make_int2(-1, 0));
+      B->grid[k][mode - 1] = int(tab.size());
+      MPAS_HIP(hipMalloc(&B->d_tab[k][mode - 1], sizeof(int2) * tab.size()));
+      MPAS_HIP(hipMemcpy(B->d_tab[k][mode - 1], tab.data(), sizeof(int2) * tab.size(),
+                         hipMemcpyHostToDevice));
     }
-    while (tab.empty() || tab.size() % kXcds) tab.push_back(make_int2(-1, 0));
-    B->grid[k] = int(tab.size());
-    MPAS_HIP(hipMalloc(&B->d_tab[k], sizeof(int2) * tab.size()));
-    MPAS_HIP(hipMemcpy(B->d_tab[k], tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
-  }
   B->tables_ok = true;
   return 0;
 }
 
-BatchArgs args_of(const mpas_batch *B, int k) { return BatchArgs{B->d_grp, B->d_tab[k], B->xcd}; }
+BatchArgs args_of(const mpas_batch *B, int k, int mode) {
+  return BatchArgs{B->d_grp, B->d_tab[k][mode - 1], B->xcd};
+}
+
+// One phase: the launch of the groups of each layout.
+#define BL(k, kern)                                                                       \
+  do {                                                                                    \
+    if (B->grid[k][0]) {                                                                  \
+      constexpr int MODE = 1;                                                             \
+      hipLaunchKernelGGL(kern, dim3(B->grid[k][0]), dim3(B->block[k]), 0, st,             \
+                         args_of(B, k, 1), s);                                            \
+      if (int rc = after_launch(#kern)) return rc;                                        \
+    }                                                                                     \
+    if (B->grid[k][1]) {                                                                  \
+      constexpr int MODE = 2;                                                             \
+      hipLaunchKernelGGL(kern, dim3(B->grid[k][1]), dim3(B->block[k]), 0, st,             \
+                         args_of(B, k, 2), s);                                            \
+      if (int rc = after_launch(#kern)) return rc;                                        \
+    }                                                                                     \
+  } while (0)
 
 template <int NT, int W, int WE>
 int launch_batch_stage_t(mpas_batch *B, int s) {
   hipStream_t st = B->stream;
   const bool fct = NT > 0 && B->desc[0].S[0].scheme == MPAS_TRACER_FCT;
-#define BL(k, kern)                                                                     \
-  do {                                                                                  \
-    hipLaunchKernelGGL(kern, dim3(B->grid[k]), dim3(B->block[k]), 0, st, args_of(B, k), s); \
-    if (int rc = after_launch(#kern)) return rc;                                        \
-  } while (0)
-  BL(0, (kb_stage_edges<WE>));
+  BL(0, (kb_stage_edges<WE, MODE>));
   if constexpr (NT > 0) {
-    if (fct) BL(1, (kb_stage_ho<NT, WE>));
+    if (fct) BL(1, (kb_stage_ho<NT, WE, MODE>));
   }
-  BL(2, (kb_stage_cells<NT, W>));
+  BL(2, (kb_stage_cells<NT, W, MODE>));
   if constexpr (NT > 0) {
-    if (fct) BL(3, (kb_stage_tracer<NT, W>));
+    if (fct) BL(3, (kb_stage_tracer<NT, W, MODE>));
   }
-  BL(4, kb_diag_vertex);
-  BL(5, kb_diag_edge);
-#undef BL
+  BL(4, kb_diag_vertex<MODE>);
+  BL(5, kb_diag_edge<MODE>);
   return 0;
 }
+#undef BL
 
 template <int NT>
 int launch_batch_stage_w(mpas_batch *B, int s) {
@@ -595,7 +650,7 @@ int mpas_batch_sync(mpas_batch *B) {
 int mpas_batch_groups(const mpas_batch *B) { return B ? int(B->groups.size()) : MPAS_EINVAL; }
 
 int mpas_batch_config(mpas_batch *B, const int32_t *block, int members_per_lane, int order,
-                      int xcd) {
+                      int xcd, int wave_entity) {
   if (int rc = check_batch(B)) return rc;
   if (order < 0 || order > 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: order is 0 or 1");
   if (members_per_lane < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: members_per_lane < 1");
@@ -608,16 +663,20 @@ int mpas_batch_config(mpas_batch *B, const int32_t *block, int members_per_lane,
   B->lpl = members_per_lane;
   B->order = order;
   B->xcd = xcd != 0;
+  if (wave_entity < 0 || wave_entity > 2)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: wave_entity is 0, 1 or 2");
+  B->wave_entity = wave_entity;
   return build_tables(B);
 }
 
 int mpas_batch_get_config(const mpas_batch *B, int32_t *block, int *members_per_lane, int *order,
-                          int *xcd) {
+                          int *xcd, int *wave_entity) {
   if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
   if (block) std::copy(B->block, B->block + kBatchKernels, block);
   if (members_per_lane) *members_per_lane = B->lpl;
   if (order) *order = B->order;
   if (xcd) *xcd = B->xcd;
+  if (wave_entity) *wave_entity = B->wave_entity;
   return 0;
 }
 
@@ -641,25 +700,29 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     char line[256];
     std::snprintf(line, sizeof line,
                   "%s{\"trial\": \"%s\", \"block\": [%d, %d, %d, %d, %d, %d], "
-                  "\"members_per_lane\": %d, \"order\": %d, \"xcd\": %d, \"ms_per_step\": %.4f}",
+                  "\"members_per_lane\": %d, \"wave_entity\": %d, \"order\": %d, \"xcd\": %d, "
+                  "\"ms_per_step\": %.4f}",
                   log.size() > 1 ? ", " : "", what, B->block[0], B->block[1], B->block[2],
-                  B->block[3], B->block[4], B->block[5], B->lpl, B->order, B->xcd,
-                  1e3 * t / trial_steps);
+                  B->block[3], B->block[4], B->block[5], B->lpl, B->wave_entity, B->order,
+                  B->xcd, 1e3 * t / trial_steps);
     log += line;
     return rc ? 1e30 : t;
   };
   double best = trial("start");
   size_t gmax = 1;
   for (const GroupInfo &G : B->groups) gmax = std::max(gmax, G.members.size());
-  // members per lane, placement and order, then each kernel's workgroup size
-  for (int lpl : {1, 2, 4, 8, 16}) {
-    if (lpl == B->lpl || size_t(lpl) > gmax) continue;
-    const int keep = B->lpl;
-    B->lpl = lpl;
-    const double t = trial("members_per_lane");
-    if (t < best) best = t;
-    else B->lpl = keep;
-  }
+  // entity per wave and members per lane, placement and order, then each
+  // kernel's workgroup size
+  for (int we = 0; we < 3; we++)
+    for (int lpl : {1, 2, 4, 8, 16}) {
+      if ((lpl == B->lpl && we == B->wave_entity) || size_t(lpl) > gmax) continue;
+      const int keep = B->lpl, kwe = B->wave_entity;
+      B->lpl = lpl;
+      B->wave_entity = we;
+      const double t = trial("members_per_lane");
+      if (t < best) best = t;
+      else B->lpl = keep, B->wave_entity = kwe;
+    }
   for (int x = 0; x < 2; x++)
     for (int o = 0; o < 2; o++) {
       if (x == B->xcd && o == B->order) continue;

@@ -88,6 +88,21 @@ __device__ __forceinline__ T ld(const T *__restrict__ p, uint32_t i) {
   const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);
   return *(const MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T)));
 }
+// Loads of the static mesh arrays (never written while a kernel runs) in
+// the constant address space: when the index is the same for every lane of
+// a wave (one mesh entity per wave, as the ensemble kernels arrange it) the
+// compiler turns them into scalar loads, which leave the vector memory path
+// to the state; with a per-lane index they are ordinary vector loads.
+#ifdef __HIP_DEVICE_COMPILE__
+#define MPAS_CONST __attribute__((address_space(4)))
+#else
+#define MPAS_CONST
+#endif
+template <class T>
+__device__ __forceinline__ T ldc(const T *__restrict__ p, uint32_t i) {
+  const MPAS_CONST char *b = (const MPAS_CONST char *)(p);
+  return *(const MPAS_CONST T *)(b + i * uint32_t(sizeof(T)));
+}
 template <class T>
 __device__ __forceinline__ void st(T *p, uint32_t i, T v) {
   MPAS_GLOBAL char *b = (MPAS_GLOBAL char *)(p);
@@ -95,10 +110,10 @@ __device__ __forceinline__ void st(T *p, uint32_t i, T v) {
 }
 
 __device__ __forceinline__ int lvl_cell(const DMesh &M, int c) {
-  return M.mlc ? ld(M.mlc, uint32_t(c)) : M.nL;
+  return M.mlc ? ldc(M.mlc, uint32_t(c)) : M.nL;
 }
 __device__ __forceinline__ int lvl_edge(const DMesh &M, int e) {
-  return M.mlet ? ld(M.mlet, uint32_t(e)) : M.nL;
+  return M.mlet ? ldc(M.mlet, uint32_t(e)) : M.nL;
 }
 
 // Grid-stride helpers: one work item per lane.

@@ -16,12 +16,12 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
                                             double *pv_vertex, uint32_t lv0 = 0,
                                             uint32_t lv1 = ~0u) {
   const uint32_t L = M.nL, V = M.nV;
-  const int x0 = ld(M.v2e, v), x1 = ld(M.v2e, V + v), x2 = ld(M.v2e, 2 * V + v);
+  const int x0 = ldc(M.v2e, v), x1 = ldc(M.v2e, V + v), x2 = ldc(M.v2e, 2 * V + v);
   const uint32_t e0 = v2e_id(x0), e1 = v2e_id(x1), e2 = v2e_id(x2);
-  const uint32_t c0 = ld(M.v2c, v), c1 = ld(M.v2c, V + v), c2 = ld(M.v2c, 2 * V + v);
-  const double k0 = ld(M.kite, v), k1 = ld(M.kite, V + v), k2 = ld(M.kite, 2 * V + v);
-  const double d0 = ld(M.dc, e0), d1 = ld(M.dc, e1), d2 = ld(M.dc, e2);
-  const double at = ld(M.areaTri, v), f = ld(M.fV, v);
+  const uint32_t c0 = ldc(M.v2c, v), c1 = ldc(M.v2c, V + v), c2 = ldc(M.v2c, 2 * V + v);
+  const double k0 = ldc(M.kite, v), k1 = ldc(M.kite, V + v), k2 = ldc(M.kite, 2 * V + v);
+  const double d0 = ldc(M.dc, e0), d1 = ldc(M.dc, e1), d2 = ldc(M.dc, e2);
+  const double at = ldc(M.areaTri, v), f = ldc(M.fV, v);
   for (uint32_t k = lv0; k < min(lv1, L); k++) {
     double circ = 0.0;
     circ = circ + v2e_sign(x0) * (d0 * ld(u, e0 * L + k));
@@ -48,8 +48,8 @@ __device__ __forceinline__ void diag_edge(const DMesh &M, int e, const double *_
                                           double *h_edge, double *pv_edge, uint32_t lv0 = 0,
                                           uint32_t lv1 = ~0u) {
   const uint32_t L = M.nL;
-  const int2 cc = ld(M.e2c, e);
-  const int2 vv = ld(M.e2v, e);
+  const int2 cc = ldc(M.e2c, e);
+  const int2 vv = ldc(M.e2v, e);
   const uint32_t va = min(vv.x, vv.y), vb = max(vv.x, vv.y);
   for (uint32_t k = lv0; k < min(lv1, L); k++) {
     const uint32_t o = e * L + k;
@@ -71,17 +71,17 @@ __device__ __forceinline__ double tend_u_at(const DMesh &M, int e, int k, int2 c
                                             const double *__restrict__ h_s) {
   const uint32_t L = M.nL, E = M.nE;
   const double pve = ld(pv_edge, e * L + k);
-  const int n = ld(M.nEoE, e);
+  const int n = ldc(M.nEoE, e);
   double q = 0;
   for (int j = 0; j < n; j++) {
-    const uint32_t eoe = ld(M.eoe, j * E + e);
+    const uint32_t eoe = ldc(M.eoe, j * E + e);
     const double workpv = 0.5 * (pve + ld(pv_edge, eoe * L + k));
-    q = q + ld(M.woe, j * E + e) * ld(u, eoe * L + k) * workpv * ld(h_edge, eoe * L + k);
+    q = q + ldc(M.woe, j * E + e) * ld(u, eoe * L + k) * workpv * ld(h_edge, eoe * L + k);
   }
   const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
   return q - (ld(ke, o2) - ld(ke, o1) +
               M.gravity * (ld(h, o2) + ld(h_s, cc.y) - ld(h, o1) - ld(h_s, cc.x))) /
-                 ld(M.dc, e);
+                 ldc(M.dc, e);
 }
 
 // del2 viscosity on u (no reference counterpart):
@@ -91,9 +91,9 @@ __device__ __forceinline__ double visc_at(const DMesh &M, int e, int k, int2 cc,
                                           const double *__restrict__ vort, double coef) {
   const uint32_t L = M.nL;
   const double ud = (ld(div, uint32_t(cc.y) * L + k) - ld(div, uint32_t(cc.x) * L + k)) /
-                        ld(M.dc, e) -
+                        ldc(M.dc, e) -
                     (ld(vort, uint32_t(vv.y) * L + k) - ld(vort, uint32_t(vv.x) * L + k)) /
-                        ld(M.dv, e);
+                        ldc(M.dv, e);
   return coef * ud;
 }
 

@@ -82,27 +82,41 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 // adjacent and the first of them writes the entity's non-tracer outputs.
 
 // Levels of a work item.  The single-run kernels loop over every level in
-// each lane (LOL = false: the static stencil data of an entity is loaded once
-// and reused for all levels).  With levels on lanes (LOL = true) a lane takes
+// each lane (LOL = 0: the static stencil data of an entity is loaded once
+// and reused for all levels).  With levels on lanes (LOL = 1) a lane takes
 // S.lpl consecutive levels of one entity, and the lanes of an entity cover
 // its S.nchunk chunks of levels side by side: when the levels are ensemble
 // members (every path kernel treats levels independently), neighbouring
 // lanes then read neighbouring members of the same entity, so every state
-// access and gather is one contiguous run and the stencil loads are shared.
-// Item order: entity, then level chunk, then tracer lane group.
-template <bool LOL, int NPL>
+// access and gather is one contiguous run.  Item order: entity, then level
+// chunk, then tracer lane group.  LOL = 2 pads each entity's lanes to a
+// multiple of the wave size, so all lanes of a wave share one entity: the
+// entity index is then wave-uniform (taken from the wave's first lane) and
+// every static mesh load becomes a scalar load; padding lanes get no
+// levels.
+template <int LOL, int NPL>
 __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int item, int &sub,
                                             uint32_t &k0, uint32_t &k1) {
-  const int q = item / NPL;
-  sub = item - q * NPL;
-  if constexpr (!LOL) {
+  if constexpr (LOL == 0) {
+    const int q = item / NPL;
+    sub = item - q * NPL;
     k0 = 0;
     k1 = L;
     return q;
-  } else {
+  } else if constexpr (LOL == 1) {
+    const int q = item / NPL;
+    sub = item - q * NPL;
     const int ent = q / S.nchunk, ch = q - ent * S.nchunk;
     k0 = uint32_t(ch) * uint32_t(S.lpl);
     k1 = min(k0 + uint32_t(S.lpl), L);
+    return ent;
+  } else {
+    const int pe = (S.nchunk * NPL + 63) / 64 * 64;
+    const int ent = __builtin_amdgcn_readfirstlane(item) / pe;
+    const int r = item - ent * pe, q = r / NPL;
+    sub = r - q * NPL;
+    k0 = uint32_t(q) * uint32_t(S.lpl);
+    k1 = q < S.nchunk ? min(k0 + uint32_t(S.lpl), L) : k0;
     return ent;
   }
 }
@@ -111,7 +125,7 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-template <int WE, bool LOL = false>
+template <int WE, int LOL = 0>
 __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
   int sub;
   uint32_t k0, k1;
@@ -123,16 +137,16 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
       for (uint32_t k = k0; k < k1; k++) st(S.yu, e * L + k, ld(S.u0, e * L + k));
     return;
   }
-  const int2 cc = ld(M.e2c, e);
-  const int n = ld(M.nEoE, e);
+  const int2 cc = ldc(M.e2c, e);
+  const int n = ldc(M.nEoE, e);
   uint32_t id[WE];
   double w[WE];
 #pragma unroll
   for (int j = 0; j < WE; j++) {
-    id[j] = ld(M.eoe, j * E + e);
-    w[j] = ld(M.woe, j * E + e);
+    id[j] = ldc(M.eoe, j * E + e);
+    w[j] = ldc(M.woe, j * E + e);
   }
-  const double dc = ld(M.dc, e);
+  const double dc = ldc(M.dc, e);
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     double uu[WE], pv[WE], he[WE];
@@ -153,10 +167,10 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
     }
     const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
     double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
-                    M.gravity * (ld(S.xh, o2) + ld(S.h_s, cc.y) - ld(S.xh, o1) -
-                                 ld(S.h_s, cc.x))) /
+                    M.gravity * (ld(S.xh, o2) + ldc(S.h_s, cc.y) - ld(S.xh, o1) -
+                                 ldc(S.h_s, cc.x))) /
                        dc;
-    if (S.use_visc) t += visc_at(M, e, k, cc, ld(M.e2v, e), S.div, S.vort, S.visc_coef);
+    if (S.use_visc) t += visc_at(M, e, k, cc, ldc(M.e2v, e), S.div, S.vort, S.visc_coef);
     if (S.stage == 0) {
       const double u0 = ld(S.u0, o);
       st(S.yu, o, u0 + S.w_sub * t);
@@ -174,7 +188,7 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // calc_normal_thickness_flux, the high-order flux and its correction by the
 // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
 // lane.
-template <int NT, int WA, bool LOL = false>
+template <int NT, int WA, int LOL = 0>
 __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
@@ -183,21 +197,21 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
   const int t0 = sub * TPL;
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
-  const int2 cc = ld(M.e2c, e);
-  const double dv = ld(M.dv, e);
-  const int na = ld(M.nAdv, e);
+  const int2 cc = ldc(M.e2c, e);
+  const double dv = ldc(M.dv, e);
+  const int na = ldc(M.nAdv, e);
   uint32_t ic[WA];
   double c1[WA], c3[WA];
 #pragma unroll
   for (int i = 0; i < WA; i++) {
-    ic[i] = ld(M.adv, i * E + e);
-    c1[i] = ld(M.advCoef, i * E + e);
-    c3[i] = ld(M.advCoef3, i * E + e) * M.coef3rd;
+    ic[i] = ldc(M.adv, i * E + e);
+    c1[i] = ldc(M.advCoef, i * E + e);
+    c3[i] = ldc(M.advCoef3, i * E + e) * M.coef3rd;
   }
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     const double F = ld(S.xu, o) * ld(S.h_edge, o);
-    const int mask = M.hoMask ? ld(M.hoMask, o) : 1;
+    const int mask = M.hoMask ? ldc(M.hoMask, o) : 1;
     const double wgt = F * double(mask);
     const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
     double tv[WA][TPL];
@@ -254,7 +268,7 @@ __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uin
 // cell's first lane stores them), and per tracer either the centred
 // tendency with its update or the FCT bounds, upwind tendency and limiter
 // factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
-template <int NT, int W, bool LOL = false>
+template <int NT, int W, int LOL = 0>
 __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = NT > 0 ? tracers_per_lane<NT>() : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
@@ -264,17 +278,17 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
-  const int n = ld(M.nEoC, c);
-  const uint8_t neg = ld(M.sgnNeg, c);
-  const uint32_t rank = ld(M.ascRank, c);
-  const double ar = ld(M.area, c);
+  const int n = ldc(M.nEoC, c);
+  const uint8_t neg = ldc(M.sgnNeg, c);
+  const uint32_t rank = ldc(M.ascRank, c);
+  const double ar = ldc(M.area, c);
   const bool lead = t0 == 0;
   uint32_t e[W];
   double dv[W];
 #pragma unroll
-  for (int j = 0; j < W; j++) e[j] = ld(M.c2e, j * C + c);
+  for (int j = 0; j < W; j++) e[j] = ldc(M.c2e, j * C + c);
 #pragma unroll
-  for (int j = 0; j < W; j++) dv[j] = ld(M.dv, e[j]);
+  for (int j = 0; j < W; j++) dv[j] = ldc(M.dv, e[j]);
 
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
@@ -284,7 +298,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
       for (int j = 0; j < W; j++) {
         yu[j] = ld(S.yu, e[j] * L + k);
-        dc[j] = ld(M.dc, e[j]);
+        dc[j] = ldc(M.dc, e[j]);
       }
       double ke = 0.0, dvg = 0.0;
 #pragma unroll
@@ -334,7 +348,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       {
         uint32_t op[W];
 #pragma unroll
-        for (int j = 0; j < W; j++) op[j] = ld(M.cOpp, j * C + c);
+        for (int j = 0; j < W; j++) op[j] = ldc(M.cOpp, j * C + c);
 #pragma unroll
         for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
       }
@@ -356,7 +370,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
             for (int j = 0; j < W; j++) {
               const bool first = (neg >> j) & 1;
-              const double rho_scaling = M.msd * S.kappa * dv[j] / ld(M.dc, e[j]);
+              const double rho_scaling = M.msd * S.kappa * dv[j] / ldc(M.dc, e[j]);
               const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
               const double flux = he[j] * turb * rho_scaling;
               const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
@@ -431,7 +445,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 }
 
 // ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
-template <int NT, int W, bool LOL = false>
+template <int NT, int W, int LOL = 0>
 __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
@@ -440,14 +454,14 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
-  const int n = ld(M.nEoC, c);
-  const uint8_t neg = ld(M.sgnNeg, c);
-  const double invAreaCell1 = 1.0 / ld(M.area, c);
+  const int n = ldc(M.nEoC, c);
+  const uint8_t neg = ldc(M.sgnNeg, c);
+  const double invAreaCell1 = 1.0 / ldc(M.area, c);
   uint32_t e[W], op[W];
 #pragma unroll
   for (int j = 0; j < W; j++) {
-    e[j] = ld(M.c2e, j * C + c);
-    op[j] = ld(M.cOpp, j * C + c);
+    e[j] = ldc(M.c2e, j * C + c);
+    op[j] = ldc(M.cOpp, j * C + c);
   }
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k, ot = ol * NT + t0;
@@ -498,7 +512,7 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
       for (int j = 0; j < W; j++) {
         ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
         he[j] = ld(S.h_edge, e[j] * L + k);
-        rho[j] = M.msd * S.kappa * ld(M.dv, e[j]) / ld(M.dc, e[j]);
+        rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
       }
 #pragma unroll
       for (int j = 0; j < W; j++) {

@@ -143,8 +143,8 @@ SIGNATURES = {
     "mpas_batch_step": (c_int, [c_void_p, c_int, c_int]),
     "mpas_batch_sync": (c_int, [c_void_p]),
     "mpas_batch_groups": (c_int, [c_void_p]),
-    "mpas_batch_config": (c_int, [c_void_p, P_i32, c_int, c_int, c_int]),
-    "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 3),
+    "mpas_batch_config": (c_int, [c_void_p, P_i32, c_int, c_int, c_int, c_int]),
+    "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 4),
     "mpas_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
 }
 

@@ -445,22 +445,23 @@ class Batch:
     def groups(self):
         return L.check_count(L.hip().mpas_batch_groups(self._b), "mpas_batch_groups")
 
-    def config(self, block=None, members_per_lane=1, order=0, xcd=1):
+    def config(self, block=None, members_per_lane=1, order=0, xcd=1, wave_entity=1):
         blk = None
         if block is not None:
             blk = np.ascontiguousarray(block, dtype=np.int32)
         L.check(L.hip().mpas_batch_config(self._b, blk.ctypes.data_as(L.P_i32) if blk is not None
-                                          else None, int(members_per_lane), int(order), int(xcd)),
-                "mpas_batch_config")
+                                          else None, int(members_per_lane), int(order), int(xcd),
+                                          int(wave_entity)), "mpas_batch_config")
 
     def get_config(self):
         blk = np.zeros(6, dtype=np.int32)
-        lpl, o, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lpl, o, x, we = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         L.check(L.hip().mpas_batch_get_config(self._b, blk.ctypes.data_as(L.P_i32),
                                               ctypes.byref(lpl), ctypes.byref(o),
-                                              ctypes.byref(x)), "mpas_batch_get_config")
+                                              ctypes.byref(x), ctypes.byref(we)),
+                "mpas_batch_get_config")
         return {"block": blk.tolist(), "members_per_lane": lpl.value, "order": o.value,
-                "xcd": x.value}
+                "xcd": x.value, "wave_entity": we.value}
 
     def autotune(self, trial_steps=3):
         import json

@@ -76,15 +76,16 @@ def test_members_equal_single_runs_both_modes():
 def test_configurations_keep_results():
     ctxs, b, states, mesh_of = _ensemble()
     ref = _singles(ctxs, states, mesh_of, 4)
-    for blk, lpl, order, xcd in (([64] * 6, 1, 1, 0), ([512] * 6, 2, 0, 1),
-                                 ([128, 256, 64, 512, 128, 64], 3, 1, 1), ([256] * 6, 8, 0, 0)):
-        b.config(blk, lpl, order, xcd)
+    for blk, lpl, order, xcd, we in (([64] * 6, 1, 1, 0, 0), ([512] * 6, 2, 0, 1, 0),
+                                     ([128, 256, 64, 512, 128, 64], 3, 1, 1, 1),
+                                     ([256] * 6, 8, 0, 0, 2), ([256] * 6, 1, 0, 1, 2)):
+        b.config(blk, lpl, order, xcd, we)
         _load(b, states)
         b.step(4)
         b.sync()
         for i in range(b.n):
             for a, r in zip(b.download(i), ref[i]):
-                assert bitwise_equal(a, r), (blk, lpl, order, xcd, i)
+                assert bitwise_equal(a, r), (blk, lpl, order, xcd, we, i)
     _close(ctxs, b)
 
 
