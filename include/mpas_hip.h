@@ -292,6 +292,13 @@ int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host,
 /* nsteps calls of advance_timestep (apps/sw/run.cc:13-70, loop 97-106),
    with the stage-fused kernels; the result equals the per-task sequence. */
 int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps);
+/* nsteps calls of the tracer-only ocean driver's rk4_timestep
+   (apps/ocean/run.cc:10-50, loop 70-72): the uploaded
+   u (normalVelocity) and h (layerThickness) stay fixed, layerThicknessEdge
+   is the h_edge of the upload (init_functions.cc:30-45), and the tracers
+   are advanced with FCT and, when kappa2 != 0, del2 mixing, by the same
+   stage-fused kernels; the result equals the per-task sequence. */
+int mpas_ocean_rk4_step(mpas_ctx *ctx, double dt, int nsteps);
 
 /* ---- tangent-linear model and discrete adjoint of mpas_rk4_step ----------
    No counterpart in the reference (it has no derivative code); the forward

@@ -523,6 +523,22 @@ int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps) {
   return sw_rk4_steps(ctx, dt, nsteps);
 }
 
+int mpas_ocean_rk4_step(mpas_ctx *ctx, double dt, int nsteps) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->sw || !ctx->sw->ready)
+    MPAS_FAIL(MPAS_ESTATE, "mpas_ocean_rk4_step: upload a state first (mpas_sw_state_upload)");
+  if (nsteps < 0) MPAS_FAIL(MPAS_EINVAL, "mpas_ocean_rk4_step: nsteps < 0");
+  if (!(dt > 0)) MPAS_FAIL(MPAS_EINVAL, "mpas_ocean_rk4_step: dt must be positive");
+  const DMesh &M = ctx->dm;
+  if (!M.nT || ctx->params.tracer_scheme != MPAS_TRACER_FCT)
+    MPAS_FAIL(MPAS_EINVAL, "the fused ocean run advects tracers with FCT (use the per-task "
+                           "entry points for del2 mixing alone)");
+  if (M.mlc || !ctx->aligned_c2c)
+    MPAS_FAIL(MPAS_EINVAL, "the fused ocean run needs every level active and cellsOnCell[j] "
+                           "across edgesOnCell[j]");
+  return ocean_rk4_steps(ctx, dt, nsteps);
+}
+
 }  // extern "C"
 
 // ---- per-kernel profiling ----------------------------------------------------

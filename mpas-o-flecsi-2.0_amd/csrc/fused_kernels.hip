@@ -122,6 +122,10 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   }
   b.vertex = 44 * V + 2 * d * V + d * E + d * L * (E + C + 2 * V);
   b.edge_diag = 16 * E + d * L * (C + V + 2 * E);
+  if (S.ocean) {  // tracer-only run: no momentum, thickness or diagnostics
+    b.edges = b.vertex = b.edge_diag = 0;
+    b.cells -= d * E + d * L * E + d * L * C * (touches + 1 + (S.use_visc ? 1 : 0));
+  }
   return b;
 }
 
@@ -140,10 +144,12 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   do {                  \
     if (int rc = (expr)) return rc; \
   } while (0)
-  MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
-  hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S);
-  MPAS_STEP(after_launch("k_stage_edges"));
-  MPAS_STEP(prof_end(ctx));
+  if (!S.ocean) {
+    MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
+    hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S);
+    MPAS_STEP(after_launch("k_stage_edges"));
+    MPAS_STEP(prof_end(ctx));
+  }
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
@@ -167,6 +173,7 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
       MPAS_STEP(prof_end(ctx));
     }
   }
+  if (S.ocean) return 0;  // u and h do not change: no diagnostics
   MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
   hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, S.yh, S.yu,
                      vort_y, pv_vertex);
@@ -264,6 +271,35 @@ int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot) {
   SwState &S = *ctx->sw;
   return diagnostics_into(ctx, u, h, S.ke[slot], S.div[slot], S.vort[slot], S.pv_vertex,
                           S.h_edge[slot], S.pv_edge[slot]);
+}
+
+// nsteps of rk4_timestep (apps/ocean/run.cc:10-50): the FCT tendency of
+// the tracers under the fixed u (normalVelocity), h (layerThickness) and
+// h_edge (layerThicknessEdge, init_functions.cc:30-45, the h_edge the
+// state upload formed), ping-pong buffers for the stage tracers (prev).
+int ocean_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
+  SwState &W = *ctx->sw;
+  for (int step = 0; step < nsteps; step++) {
+    for (int s = 0; s < 4; s++) {
+      const int cur = s % 2, nxt = 1 - cur;
+      StageIO S = stage_params(ctx, s, dt);
+      S.ocean = 1;
+      S.xu = W.u;
+      S.xh = W.h;
+      S.xtr = s == 0 ? W.tr : W.ptr[cur];
+      S.h_edge = W.h_edge[W.cur];
+      S.h_s = W.h_s;
+      S.tr0 = W.tr;
+      S.ntr = W.ntr;
+      S.ytr = s < 3 ? W.ptr[nxt] : W.tr;
+      S.anti = W.anti;
+      S.rfac = W.rfac;
+      S.tlo = W.tlo;
+      if (int rc = launch_stage(ctx, S, nullptr, nullptr, nullptr, nullptr)) return rc;
+    }
+    W.elapsed += dt;
+  }
+  return 0;
 }
 
 // nsteps of apps/sw/run.cc:13-70.

@@ -8,6 +8,9 @@ namespace mpasx {
 
 struct StageIO {
   int stage, scheme, freeze_u, use_visc, use_hmix;
+  // 1: the tracer-only (ocean) run: u, h and h_edge fixed, the FCT phases
+  // only, ocean tracer bookkeeping
+  int ocean;
   // 1: the limiter factors in `rfac` are given (a frozen-limiter replay)
   // and k_stage_cells leaves them untouched
   int keep_rfac;
@@ -63,6 +66,10 @@ int diagnostics_dm(const DMesh &M, hipStream_t st, int use_visc, const double *u
 
 // diagnostics_into the context's ping-pong slot `slot`.
 int sw_diagnostics(mpas_ctx *ctx, const double *u, const double *h, int slot);
+
+// nsteps RK4 steps of the tracer-only (ocean) run on the resident state:
+// tracers advected by the fixed u with the fixed h and h_edge.
+int ocean_rk4_steps(mpas_ctx *ctx, double dt, int nsteps);
 
 // nsteps fused RK4 steps of the resident state.  frozen_rfac, when given,
 // holds per (step, stage) the limiter factors to use instead of computing

@@ -263,6 +263,24 @@ __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uin
   }
 }
 
+// step_tracers_forward and update_tracers of the tracer-only (ocean) run for
+// one tracer of one cell (ocean/solve.cc:368-414, run.cc:10-50): next =
+// curr + w_rk tend at stage 0 (copy_tracers, then +=), next += w_rk tend,
+// prev = curr + w_sub tend; the last stage writes curr = next + w_rk tend
+// (its prev, curr + 0 tend, is not read again).
+__device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, double te) {
+  if (S.stage == 0) {
+    const double t0 = ld(S.tr0, ot);
+    st(S.ntr, ot, t0 + S.w_rk * te);
+    st(S.ytr, ot, t0 + S.w_sub * te);
+  } else if (S.stage < 3) {
+    st(S.ntr, ot, ld(S.ntr, ot) + S.w_rk * te);
+    st(S.ytr, ot, ld(S.tr0, ot) + S.w_sub * te);
+  } else {
+    st(S.ytr, ot, ld(S.ntr, ot) + S.w_rk * te);
+  }
+}
+
 // ---- cells (x tracers) -------------------------------------------------------
 // Thickness tendency and update, ke and divergence of the new state (the
 // cell's first lane stores them), and per tracer either the centred
@@ -292,7 +310,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
-    if (lead) {
+    if (lead && !S.ocean) {
       // ke (solve.cc:63-77) and divergence of the new state
       double yu[W], dc[W];
 #pragma unroll
@@ -317,27 +335,30 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       xu[j] = ld(S.xu, e[j] * L + k);
       he[j] = ld(S.h_edge, e[j] * L + k);
     }
-    // compute_tend, thickness part: edges in ascending id (solve.cc:191-205)
-    double th;
-    {
-      double fl[W];
+    // compute_tend, thickness part: edges in ascending id (solve.cc:191-205);
+    // the tracer-only (ocean) run keeps the thickness fixed
+    double hy = 0.0;
+    if (!S.ocean) {
+      double th;
+      {
+        double fl[W];
 #pragma unroll
-      for (int j = 0; j < W; j++) fl[j] = xu[j] * dv[j] * he[j];
-      th = sum_ascending<W>(0.0, fl, rank, neg, n);
+        for (int j = 0; j < W; j++) fl[j] = xu[j] * dv[j] * he[j];
+        th = sum_ascending<W>(0.0, fl, rank, neg, n);
+      }
+      th = th / ar;
+      if (S.stage == 0) {
+        const double h0 = ld(S.h0, ol);
+        hy = h0 + S.w_sub * th;
+        if (lead) st(S.nh, ol, h0 + S.w_rk * th);
+      } else if (S.stage < 3) {
+        hy = ld(S.h0, ol) + S.w_sub * th;
+        if (lead) st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
+      } else {
+        hy = ld(S.nh, ol) + S.w_rk * th;
+      }
+      if (lead) st(S.yh, ol, hy);
     }
-    th = th / ar;
-    double hy;
-    if (S.stage == 0) {
-      const double h0 = ld(S.h0, ol);
-      hy = h0 + S.w_sub * th;
-      if (lead) st(S.nh, ol, h0 + S.w_rk * th);
-    } else if (S.stage < 3) {
-      hy = ld(S.h0, ol) + S.w_sub * th;
-      if (lead) st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
-    } else {
-      hy = ld(S.nh, ol) + S.w_rk * th;
-    }
-    if (lead) st(S.yh, ol, hy);
 
     if constexpr (NT > 0) {
       const double invArea = 1.0 / ar;
@@ -526,9 +547,14 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
         }
       }
     }
-    const double hy = ld(S.yh, ol);
+    if (S.ocean) {
 #pragma unroll
-    for (int g = 0; g < TPL; g++) tracer_update(S, ol, ot + g, te[g], hy);
+      for (int g = 0; g < TPL; g++) ocean_update(S, ot + g, te[g]);
+    } else {
+      const double hy = ld(S.yh, ol);
+#pragma unroll
+      for (int g = 0; g < TPL; g++) tracer_update(S, ol, ot + g, te[g], hy);
+    }
   }
 }
 

@@ -346,6 +346,11 @@ class Context:
     def rk4_step(self, dt, nsteps=1):
         L.check(L.hip().mpas_rk4_step(self._h, float(dt), int(nsteps)), "mpas_rk4_step")
 
+    def ocean_rk4_step(self, dt, nsteps=1):
+        """The tracer-only ocean driver (u, h fixed) on the uploaded state."""
+        L.check(L.hip().mpas_ocean_rk4_step(self._h, float(dt), int(nsteps)),
+                "mpas_ocean_rk4_step")
+
     # ---- tangent-linear model and adjoint (device arrays in, device arrays out) --
     def state_arrays(self):
         """Three new device arrays shaped like the state (u, h, tracers)."""
