@@ -315,7 +315,8 @@ def main():
     ap.add_argument("--dt", type=float, default=60.0)
     ap.add_argument("--prof-steps", type=int, default=50,
                     help="steps of the per-kernel event-timed pass")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="oracle steps of the CPU baseline (about 1.3 s each on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (level 6, 200 steps, 1 FCT tracer)")
