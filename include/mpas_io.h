@@ -59,6 +59,57 @@ typedef struct mpas_io_dataset {
 int mpas_mesh_file_write(const char *path, double sphere_radius, int n,
                          const mpas_io_dataset *datasets);
 
+/* ---- output time series ----------------------------------------------------
+   The run's output file, as apps/sw/run.cc:73-113 writes it.  The caller
+   copies the fields to the host only at output steps. */
+typedef struct mpas_output mpas_output;
+
+/* What mesh_output_init (mpasoflecsi/tasks/io.cc:160-244) writes once. */
+typedef struct mpas_output_mesh {
+  int64_t nCells, nEdges, nVertices;
+  int64_t vertexDegree, nVertLevels, nTracers;
+  const double *areaCell, *xCell, *yCell, *zCell, *latCell, *lonCell; /* [nCells]    */
+  const double *latVertex, *lonVertex;                                /* [nVertices] */
+  const double *meshDensity;   /* [nCells], written as "mDensity"; NULL = 1        */
+  const double *dvEdge, *dcEdge;                                      /* [nEdges]    */
+  const int32_t *cellsOnVertex; /* [nVertices * vertexDegree], 1-based             */
+} mpas_output_mesh;
+
+/* mesh_output_init: creates the file with the attribute on_a_sphere = "YES",
+   the dimension scales nCells, nEdges, nVertices, vertexDegree, nVertLevels,
+   Time (num_output_times long) and nTracers, and the mesh fields above. */
+int mpas_output_create(const char *path, int num_output_times,
+                       const mpas_output_mesh *mesh, mpas_output **out);
+
+enum { MPAS_OUT_CELLS = 0, MPAS_OUT_EDGES = 1, MPAS_OUT_VERTICES = 2 };
+/* One field of one output time (field_output_wrapper::output,
+   mpasoflecsi/tasks/io.hh:13-47): data[entity][level][tracer] with
+   levels = 0 for a field without levels (h_s) and tracers = 0 for one
+   without tracers. */
+typedef struct mpas_output_field {
+  const char *name; /* h, h_s, vorticity, potential_vorticity, tracers */
+  const double *data;
+  int32_t space;    /* MPAS_OUT_CELLS / _EDGES / _VERTICES */
+  int32_t levels, tracers;
+} mpas_output_field;
+/* Writes time row t of the field's dataset [Time][entity](level)(tracer)
+   (io/helpers.hh:127-258); the dataset is created, with its dimension
+   scales attached, at the field's first write.  t must lie in
+   [0, num_output_times). */
+int mpas_output_write(mpas_output *o, int t, const mpas_output_field *field);
+/* mesh_output_finalize (tasks/io.cc:247-251). */
+int mpas_output_close(mpas_output *o);
+
+/* Reading an output file back (post-processing, tests): the rank and
+   dimensions of a dataset, and, when out is not NULL, all its n entries
+   converted to double. */
+int mpas_output_read(const char *path, const char *name, int *rank, int64_t dims[4],
+                     double *out, int64_t n);
+/* Name of the dimension scale attached to dimension dim of a dataset
+   (empty when none is). */
+int mpas_output_scale_name(const char *path, const char *name, int dim, char *buf,
+                           int64_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,7 +1,9 @@
 // MPAS mesh files through the HDF5 C API: the datasets the reference reads
 // by name (mpasoflecsi/tasks/io.cc:14-158), in the file's own conventions.
 #include <hdf5.h>
+#include <hdf5_hl.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -213,34 +215,42 @@ int mpas_mesh_file_read(const mpas_mesh_file *f, const char *name, void *out, in
   return 0;
 }
 
+namespace {
+
+int write_dataset(hid_t file, const mpas_io_dataset &x) {
+  if (!x.name || !x.data || x.rows < 0 || x.cols < 1)
+    return fail(MPAS_EINVAL, "bad dataset");
+  const int rank = x.cols == 1 ? 1 : 2;
+  const hsize_t dims[2] = {hsize_t(x.rows), hsize_t(x.cols)};
+  Handle sp{H5Screate_simple(rank, dims, nullptr), H5Sclose};
+  const hid_t ty = x.is_int ? H5T_STD_I32LE : H5T_IEEE_F64LE;
+  Handle ds{H5Dcreate2(file, x.name, ty, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+            H5Dclose};
+  if (ds.id < 0) return fail(MPAS_EINVAL, std::string("cannot create dataset ") + x.name);
+  if (H5Dwrite(ds.id, x.is_int ? H5T_NATIVE_INT32 : H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL,
+               H5P_DEFAULT, x.data) < 0)
+    return fail(MPAS_EINVAL, std::string("cannot write dataset ") + x.name);
+  return 0;
+}
+
+void write_radius(hid_t file, double sphere_radius) {
+  Handle sp{H5Screate(H5S_SCALAR), H5Sclose};
+  Handle at{H5Acreate2(file, "sphere_radius", H5T_IEEE_F64LE, sp.id, H5P_DEFAULT, H5P_DEFAULT),
+            H5Aclose};
+  H5Awrite(at.id, H5T_NATIVE_DOUBLE, &sphere_radius);
+}
+
+}  // namespace
+
 int mpas_mesh_file_write(const char *path, double sphere_radius, int n,
                          const mpas_io_dataset *datasets) {
   if (!path || (n && !datasets)) return fail(MPAS_EINVAL, "mpas_mesh_file_write: NULL argument");
   quiet();
   Handle file{H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose};
   if (file.id < 0) return fail(MPAS_EINVAL, std::string("cannot create ") + path);
-  for (int i = 0; i < n; i++) {
-    const mpas_io_dataset &x = datasets[i];
-    if (!x.name || !x.data || x.rows < 0 || x.cols < 1)
-      return fail(MPAS_EINVAL, "mpas_mesh_file_write: bad dataset " + std::to_string(i));
-    const int rank = x.cols == 1 ? 1 : 2;
-    const hsize_t dims[2] = {hsize_t(x.rows), hsize_t(x.cols)};
-    Handle sp{H5Screate_simple(rank, dims, nullptr), H5Sclose};
-    const hid_t ty = x.is_int ? H5T_STD_I32LE : H5T_IEEE_F64LE;
-    Handle ds{H5Dcreate2(file.id, x.name, ty, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
-              H5Dclose};
-    if (ds.id < 0) return fail(MPAS_EINVAL, std::string("cannot create dataset ") + x.name);
-    if (H5Dwrite(ds.id, x.is_int ? H5T_NATIVE_INT32 : H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL,
-                 H5P_DEFAULT, x.data) < 0)
-      return fail(MPAS_EINVAL, std::string("cannot write dataset ") + x.name);
-  }
-  {
-    Handle sp{H5Screate(H5S_SCALAR), H5Sclose};
-    Handle at{H5Acreate2(file.id, "sphere_radius", H5T_IEEE_F64LE, sp.id, H5P_DEFAULT,
-                         H5P_DEFAULT),
-              H5Aclose};
-    H5Awrite(at.id, H5T_NATIVE_DOUBLE, &sphere_radius);
-  }
+  for (int i = 0; i < n; i++)
+    if (int rc = write_dataset(file.id, datasets[i])) return rc;
+  write_radius(file.id, sphere_radius);
   {
     Handle sp{H5Screate(H5S_SCALAR), H5Sclose};
     Handle ty{H5Tcopy(H5T_C_S1), H5Tclose};
@@ -249,6 +259,224 @@ int mpas_mesh_file_write(const char *path, double sphere_radius, int n,
               H5Aclose};
     H5Awrite(at.id, ty.id, "YES");
   }
+  return 0;
+}
+
+}  // extern "C"
+
+struct mpas_output {
+  hid_t file = -1;
+  int ntimes = 0;
+  int64_t n[3] = {0, 0, 0};  // cells, edges, vertices
+  int64_t levels = 0, tracers = 0;
+  ~mpas_output() {
+    if (file >= 0) H5Fclose(file);
+  }
+};
+
+namespace {
+
+const char *const kSpace[3] = {"nCells", "nEdges", "nVertices"};
+
+// A dimension scale: a zero-filled int dataset of the dimension's length
+// marked as a scale (helper1d::write_dimscale, io/helpers.hh:82-118).
+int dim_scale(hid_t file, const char *name, int64_t len) {
+  std::vector<int32_t> zero(size_t(len > 0 ? len : 1), 0);  // a valid pointer when len = 0
+  mpas_io_dataset d{name, zero.data(), len, 1, 1};
+  if (int rc = write_dataset(file, d)) return rc;
+  Handle ds{H5Dopen2(file, name, H5P_DEFAULT), H5Dclose};
+  if (ds.id < 0 || H5DSset_scale(ds.id, name) < 0)
+    return fail(MPAS_EINVAL, std::string("cannot make scale ") + name);
+  return 0;
+}
+
+int attach(hid_t file, const char *dataset, const char *scale, unsigned idx) {
+  Handle ds{H5Dopen2(file, dataset, H5P_DEFAULT), H5Dclose};
+  Handle sc{H5Dopen2(file, scale, H5P_DEFAULT), H5Dclose};
+  if (ds.id < 0 || sc.id < 0 || H5DSattach_scale(ds.id, sc.id, idx) < 0)
+    return fail(MPAS_EINVAL, std::string("cannot attach ") + scale + " to " + dataset);
+  return 0;
+}
+
+int mesh_field(hid_t file, const char *name, const void *data, int64_t rows, int64_t cols,
+               int is_int, int space) {
+  if (!data) return fail(MPAS_EINVAL, std::string("mpas_output_create: ") + name + " is NULL");
+  if (int rc = write_dataset(file, mpas_io_dataset{name, data, rows, cols, is_int})) return rc;
+  return attach(file, name, kSpace[space], 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpas_output_create(const char *path, int ntimes, const mpas_output_mesh *m,
+                       mpas_output **out) {
+  if (!path || !out || !m || ntimes < 1 || m->nCells < 1 || m->nEdges < 1 || m->nVertices < 1 ||
+      m->vertexDegree < 1 || m->nVertLevels < 1 || m->nTracers < 0)
+    return fail(MPAS_EINVAL, "mpas_output_create: bad argument");
+  *out = nullptr;
+  quiet();
+  auto o = std::make_unique<mpas_output>();
+  o->file = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (o->file < 0) return fail(MPAS_EINVAL, std::string("cannot create ") + path);
+  o->ntimes = ntimes;
+  o->n[0] = m->nCells;
+  o->n[1] = m->nEdges;
+  o->n[2] = m->nVertices;
+  o->levels = m->nVertLevels;
+  o->tracers = m->nTracers;
+  {
+    Handle sp{H5Screate(H5S_SCALAR), H5Sclose};
+    Handle ty{H5Tcopy(H5T_C_S1), H5Tclose};
+    H5Tset_size(ty.id, 3);
+    H5Tset_strpad(ty.id, H5T_STR_NULLTERM);
+    Handle at{H5Acreate2(o->file, "on_a_sphere", ty.id, sp.id, H5P_DEFAULT, H5P_DEFAULT),
+              H5Aclose};
+    H5Awrite(at.id, ty.id, "YES");
+  }
+  const hid_t f = o->file;
+  const struct { const char *name; int64_t len; } scales[] = {
+      {"nCells", m->nCells},           {"nEdges", m->nEdges},
+      {"nVertices", m->nVertices},     {"vertexDegree", m->vertexDegree},
+      {"nVertLevels", m->nVertLevels}, {"Time", ntimes},
+      {"nTracers", m->nTracers}};
+  for (const auto &s : scales)
+    if (int rc = dim_scale(f, s.name, s.len)) return rc;
+  std::vector<double> ones;
+  const double *density = m->meshDensity;
+  if (!density) {
+    ones.assign(size_t(m->nCells), 1.0);
+    density = ones.data();
+  }
+  const struct { const char *name; const double *data; int space; } fields[] = {
+      {"areaCell", m->areaCell, 0},   {"xCell", m->xCell, 0},
+      {"yCell", m->yCell, 0},         {"zCell", m->zCell, 0},
+      {"latCell", m->latCell, 0},     {"lonCell", m->lonCell, 0},
+      {"latVertex", m->latVertex, 2}, {"lonVertex", m->lonVertex, 2},
+      {"mDensity", density, 0},       {"dvEdge", m->dvEdge, 1},
+      {"dcEdge", m->dcEdge, 1}};
+  for (const auto &x : fields)
+    if (int rc = mesh_field(f, x.name, x.data, o->n[x.space], 1, 0, x.space)) return rc;
+  if (int rc = mesh_field(f, "cellsOnVertex", m->cellsOnVertex, m->nVertices, m->vertexDegree,
+                          1, 2))
+    return rc;
+  if (int rc = attach(f, "cellsOnVertex", "vertexDegree", 1)) return rc;
+  *out = o.release();
+  return 0;
+}
+
+int mpas_output_write(mpas_output *o, int t, const mpas_output_field *x) {
+  if (!o || !x || !x->name || !x->data || x->space < 0 || x->space > 2 || x->levels < 0 ||
+      x->tracers < 0 || (x->tracers && !x->levels))
+    return fail(MPAS_EINVAL, "mpas_output_write: bad argument");
+  if (t < 0 || t >= o->ntimes)
+    return fail(MPAS_EINVAL, "mpas_output_write: time index " + std::to_string(t) +
+                                 " outside [0, " + std::to_string(o->ntimes) + ")");
+  if ((x->levels && x->levels != o->levels) || (x->tracers && x->tracers != o->tracers))
+    return fail(MPAS_ESHAPE, std::string(x->name) + ": levels or tracers differ from the file's");
+  int rank = 2;
+  hsize_t dims[4] = {hsize_t(o->ntimes), hsize_t(o->n[x->space]), 1, 1};
+  const char *scale[4] = {"Time", kSpace[x->space], "nVertLevels", "nTracers"};
+  if (x->levels) dims[rank++] = hsize_t(x->levels);
+  if (x->tracers) dims[rank++] = hsize_t(x->tracers);
+  const bool fresh = !has_dataset(o->file, x->name);
+  hid_t ds;
+  if (fresh) {
+    Handle sp{H5Screate_simple(rank, dims, nullptr), H5Sclose};
+    ds = H5Dcreate2(o->file, x->name, H5T_IEEE_F64LE, sp.id, H5P_DEFAULT, H5P_DEFAULT,
+                    H5P_DEFAULT);
+  } else {
+    ds = H5Dopen2(o->file, x->name, H5P_DEFAULT);
+  }
+  {
+    Handle d{ds, H5Dclose};
+    if (d.id < 0) return fail(MPAS_EINVAL, std::string("cannot open dataset ") + x->name);
+    Handle fs{H5Dget_space(d.id), H5Sclose};
+    hsize_t have[4] = {0, 0, 0, 0};
+    if (H5Sget_simple_extent_ndims(fs.id) != rank ||
+        H5Sget_simple_extent_dims(fs.id, have, nullptr) != rank ||
+        !std::equal(have + 1, have + rank, dims + 1))
+      return fail(MPAS_ESHAPE, std::string(x->name) + ": shape differs from its first write");
+    hsize_t start[4] = {hsize_t(t), 0, 0, 0}, count[4];
+    std::copy(dims, dims + 4, count);
+    count[0] = 1;
+    H5Sselect_hyperslab(fs.id, H5S_SELECT_SET, start, nullptr, count, nullptr);
+    Handle ms{H5Screate_simple(rank, count, nullptr), H5Sclose};
+    if (H5Dwrite(d.id, H5T_NATIVE_DOUBLE, ms.id, fs.id, H5P_DEFAULT, x->data) < 0)
+      return fail(MPAS_EINVAL, std::string("cannot write ") + x->name);
+  }
+  if (fresh) {  // scales attached at the first write (tasks/io.hh:24-29)
+    if (int rc = attach(o->file, x->name, scale[0], 0)) return rc;
+    if (int rc = attach(o->file, x->name, scale[1], 1)) return rc;
+    for (int d = 2; d < rank; d++)
+      if (int rc = attach(o->file, x->name, scale[d], unsigned(d))) return rc;
+  }
+  return 0;
+}
+
+int mpas_output_close(mpas_output *o) {
+  if (!o) return fail(MPAS_EINVAL, "mpas_output_close: NULL");
+  const hid_t f = o->file;
+  o->file = -1;
+  delete o;
+  if (f >= 0 && H5Fclose(f) < 0) return fail(MPAS_EINVAL, "mpas_output_close: H5Fclose failed");
+  return 0;
+}
+
+int mpas_output_read(const char *path, const char *name, int *rank, int64_t dims[4],
+                     double *out, int64_t n) {
+  if (!path || !name || !rank || !dims) return fail(MPAS_EINVAL, "mpas_output_read: NULL");
+  quiet();
+  Handle f{H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose};
+  if (f.id < 0) return fail(MPAS_EINVAL, std::string("cannot open ") + path);
+  if (!has_dataset(f.id, name)) return fail(MPAS_EINVAL, std::string("no dataset ") + name);
+  Handle ds{H5Dopen2(f.id, name, H5P_DEFAULT), H5Dclose};
+  Handle sp{H5Dget_space(ds.id), H5Sclose};
+  const int r = H5Sget_simple_extent_ndims(sp.id);
+  if (r < 0 || r > 4) return fail(MPAS_ESHAPE, std::string(name) + ": rank above 4");
+  hsize_t d[4] = {1, 1, 1, 1};
+  H5Sget_simple_extent_dims(sp.id, d, nullptr);
+  *rank = r;
+  int64_t total = 1;
+  for (int i = 0; i < 4; i++) {
+    dims[i] = i < r ? int64_t(d[i]) : 1;
+    total *= dims[i];
+  }
+  if (!out) return 0;
+  if (n != total)
+    return fail(MPAS_ESHAPE, std::string(name) + " has " + std::to_string(total) +
+                                 " entries, not " + std::to_string(n));
+  if (H5Dread(ds.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0)
+    return fail(MPAS_EINVAL, std::string("cannot read ") + name);
+  return 0;
+}
+
+namespace {
+herr_t scale_name_of(hid_t, unsigned, hid_t scale, void *data) {
+  auto *s = static_cast<std::string *>(data);
+  const ssize_t len = H5DSget_scale_name(scale, nullptr, 0);
+  if (len > 0) {
+    std::vector<char> b(size_t(len) + 1);
+    H5DSget_scale_name(scale, b.data(), b.size());
+    *s = b.data();
+  }
+  return 1;  // the first scale attached
+}
+}  // namespace
+
+int mpas_output_scale_name(const char *path, const char *name, int dim, char *buf,
+                           int64_t len) {
+  if (!path || !name || !buf || len < 1) return fail(MPAS_EINVAL, "mpas_output_scale_name: NULL");
+  quiet();
+  Handle f{H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose};
+  if (f.id < 0) return fail(MPAS_EINVAL, std::string("cannot open ") + path);
+  if (!has_dataset(f.id, name)) return fail(MPAS_EINVAL, std::string("no dataset ") + name);
+  Handle ds{H5Dopen2(f.id, name, H5P_DEFAULT), H5Dclose};
+  std::string s;
+  if (H5DSget_num_scales(ds.id, unsigned(dim)) > 0)
+    H5DSiterate_scales(ds.id, unsigned(dim), nullptr, scale_name_of, &s);
+  if (int64_t(s.size()) >= len) return fail(MPAS_EINVAL, "mpas_output_scale_name: buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
   return 0;
 }
 

@@ -3,21 +3,27 @@
 // task layer and the MI355X library.
 //
 //   sw [--level L] [--unrotated] [-t 1|5|balanced] [-s nsteps] [-d dt]
-//      [--tracers N] [--fused]
+//      [--tracers N] [--fused] [--num_output N] [--output PATH]
 //
 // Defaults follow apps/sw/state.hh:51-64 (test case 1, 10 steps, dt 40 s).
 // Case 1 ends with the reference's L2 gate (apps/sw/finalize.cc:15-29):
 // exit status 1 when the error exceeds 0.0015.  One JSON line reports the
-// error and an FNV-1a hash of the final h bytes.
+// error and an FNV-1a hash of the final h bytes.  With --num_output N (the
+// reference's option, mpasoflecsi/io/state.hh:13-17) the run writes its
+// output file (default output.ncdf) as apps/sw/run.cc:73-113 does.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../host/initial.h"
 #include "sw_run.hh"
+#ifdef MPAS_WITH_IO
+#include "mpas_io.h"
+#endif
 
 using namespace mpas::hip;
 
@@ -43,6 +49,64 @@ uint64_t fnv1a(const std::vector<double> &v) {
   return h;
 }
 
+#ifdef MPAS_WITH_IO
+// The output file of apps/sw/run.cc:73-113: mesh_output_init, then the
+// fields of apps/sw/state.hh:66-70 per output time.
+class output_file {
+ public:
+  output_file(const std::string &path, int ntimes, const mg_mesh *g, int64_t nTracers) {
+    auto D = [&](const char *n) { return mg_dbl_field(g, n, nullptr); };
+    mpas_output_mesh om{};
+    om.nCells = mg_dim(g, "nCells");
+    om.nEdges = mg_dim(g, "nEdges");
+    om.nVertices = nV_ = mg_dim(g, "nVertices");
+    om.vertexDegree = 3;
+    om.nVertLevels = 1;
+    om.nTracers = nTracers;
+    om.areaCell = D("areaCell");
+    om.xCell = D("xCell");
+    om.yCell = D("yCell");
+    om.zCell = D("zCell");
+    om.latCell = D("latCell");
+    om.lonCell = D("lonCell");
+    om.latVertex = D("latVertex");
+    om.lonVertex = D("lonVertex");
+    om.meshDensity = D("meshDensity");
+    om.dvEdge = D("dvEdge");
+    om.dcEdge = D("dcEdge");
+    om.cellsOnVertex = mg_int_field(g, "cellsOnVertex", nullptr);
+    if (int rc = mpas_output_create(path.c_str(), ntimes, &om, &o_))
+      throw error(rc, std::string("mpas_output_create: ") + mpas_io_last_error());
+    nT_ = nTracers;
+  }
+  ~output_file() {
+    if (o_) mpas_output_close(o_);
+  }
+  // The five fields of one output time; vorticity and pv_vertex are zero at
+  // time 0 (init_extra_fields, apps/sw/tasks/testcase.cc:16-35).
+  void write(int t, const std::vector<double> &h, const std::vector<double> &h_s,
+             const std::vector<double> &vort, const std::vector<double> &pv,
+             const std::vector<double> &tr) {
+    std::vector<double> zero(size_t(nV_), 0.0);
+    const double *v = t == 0 ? zero.data() : vort.data(), *p = t == 0 ? zero.data() : pv.data();
+    const mpas_output_field fs[] = {{"h", h.data(), MPAS_OUT_CELLS, 1, 0},
+                                    {"h_s", h_s.data(), MPAS_OUT_CELLS, 0, 0},
+                                    {"vorticity", v, MPAS_OUT_VERTICES, 1, 0},
+                                    {"potential_vorticity", p, MPAS_OUT_VERTICES, 1, 0},
+                                    {"tracers", tr.data(), MPAS_OUT_CELLS, 1, int32_t(nT_)}};
+    for (const auto &x : fs) {
+      if (std::string(x.name) == "tracers" && !nT_) continue;
+      if (int rc = mpas_output_write(o_, t, &x))
+        throw error(rc, std::string("mpas_output_write: ") + mpas_io_last_error());
+    }
+  }
+
+ private:
+  mpas_output *o_ = nullptr;
+  int64_t nV_ = 0, nT_ = 0;
+};
+#endif
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -50,6 +114,8 @@ int main(int argc, char **argv) {
   double dt = 40;
   std::string tc = "1";
   bool fused = false;
+  int num_output = 0;
+  std::string out_path = "output.ncdf";
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     auto next = [&]() -> const char * {
@@ -66,9 +132,11 @@ int main(int argc, char **argv) {
     else if (a == "-d" || a == "--dt") dt = std::atof(next());
     else if (a == "--tracers") ntr = std::atoi(next());
     else if (a == "--fused") fused = true;
+    else if (a == "--num_output") num_output = std::atoi(next());
+    else if (a == "--output") out_path = next();
     else {
       std::fprintf(stderr, "usage: sw [--level L] [--unrotated] [-t 1|5|balanced] [-s n] "
-                           "[-d dt] [--tracers N] [--fused]\n");
+                           "[-d dt] [--tracers N] [--fused] [--num_output N] [--output PATH]\n");
       return 2;
     }
   }
@@ -139,14 +207,48 @@ int main(int argc, char **argv) {
   p.gravity = kG;
   p.sphere_radius = radius;
   p.coef3rd = 0.25;
+#ifndef MPAS_WITH_IO
+  if (num_output > 0) {
+    std::fprintf(stderr, "--num_output needs the HDF5 output library (lib/libmpas_io.so)\n");
+    mg_free(g);
+    return 2;
+  }
+#endif
   try {
     mesh m(d, p);
+#ifdef MPAS_WITH_IO
+    std::unique_ptr<output_file> out;
+    if (num_output > 0) out = std::make_unique<output_file>(out_path, num_output, g, ntr);
+#endif
     std::vector<double> h_final;
     if (fused) {
       check(mpas_sw_state_upload(m.ctx(), h.data(), u.data(), ntr ? tr.data() : nullptr,
                                  h_s.data()),
             "mpas_sw_state_upload");
+#ifdef MPAS_WITH_IO
+      // apps/sw/run.cc:77-106 with the steps between output times fused.
+      auto field_of = [&](const char *name, int64_t n) {
+        std::vector<double> v(static_cast<size_t>(n));
+        if (n) check(mpas_sw_field_download(m.ctx(), name, v.data(), n), "mpas_sw_field_download");
+        return v;
+      };
+      auto write = [&](int t) {
+        out->write(t, field_of("h", nC), field_of("h_s", nC), field_of("vorticity", nV),
+                   field_of("pv_vertex", nV), field_of("tracers", nC * ntr));
+      };
+      int done = 0, t = 0;
+      if (out) {
+        write(t++);
+        for (int i : sw::output_steps(nsteps, num_output)) {
+          check(mpas_rk4_step(m.ctx(), dt, i - done), "mpas_rk4_step");
+          done = i;
+          write(t++);
+        }
+      }
+      if (nsteps > done) check(mpas_rk4_step(m.ctx(), dt, nsteps - done), "mpas_rk4_step");
+#else
       check(mpas_rk4_step(m.ctx(), dt, nsteps), "mpas_rk4_step");
+#endif
       h_final.resize(nC);
       check(mpas_sw_state_download(m.ctx(), h_final.data(), nullptr, nullptr),
             "mpas_sw_state_download");
@@ -156,7 +258,19 @@ int main(int argc, char **argv) {
       f.u[sw::curr].upload(u);
       if (ntr) f.tracers[sw::curr].upload(tr);
       f.bottomDepth.upload(h_s);
+#ifdef MPAS_WITH_IO
+      if (out) {
+        sw::run(m, f, dt, nsteps, num_output, [&](int t) {
+          out->write(t, f.h[sw::curr].download(), f.bottomDepth.download(),
+                     f.vorticity.download(), f.pv_vertex.download(),
+                     ntr ? f.tracers[sw::curr].download() : std::vector<double>());
+        });
+      } else {
+        sw::run(m, f, dt, nsteps);
+      }
+#else
       sw::run(m, f, dt, nsteps);
+#endif
       m.sync();
       h_final = f.h[sw::curr].download();
     }

@@ -3,6 +3,7 @@
 #include "sw_run.hh"
 
 #include <array>
+#include <cstdint>
 
 namespace mpas { namespace hip { namespace sw {
 
@@ -58,11 +59,31 @@ void advance_timestep(mesh &m, fields &f, double dt) {
                f.ke[next], f.pv_edge[next]);
 }
 
-void run(mesh &m, fields &f, double dt, int nsteps) {
+std::vector<int> output_steps(int nsteps, int num_output_times) {
+  std::vector<int> out;
+  if (num_output_times < 2 || nsteps < 1) return out;
+  const uint64_t n = uint64_t(nsteps), P = uint64_t(num_output_times - 1);
+  for (uint64_t i = 1; i <= n; i++) {
+    const uint64_t owner = (P * (i + 1) - 1) / n;  // block_partition::owner
+    const uint64_t high = (owner + 1) * n / P - 1;  // block_partition::high
+    if (high == i) out.push_back(int(i));
+  }
+  return out;
+}
+
+void run(mesh &m, fields &f, double dt, int nsteps, int num_output_times,
+         const std::function<void(int)> &output) {
+  const bool out = num_output_times > 0 && output;
+  if (out) output(0);
   task::compute_solve_diagnostics(m, f.h[curr], f.u[curr], f.h_edge[curr], f.h_vertex[curr],
                                   f.circulation, f.vorticity, f.ke[curr], f.pv_edge[curr],
                                   f.pv_vertex);
-  for (int i = 1; i <= nsteps; i++) advance_timestep(m, f, dt);
+  const std::vector<int> at = out ? output_steps(nsteps, num_output_times) : std::vector<int>();
+  size_t next_out = 0;
+  for (int i = 1; i <= nsteps; i++) {
+    advance_timestep(m, f, dt);
+    if (next_out < at.size() && at[next_out] == i) output(int(++next_out));
+  }
 }
 
 }}}  // namespace mpas::hip::sw
