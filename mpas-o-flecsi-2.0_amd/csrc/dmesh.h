@@ -103,10 +103,55 @@ __device__ __forceinline__ T ldc(const T *__restrict__ p, uint32_t i) {
   const MPAS_CONST char *b = (const MPAS_CONST char *)(p);
   return *(const MPAS_CONST T *)(b + i * uint32_t(sizeof(T)));
 }
+// Streams read or written once per launch (the slot-major stencils, a
+// lane's own state entries): with NT true, non-temporal accesses, which
+// spare the caches for the neighbour gathers.  The switches select which
+// streams of the one-entity-per-lane launches take them.
+#ifndef MPAS_NT_STENCIL
+#define MPAS_NT_STENCIL 1
+#endif
+#ifndef MPAS_NT_CELLS
+#define MPAS_NT_CELLS 1
+#endif
+#ifndef MPAS_NT_NEXT
+#define MPAS_NT_NEXT 1
+#endif
+constexpr bool kNtStencil = MPAS_NT_STENCIL, kNtCells = MPAS_NT_CELLS, kNtNext = MPAS_NT_NEXT;
+template <bool NT, class T>
+__device__ __forceinline__ T ldn(const T *__restrict__ p, uint32_t i) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (NT) {
+    const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);
+    return __builtin_nontemporal_load((const MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T))));
+  }
+#endif
+  return ldc(p, i);
+}
+template <bool NT, class T>
+__device__ __forceinline__ T ldo(const T *__restrict__ p, uint32_t i) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (NT) {
+    const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);
+    return __builtin_nontemporal_load((const MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T))));
+  }
+#endif
+  return ld(p, i);
+}
 template <class T>
 __device__ __forceinline__ void st(T *p, uint32_t i, T v) {
   MPAS_GLOBAL char *b = (MPAS_GLOBAL char *)(p);
   *(MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T))) = v;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void stn(T *p, uint32_t i, T v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (NT) {
+    MPAS_GLOBAL char *b = (MPAS_GLOBAL char *)(p);
+    __builtin_nontemporal_store(v, (MPAS_GLOBAL T *)(b + i * uint32_t(sizeof(T))));
+    return;
+  }
+#endif
+  st(p, i, v);
 }
 
 __device__ __forceinline__ int lvl_cell(const DMesh &M, int c) {

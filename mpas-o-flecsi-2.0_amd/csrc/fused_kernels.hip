@@ -25,32 +25,84 @@ namespace mpasx {
 namespace {
 
 constexpr int BS = 256;
+// Workgroup size of each stage kernel.
+#ifndef MPAS_BS_EDGES
+#define MPAS_BS_EDGES 256
+#endif
+#ifndef MPAS_BS_HO
+#define MPAS_BS_HO 256
+#endif
+#ifndef MPAS_BS_CELLS
+#define MPAS_BS_CELLS 256
+#endif
+#ifndef MPAS_BS_TRACER
+#define MPAS_BS_TRACER 256
+#endif
+#ifndef MPAS_BS_DIAG
+#define MPAS_BS_DIAG 256
+#endif
+constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_CELLS,
+              BS_TRACER = MPAS_BS_TRACER, BS_DIAG = MPAS_BS_DIAG;
+
+// Waves per SIMD each stage kernel is compiled for (0: the compiler's
+// choice).  A target caps the registers per lane at 512 / waves.
+#ifndef MPAS_WPE_EDGES
+#define MPAS_WPE_EDGES 0
+#endif
+#ifndef MPAS_WPE_HO
+#define MPAS_WPE_HO 0
+#endif
+#ifndef MPAS_WPE_CELLS
+#define MPAS_WPE_CELLS 0
+#endif
+#ifndef MPAS_WPE_TRACER
+#define MPAS_WPE_TRACER 0
+#endif
+#define MPAS_WAVES(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, (n) > 0 ? (n) : 8)))
 
 template <int WE>
-__global__ __launch_bounds__(BS) void k_stage_edges(DMesh M, StageIO S) {
+__global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
+                                                                               StageIO S) {
   stage_edges<WE>(M, S, xcd_item());
 }
 template <int NT, int WA>
-__global__ __launch_bounds__(BS) void k_stage_ho(DMesh M, StageIO S) {
+__global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
   stage_ho<NT, WA>(M, S, xcd_item());
 }
-template <int NT, int W>
-__global__ __launch_bounds__(BS) void k_stage_cells(DMesh M, StageIO S) {
-  stage_cells<NT, W>(M, S, xcd_item());
+// Tracers per lane of the cell kernels (0: pairs when the count is even).
+#ifndef MPAS_TPL_CELLS
+#define MPAS_TPL_CELLS 0
+#endif
+#ifndef MPAS_TPL_TRACER
+#define MPAS_TPL_TRACER 1
+#endif
+template <int NT, int X>
+constexpr int lanes_per_cell() {
+  return NT > 0 ? NT / (X && NT % X == 0 ? X : tracers_per_lane<NT>()) : 1;
+}
+template <int NT>
+constexpr int tpl_of(int x) {
+  return x && NT % x == 0 ? x : 0;
 }
 template <int NT, int W>
-__global__ __launch_bounds__(BS) void k_stage_tracer(DMesh M, StageIO S) {
-  stage_tracer<NT, W>(M, S, xcd_item());
+__global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
+                                                                               StageIO S) {
+  stage_cells<NT, W, 0, tpl_of<NT>(MPAS_TPL_CELLS)>(M, S, xcd_item());
+}
+template <int NT, int W>
+__global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
+                                                                                 StageIO S) {
+  stage_tracer<NT, W, 0, tpl_of<NT>(MPAS_TPL_TRACER)>(M, S, xcd_item());
 }
 
-__global__ __launch_bounds__(BS) void k_diag_vertex(DMesh M, const double *__restrict__ h,
+__global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, const double *__restrict__ h,
                                                     const double *__restrict__ u,
                                                     double *vorticity, double *pv_vertex) {
   const int v = xcd_item();
   if (v < M.nV) diag_vertex(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex);
 }
 
-__global__ __launch_bounds__(BS) void k_diag_edge(DMesh M, const double *__restrict__ h,
+__global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, const double *__restrict__ h,
                                                   const double *__restrict__ pv_vertex,
                                                   double *h_edge, double *pv_edge) {
   const int e = xcd_item();
@@ -146,41 +198,47 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   } while (0)
   if (!S.ocean) {
     MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
-    hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S);
+    hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS_EDGES)), dim3(BS_EDGES), 0, st,
+                       M, S);
     MPAS_STEP(after_launch("k_stage_edges"));
     MPAS_STEP(prof_end(ctx));
   }
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
-      hipLaunchKernelGGL((k_stage_ho<NT, WE>), dim3(xcd_grid(int64_t(M.nE) * NPL, BS)), dim3(BS),
+      hipLaunchKernelGGL((k_stage_ho<NT, WE>), dim3(xcd_grid(int64_t(M.nE) * NPL, BS_HO)),
+                         dim3(BS_HO),
                          0, st, M, S);
       MPAS_STEP(after_launch("k_stage_ho"));
       MPAS_STEP(prof_end(ctx));
     }
   }
   MPAS_STEP(prof_begin(ctx, "k_stage_cells", b.cells));
-  hipLaunchKernelGGL((k_stage_cells<NT, W>), dim3(xcd_grid(int64_t(M.nC) * NPL, BS)), dim3(BS),
+  hipLaunchKernelGGL((k_stage_cells<NT, W>),
+                     dim3(xcd_grid(int64_t(M.nC) * lanes_per_cell<NT, MPAS_TPL_CELLS>(), BS_CELLS)),
+                     dim3(BS_CELLS),
                      0, st, M, S);
   MPAS_STEP(after_launch("k_stage_cells"));
   MPAS_STEP(prof_end(ctx));
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
-      hipLaunchKernelGGL((k_stage_tracer<NT, W>), dim3(xcd_grid(int64_t(M.nC) * NPL, BS)),
-                         dim3(BS), 0, st, M, S);
+      hipLaunchKernelGGL((k_stage_tracer<NT, W>),
+                         dim3(xcd_grid(int64_t(M.nC) * lanes_per_cell<NT, MPAS_TPL_TRACER>(),
+                                       BS_TRACER)),
+                         dim3(BS_TRACER), 0, st, M, S);
       MPAS_STEP(after_launch("k_stage_tracer"));
       MPAS_STEP(prof_end(ctx));
     }
   }
   if (S.ocean) return 0;  // u and h do not change: no diagnostics
   MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
-  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, S.yh, S.yu,
+  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS_DIAG)), dim3(BS_DIAG), 0, st, M, S.yh, S.yu,
                      vort_y, pv_vertex);
   MPAS_STEP(after_launch("k_diag_vertex"));
   MPAS_STEP(prof_end(ctx));
   MPAS_STEP(prof_begin(ctx, "k_diag_edge", b.edge_diag));
-  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, S.yh, pv_vertex,
+  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS_DIAG)), dim3(BS_DIAG), 0, st, M, S.yh, pv_vertex,
                      h_edge_y, pv_edge_y);
   MPAS_STEP(after_launch("k_diag_edge"));
   MPAS_STEP(prof_end(ctx));
@@ -251,11 +309,11 @@ int diagnostics_dm(const DMesh &M, hipStream_t st, int use_visc, const double *u
   hipLaunchKernelGGL(k_diag_cell, dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, u, ke, div,
                      use_visc);
   if (int rc = after_launch("k_diag_cell")) return rc;
-  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS)), dim3(BS), 0, st, M, h, u, vort,
-                     pv_vertex);
+  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS_DIAG)), dim3(BS_DIAG), 0, st, M, h, u,
+                     vort, pv_vertex);
   if (int rc = after_launch("k_diag_vertex")) return rc;
-  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS)), dim3(BS), 0, st, M, h, pv_vertex,
-                     h_edge, pv_edge);
+  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS_DIAG)), dim3(BS_DIAG), 0, st, M, h,
+                     pv_vertex, h_edge, pv_edge);
   return after_launch("k_diag_edge");
 }
 

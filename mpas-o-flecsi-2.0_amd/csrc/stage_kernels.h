@@ -143,8 +143,8 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
   double w[WE];
 #pragma unroll
   for (int j = 0; j < WE; j++) {
-    id[j] = ldc(M.eoe, j * E + e);
-    w[j] = ldc(M.woe, j * E + e);
+    id[j] = ldn<LOL == 0 && kNtStencil>(M.eoe, j * E + e);
+    w[j] = ldn<LOL == 0 && kNtStencil>(M.woe, j * E + e);
   }
   const double dc = ldc(M.dc, e);
   for (uint32_t k = k0; k < k1; k++) {
@@ -171,15 +171,16 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
                                  ldc(S.h_s, cc.x))) /
                        dc;
     if (S.use_visc) t += visc_at(M, e, k, cc, ldc(M.e2v, e), S.div, S.vort, S.visc_coef);
+    constexpr bool NTS = LOL == 0 && kNtNext;
     if (S.stage == 0) {
-      const double u0 = ld(S.u0, o);
+      const double u0 = ldo<NTS>(S.u0, o);
       st(S.yu, o, u0 + S.w_sub * t);
-      st(S.nu, o, u0 + S.w_rk * t);
+      stn<NTS>(S.nu, o, u0 + S.w_rk * t);
     } else if (S.stage < 3) {
-      st(S.yu, o, ld(S.u0, o) + S.w_sub * t);
-      st(S.nu, o, ld(S.nu, o) + S.w_rk * t);
+      st(S.yu, o, ldo<NTS>(S.u0, o) + S.w_sub * t);
+      stn<NTS>(S.nu, o, ldo<NTS>(S.nu, o) + S.w_rk * t);
     } else {
-      st(S.yu, o, ld(S.nu, o) + S.w_rk * t);
+      st(S.yu, o, ldo<NTS>(S.nu, o) + S.w_rk * t);
     }
   }
 }
@@ -204,9 +205,9 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
   double c1[WA], c3[WA];
 #pragma unroll
   for (int i = 0; i < WA; i++) {
-    ic[i] = ldc(M.adv, i * E + e);
-    c1[i] = ldc(M.advCoef, i * E + e);
-    c3[i] = ldc(M.advCoef3, i * E + e) * M.coef3rd;
+    ic[i] = ldn<LOL == 0 && kNtStencil>(M.adv, i * E + e);
+    c1[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef, i * E + e);
+    c3[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef3, i * E + e) * M.coef3rd;
   }
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
@@ -249,17 +250,18 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
 
 // compute_substep / accumulate_update / finalize_timestep for one tracer of
 // one cell (solve.cc:297-305, 332-340, 350-357)
+template <bool NT>
 __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uint32_t ot,
                                               double te, double hy) {
   if (S.stage == 0) {
-    const double h0 = ld(S.h0, ol), t0 = ld(S.tr0, ot);
+    const double h0 = ldo<NT>(S.h0, ol), t0 = ldo<NT>(S.tr0, ot);
     st(S.ytr, ot, (h0 * t0 + S.w_sub * te) / hy);
-    st(S.ntr, ot, t0 * h0 + S.w_rk * te);
+    stn<NT>(S.ntr, ot, t0 * h0 + S.w_rk * te);
   } else if (S.stage < 3) {
-    st(S.ytr, ot, (ld(S.h0, ol) * ld(S.tr0, ot) + S.w_sub * te) / hy);
-    st(S.ntr, ot, ld(S.ntr, ot) + S.w_rk * te);
+    st(S.ytr, ot, (ldo<NT>(S.h0, ol) * ldo<NT>(S.tr0, ot) + S.w_sub * te) / hy);
+    stn<NT>(S.ntr, ot, ldo<NT>(S.ntr, ot) + S.w_rk * te);
   } else {
-    st(S.ytr, ot, (ld(S.ntr, ot) + S.w_rk * te) / hy);
+    st(S.ytr, ot, (ldo<NT>(S.ntr, ot) + S.w_rk * te) / hy);
   }
 }
 
@@ -268,16 +270,17 @@ __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uin
 // curr + w_rk tend at stage 0 (copy_tracers, then +=), next += w_rk tend,
 // prev = curr + w_sub tend; the last stage writes curr = next + w_rk tend
 // (its prev, curr + 0 tend, is not read again).
+template <bool NT>
 __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, double te) {
   if (S.stage == 0) {
-    const double t0 = ld(S.tr0, ot);
-    st(S.ntr, ot, t0 + S.w_rk * te);
+    const double t0 = ldo<NT>(S.tr0, ot);
+    stn<NT>(S.ntr, ot, t0 + S.w_rk * te);
     st(S.ytr, ot, t0 + S.w_sub * te);
   } else if (S.stage < 3) {
-    st(S.ntr, ot, ld(S.ntr, ot) + S.w_rk * te);
-    st(S.ytr, ot, ld(S.tr0, ot) + S.w_sub * te);
+    stn<NT>(S.ntr, ot, ldo<NT>(S.ntr, ot) + S.w_rk * te);
+    st(S.ytr, ot, ldo<NT>(S.tr0, ot) + S.w_sub * te);
   } else {
-    st(S.ytr, ot, ld(S.ntr, ot) + S.w_rk * te);
+    st(S.ytr, ot, ldo<NT>(S.ntr, ot) + S.w_rk * te);
   }
 }
 
@@ -286,9 +289,9 @@ __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, doub
 // cell's first lane stores them), and per tracer either the centred
 // tendency with its update or the FCT bounds, upwind tendency and limiter
 // factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
-template <int NT, int W, int LOL = 0>
+template <int NT, int W, int LOL = 0, int TPLX = 0>
 __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
-  constexpr int TPL = NT > 0 ? tracers_per_lane<NT>() : 1;
+  constexpr int TPL = NT > 0 ? (TPLX ? TPLX : tracers_per_lane<NT>()) : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
   int sub;
   uint32_t k0, k1;
@@ -304,7 +307,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   uint32_t e[W];
   double dv[W];
 #pragma unroll
-  for (int j = 0; j < W; j++) e[j] = ldc(M.c2e, j * C + c);
+  for (int j = 0; j < W; j++) e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
 #pragma unroll
   for (int j = 0; j < W; j++) dv[j] = ldc(M.dv, e[j]);
 
@@ -347,15 +350,16 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
         th = sum_ascending<W>(0.0, fl, rank, neg, n);
       }
       th = th / ar;
+      constexpr bool NTS = LOL == 0 && kNtNext;
       if (S.stage == 0) {
-        const double h0 = ld(S.h0, ol);
+        const double h0 = ldo<NTS>(S.h0, ol);
         hy = h0 + S.w_sub * th;
-        if (lead) st(S.nh, ol, h0 + S.w_rk * th);
+        if (lead) stn<NTS>(S.nh, ol, h0 + S.w_rk * th);
       } else if (S.stage < 3) {
-        hy = ld(S.h0, ol) + S.w_sub * th;
-        if (lead) st(S.nh, ol, ld(S.nh, ol) + S.w_rk * th);
+        hy = ldo<NTS>(S.h0, ol) + S.w_sub * th;
+        if (lead) stn<NTS>(S.nh, ol, ldo<NTS>(S.nh, ol) + S.w_rk * th);
       } else {
-        hy = ld(S.nh, ol) + S.w_rk * th;
+        hy = ldo<NTS>(S.nh, ol) + S.w_rk * th;
       }
       if (lead) st(S.yh, ol, hy);
     }
@@ -369,7 +373,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       {
         uint32_t op[W];
 #pragma unroll
-        for (int j = 0; j < W; j++) op[j] = ldc(M.cOpp, j * C + c);
+        for (int j = 0; j < W; j++) op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
 #pragma unroll
         for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
       }
@@ -398,7 +402,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
               te = (j < n) ? x : te;
             }
           }
-          tracer_update(S, ol, ot + g, te, hy);
+          tracer_update<LOL == 0 && kNtNext>(S, ol, ot + g, te, hy);
         }
       } else if (S.scheme == MPAS_TRACER_FCT) {
         // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
@@ -466,9 +470,9 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 }
 
 // ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
-template <int NT, int W, int LOL = 0>
+template <int NT, int W, int LOL = 0, int TPLX = 0>
 __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item) {
-  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
   uint32_t k0, k1;
   const int c = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
@@ -481,8 +485,8 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   uint32_t e[W], op[W];
 #pragma unroll
   for (int j = 0; j < W; j++) {
-    e[j] = ldc(M.c2e, j * C + c);
-    op[j] = ldc(M.cOpp, j * C + c);
+    e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
+    op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
   }
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k, ot = ol * NT + t0;
@@ -549,11 +553,11 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     }
     if (S.ocean) {
 #pragma unroll
-      for (int g = 0; g < TPL; g++) ocean_update(S, ot + g, te[g]);
+      for (int g = 0; g < TPL; g++) ocean_update<LOL == 0 && kNtNext>(S, ot + g, te[g]);
     } else {
       const double hy = ld(S.yh, ol);
 #pragma unroll
-      for (int g = 0; g < TPL; g++) tracer_update(S, ol, ot + g, te[g], hy);
+      for (int g = 0; g < TPL; g++) tracer_update<LOL == 0 && kNtNext>(S, ol, ot + g, te[g], hy);
     }
   }
 }
