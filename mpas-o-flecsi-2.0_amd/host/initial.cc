@@ -220,12 +220,13 @@ int mg_init_ocean_case(const mg_mesh *m, int which, double a, double coef,
   for (int64_t c = 0; c < nC; c++) {
     if (which == 1) {
       const double maxG = 21983.7698959881, Knorm = 20.0 / maxG, kappa = 10.0, T0 = 50.0;
-      // x' = x cos(-rot) - y sin(-rot) (testcase.cc:196-197)
-      double rot = 2.0 * kPi * elapsed / 86400.;
+      // x' = x cos(-rot) - y sin(-rot), rot = 2 pi (elapsed / 86400)
+      // (testcase.cc:188-197)
+      double rot = 2.0 * kPi * (elapsed / 86400.);
       double xp = xC[c] * std::cos(-rot) - yC[c] * std::sin(-rot);
       tr[c] = T0 + Knorm * std::exp(kappa * xp / a);
     } else if (which == 2) {
-      double rot = 2.0 * kPi * elapsed / 86400.;
+      double rot = 2.0 * kPi * (elapsed / 86400.);  // testcase.cc:236-245
       double x = xC[c] * std::cos(rot) + yC[c] * std::sin(rot);
       double y = -xC[c] * std::sin(rot) + yC[c] * std::cos(rot);
       double lon = lonC[c] - rot;
@@ -243,6 +244,25 @@ int mg_init_ocean_case(const mg_mesh *m, int which, double a, double coef,
       tr[c] = 50. + 20. * cl * cl * std::cos(2. * lonC[c]) * std::exp(-6.0 * coef * elapsed / (a * a));
     }
   }
+  return 0;
+}
+
+int mg_ocean_error(const mg_mesh *m, int which, double a, double coef, double elapsed,
+                   const double *tr, int64_t stride, double *l2, double *max_err) {
+  if (!m || !tr || !l2 || !max_err || stride < 1 || which < 1 || which > 3) return -1;
+  const int64_t nC = mg_dim(m, "nCells"), nE = mg_dim(m, "nEdges");
+  std::vector<double> nv(static_cast<size_t>(nE)), lt(static_cast<size_t>(nC)),
+      truth(static_cast<size_t>(nC));
+  if (mg_init_ocean_case(m, which, a, coef, elapsed, nv.data(), lt.data(), truth.data()))
+    return -1;
+  double errSqd = 0.0, errMax = 0.0;
+  for (int64_t c = 0; c < nC; c++) {
+    const double error = (tr[c * stride] - truth[c]) / truth[c];
+    if (std::abs(error) > errMax) errMax = error;
+    errSqd += error * error;
+  }
+  *l2 = std::sqrt(errSqd) / double(nC);
+  *max_err = std::abs(errMax);
   return 0;
 }
 

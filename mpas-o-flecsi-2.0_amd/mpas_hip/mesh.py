@@ -52,6 +52,8 @@ def _lib():
                                          ctypes.c_int]
         lib.mg_init_sw_case6.argtypes = [vp, d, d, d, P_f64, P_f64]
         lib.mg_init_ocean_case.argtypes = [vp, ctypes.c_int, d, d, d, P_f64, P_f64, P_f64]
+        lib.mg_ocean_error.argtypes = [vp, ctypes.c_int, d, d, d, P_f64, ctypes.c_int64,
+                                       ctypes.POINTER(d), ctypes.POINTER(d)]
         _mg = lib
     return _mg
 
@@ -152,6 +154,19 @@ class Mesh:
         _lib().mg_init_sw_case5(self._h, a, omega, gravity, _p(u, P_f64), _p(h, P_f64),
                                 _p(hs, P_f64), _p(f, P_f64), _p(tr, P_f64), ntr)
         return u, h, hs, f, tr.reshape(self.nCells, ntr)
+
+    def ocean_error(self, which, tracers, elapsed, a=REF_A, coef=0.0, stride=1):
+        """(l2, max) of the ocean case's end-of-run report
+        (calc_error_case_<which>, apps/ocean/tasks/testcase.cc:175-294) for
+        tracers[c * stride], tracer 0 of level 0."""
+        tr = np.ascontiguousarray(tracers, dtype=np.float64)
+        if tr.size < (self.nCells - 1) * stride + 1:
+            raise ValueError("tracers is too short for the stride")
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        if _lib().mg_ocean_error(self._h, which, a, coef, elapsed, _p(tr, P_f64), stride,
+                                 ctypes.byref(l2), ctypes.byref(mx)):
+            raise ValueError("mg_ocean_error: bad argument")
+        return l2.value, mx.value
 
     def ocean_case(self, which, a=REF_A, coef=0.0, elapsed=0.0):
         nv, lt, tr = np.zeros(self.nEdges), np.zeros(self.nCells), np.zeros(self.nCells)
