@@ -287,6 +287,162 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
     }
 
 
+def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
+                           groups=((7, 64, 120.0), (5, 64, 480.0)), sample=2):
+    """Batched adjoint ensemble: 64 members on the level-7 mesh and 64 on the
+    level-5 mesh, 2 FCT tracers, 100 steps, checkpoints every 10 steps, each
+    member with its own seeded bump and its own seeded set of 2,000
+    observation cells (observations from a seeded-perturbed run of the
+    member).  The members of a mesh are the levels of one context, one level
+    per lane; a mesh's members are split over as few equal contexts as keep
+    every tape array below the 4 GiB the kernels address (the largest holds
+    edges x levels x tracers x 16 stencil slots); forward with checkpoints and
+    adjoint sweep of every context run back to back.  Baseline: each member
+    alone (its own single-level adjoint), timed on `sample` members per mesh
+    and scaled to the member count.  The sampled members' gradients equal
+    their single runs bit for bit."""
+    from mpas_hip import TRACER_FCT, Context, Mesh, sw_workload_state
+    kw = dict(nTracers=n_tr, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    batches, singles, total = [], [], 0
+    t_setup = time.perf_counter()
+    for L, n, dt in groups:
+        mesh = Mesh(L)
+        C, E = mesh.nCells, mesh.nEdges
+        members = []
+        for i in range(n):
+            j = total + i
+            rng = np.random.default_rng(700 + j)
+            u, h, tr = sw_workload_state(mesh, n_tr, seed=300 + j, u0=rng.uniform(10.0, 30.0),
+                                         bump_amp=rng.uniform(5.0, 20.0))
+            obs = rng.choice(C, n_obs, replace=False)
+            pert = (0.5 * rng.standard_normal(E), 2.0 * rng.standard_normal(C),
+                    0.02 * rng.standard_normal(C * n_tr))
+            members.append(((u, h, tr.ravel()), obs, pert))
+        total += n
+        cap = max(1, (2 ** 32 - 1) // (8 * E * n_tr * 16))
+        nctx = -(-n // cap)
+        per_ctx = -(-n // nctx)
+        for a in range(0, n, per_ctx):
+            part = members[a:a + per_ctx]
+            nM = len(part)
+            pack = lambda xs, w: np.stack([x.reshape(-1, w) for x in xs], axis=1).ravel()
+            X0 = [pack([m_[0][0] for m_ in part], 1), pack([m_[0][1] for m_ in part], 1),
+                  pack([m_[0][2] for m_ in part], n_tr)]
+            XP = [pack([m_[0][q] + m_[2][q] for m_ in part], 1 if q < 2 else n_tr)
+                  for q in range(3)]
+            ctx = Context(mesh, nLevels=nM, **kw)
+            ctx.sw_upload(X0[1], X0[0], X0[2], np.full(C, 4500.0))
+            ctx.adj_init(k)
+            x0, xp, x1 = ctx.state_arrays(), ctx.state_arrays(), ctx.state_arrays()
+            for d, v in zip(x0, X0):
+                d.upload(v)
+            for d, v in zip(xp, XP):
+                d.upload(v)
+            ctx.adj_forward(dt, steps, xp, x1)  # observations
+            ctx.sync()
+            ho = x1[1].numpy().reshape(C, nM)
+            to = x1[2].numpy().reshape(C, nM, n_tr)
+            batches.append(dict(ctx=ctx, mesh=mesh, L=L, dt=dt, part=part, nM=nM, x0=x0, x1=x1,
+                                ho=ho, to=to, yb=ctx.state_arrays(), g=ctx.state_arrays()))
+        singles.append((mesh, dt, members[:sample], n))
+    t_setup = time.perf_counter() - t_setup
+
+    def seeds(b, hN, tN):
+        C, nM = b["mesh"].nCells, b["nM"]
+        sh, st_ = np.zeros((C, nM)), np.zeros((C, nM, n_tr))
+        loss = 0.0
+        for q, (_, obs, _) in enumerate(b["part"]):
+            dh = hN[obs, q] - b["ho"][obs, q]
+            dT = tN[obs, q, :] - b["to"][obs, q, :]
+            sh[obs, q] = 2 * dh
+            st_[obs, q, :] = 2 * dT
+            loss += float(np.sum(dh ** 2) + np.sum(dT ** 2))
+        return sh.ravel(), st_.ravel(), loss
+
+    # batched: forward with checkpoints, then the adjoint sweep, every context
+    for b in batches:
+        b["ctx"].sync()
+    t0 = time.perf_counter()
+    for b in batches:
+        b["ctx"].adj_forward(b["dt"], steps, b["x0"], b["x1"])
+    for b in batches:
+        b["ctx"].sync()
+    t_fwd = time.perf_counter() - t0
+    loss = 0.0
+    for b in batches:  # the loss seeds (host work, not timed)
+        C, nM = b["mesh"].nCells, b["nM"]
+        sh, st_, l_ = seeds(b, b["x1"][1].numpy().reshape(C, nM),
+                            b["x1"][2].numpy().reshape(C, nM, n_tr))
+        loss += l_
+        b["yb"][0].upload(np.zeros(b["mesh"].nEdges * nM))
+        b["yb"][1].upload(sh)
+        b["yb"][2].upload(st_)
+        b["ctx"].sync()
+    t0 = time.perf_counter()
+    for b in batches:
+        b["ctx"].adj_backward(b["yb"], b["g"])
+    for b in batches:
+        b["ctx"].sync()
+    t_bwd = time.perf_counter() - t0
+    # baseline: sampled members alone, and their gradients against the batch
+    t_seq, same = 0.0, True
+    for (mesh, dt, sample_members, n), b in zip(singles, [next(x for x in batches if x["L"] == g[0])
+                                                         for g in groups]):
+        C, E = mesh.nCells, mesh.nEdges
+        c1 = Context(mesh, **kw)
+        (u, h, tr), _, _ = sample_members[0]
+        c1.sw_upload(h, u, tr, np.full(C, 4500.0))
+        c1.adj_init(k)
+        per = []
+        gb = [a.numpy() for a in b["g"]]
+        for q, ((u, h, tr), obs, _) in enumerate(sample_members):
+            a0, a1, y, g = (c1.state_arrays() for _ in range(4))
+            for d, v in zip(a0, (u, h, tr)):
+                d.upload(v)
+            c1.sync()
+            t0 = time.perf_counter()
+            c1.adj_forward(dt, steps, a0, a1)
+            c1.sync()
+            tf = time.perf_counter() - t0
+            hN, tN = a1[1].numpy(), a1[2].numpy().reshape(C, n_tr)
+            sh, st_ = np.zeros(C), np.zeros((C, n_tr))
+            sh[obs] = 2 * (hN[obs] - b["ho"][obs, q])
+            st_[obs, :] = 2 * (tN[obs, :] - b["to"][obs, q, :])
+            for d, v in zip(y, (np.zeros(E), sh, st_.ravel())):
+                d.upload(v)
+            c1.sync()
+            t1 = time.perf_counter()
+            c1.adj_backward(y, g)
+            c1.sync()
+            per.append(tf + time.perf_counter() - t1)
+            nM = b["nM"]
+            same &= np.array_equal(g[0].numpy().view(np.uint64),
+                                   gb[0].reshape(E, nM)[:, q].copy().view(np.uint64))
+            same &= np.array_equal(g[1].numpy().view(np.uint64),
+                                   gb[1].reshape(C, nM)[:, q].copy().view(np.uint64))
+            same &= np.array_equal(g[2].numpy().view(np.uint64),
+                                   gb[2].reshape(C, nM, n_tr)[:, q, :].ravel().copy()
+                                   .view(np.uint64))
+        c1.close()
+        t_seq += n * float(np.median(per))
+    for b in batches:
+        b["ctx"].close()
+    t_batch = t_fwd + t_bwd
+    return {
+        "workload": "batched adjoint ensemble: " + ", ".join(
+            f"{n} members on level {L} (dt {dt:g} s)" for L, n, dt in groups) +
+            f"; FP64 SW RK4 + FCT, {n_tr} tracers, {steps} steps, checkpoint every {k} steps, "
+            f"{n_obs} seeded observation cells per member; members of a mesh are the levels "
+            f"of one context (split in equal contexts where a tape array would pass 4 GiB)",
+        "members": total, "contexts": len(batches), "setup_s": t_setup,
+        "forward_with_checkpoints_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
+        "batched_ms": 1e3 * t_batch, "sequential_ms_estimated": 1e3 * t_seq,
+        "sequential_sampled_members_per_mesh": sample,
+        "speedup_vs_sequential": t_seq / t_batch, "loss": loss,
+        "sampled_gradients_bitwise_equal_to_single_runs": bool(same),
+    }
+
+
 def copy_calibration(nbytes):
     """k_copy over nbytes, 10 launches: reads and writes nbytes each."""
     from mpas_hip import Context, Mesh
@@ -330,6 +486,9 @@ def main():
                     help="batched steps of the ensemble measurement (its workload has 500)")
     ap.add_argument("--ens-seq-steps", type=int, default=20,
                     help="steps per member of the member-after-member baseline")
+    ap.add_argument("--no-adjoint-ensemble", action="store_true",
+                    help="skip the batched adjoint ensemble (64 x level 7 + 64 x level 5)")
+    ap.add_argument("--adjoint-ensemble-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel k_copy over BYTES, 10 times "
                          "(the known-size reference for HBM counters)")
@@ -347,6 +506,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_bench(steps=args.adj_steps)), flush=True)
+        return
+    if args.adjoint_ensemble_only:
+        print(json.dumps(adjoint_ensemble_bench()), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps)),
@@ -435,6 +597,13 @@ def main():
         log(f"  {out['ensemble']['ms_per_step_batched_graph']:.3f} ms/step batched, "
             f"{out['ensemble']['ms_per_step_sequential']:.3f} sequential, speedup "
             f"{out['ensemble']['speedup_vs_sequential']:.1f}")
+    if not args.no_adjoint_ensemble:
+        log("batched adjoint ensemble (64 x L7 + 64 x L5, 100 steps) ...")
+        out["adjoint_ensemble"] = adjoint_ensemble_bench()
+        ae = out["adjoint_ensemble"]
+        log(f"  {ae['batched_ms']:.0f} ms batched, {ae['sequential_ms_estimated']:.0f} ms "
+            f"sequential, speedup {ae['speedup_vs_sequential']:.2f}, bitwise "
+            f"{ae['sampled_gradients_bitwise_equal_to_single_runs']}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.cpu_steps,

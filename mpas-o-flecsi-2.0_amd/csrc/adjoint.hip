@@ -107,6 +107,20 @@ struct AdjParams {
 
 __device__ __forceinline__ double fsign(double F) { return (F < 0.0) ? -1.0 : double(F > 0.); }
 
+// Work item -> (entity, level): one level per lane, an entity's levels on
+// adjacent lanes (with one level, the entity is the item).  groups > 1
+// keeps `groups` adjacent lanes per (entity, level), lane = their index.
+template <int groups = 1>
+__device__ __forceinline__ int level_item(uint32_t L, uint32_t &k0, uint32_t &k1, int &lane) {
+  const int item = xcd_item();
+  const int q = item / groups;
+  lane = item - q * groups;
+  const int ent = q / int(L);
+  k0 = uint32_t(q - ent * int(L));
+  k1 = k0 + 1;
+  return ent;
+}
+
 // Slope of the rescaled antidiffusive flux in the flux x with the limiter
 // factors frozen: max(0,x)*min(out1,in2) + min(0,x)*min(in1,out2)
 // (ocean/solve.cc:270-282).
@@ -137,7 +151,9 @@ __global__ __launch_bounds__(BS) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ du,
                                                   const double *__restrict__ dh, double *dvort,
                                                   double *dpvv) {
-  const int v = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int v = level_item(M.nL, k0, k1, lane_);
   if (v >= M.nV) return;
   const uint32_t L = M.nL, V = M.nV;
   int x[3];
@@ -149,7 +165,7 @@ __global__ __launch_bounds__(BS) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
     kt[i] = ld(M.kite, i * V + v);
   }
   const double at = ld(M.areaTri, v);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     double dcirc = 0.0, dhv = 0.0;
     for (int i = 0; i < 3; i++) {
       const uint32_t e = v2e_id(x[i]);
@@ -174,13 +190,15 @@ __global__ __launch_bounds__(BS) void k_tl_edge_a(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ dT,
                                                   const double *__restrict__ dpvv, double *dhe,
                                                   double *dpve, double *dflux) {
-  const int e = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int e = level_item(M.nL, k0, k1, lane_);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE, NT = P.nT;
   const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
   const double dv = ld(M.dv, e), dc = ld(M.dc, e);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     const double dhe_ = 0.5 * (ld(dh, c1 * L + k) + ld(dh, c2 * L + k));
     st(dhe, o, dhe_);
@@ -224,13 +242,15 @@ __global__ __launch_bounds__(BS) void k_tl_cell(DMesh M, AdjParams P, Rec R, Rec
                                                 const double *__restrict__ dflux, double *dnh,
                                                 double *dQ, double *dke, double *ddiv,
                                                 double *dhy, double *dTy) {
-  const int c = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int c = level_item(M.nL, k0, k1, lane_);
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
   const int n = ld(M.nEoC, c);
   const uint8_t neg = ld(M.sgnNeg, c);
   const double ar = ld(M.area, c);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
     double dk = 0.0, dd = 0.0, dth = 0.0, dtT[8] = {};
     for (int j = 0; j < n; j++) {
@@ -290,18 +310,20 @@ __global__ __launch_bounds__(BS) void k_tl_edge_b(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ ddiv,
                                                   const double *__restrict__ dvort, double *dnu,
                                                   double *duy) {
-  const int e = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int e = level_item(M.nL, k0, k1, lane_);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
   if (P.freeze_u) {  // u stays the step's initial u (solve.cc:307-313, 360-366)
-    for (uint32_t k = 0; k < L; k++) st(duy, e * L + k, ld(du0, e * L + k));
+    for (uint32_t k = k0; k < k1; k++) st(duy, e * L + k, ld(du0, e * L + k));
     return;
   }
   const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
   const int n = ld(M.nEoE, e);
   const double dc = ld(M.dc, e), dv = ld(M.dv, e);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     const double pve = ld(R.pve, o), dpv = ld(dpve, o);
     double dq = 0.0;
@@ -345,10 +367,12 @@ __global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Re
                                                 double *ub_out, double *hb_out, double *trb_out,
                                                 double *aw_u, double *aw_h, double *aw_tr,
                                                 double *tb_u, double *tb_h, double *tb_tr) {
-  const int i = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int i = level_item(M.nL, k0, k1, lane_);
   const uint32_t L = M.nL, NT = P.nT;
   if (i < M.nE) {
-    for (uint32_t k = 0; k < L; k++) {
+    for (uint32_t k = k0; k < k1; k++) {
       const uint32_t o = i * L + k;
       const double ub = ld(ub_in, o);
       st(aw_u, o, ub);
@@ -357,7 +381,7 @@ __global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Re
     }
   }
   if (i < M.nC) {
-    for (uint32_t k = 0; k < L; k++) {
+    for (uint32_t k = k0; k < k1; k++) {
       const uint32_t ol = i * L + k;
       const double hf = ld(Rf.h, ol), h0 = ld(R0.h, ol);
       double htot = ld(hb_in, ol);
@@ -403,8 +427,9 @@ __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams 
                                                   const double *__restrict__ tb_tr, double *ubA,
                                                   double *heb, double *pveb, double *r1,
                                                   double *r2, double *hbf) {
-  const int item = xcd_item();
-  const int e = item / kEdgeLanes, lane = item % kEdgeLanes;
+  uint32_t k0, k1;
+  int lane;
+  const int e = level_item<kEdgeLanes>(M.nL, k0, k1, lane);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE, NT = P.nT;
   constexpr int Q = kChunk / kEdgeLanes;  // slots per lane and round
@@ -413,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams 
   const double A1 = ld(M.area, c1), A2 = ld(M.area, c2);
   const double dv = ld(M.dv, e), dc = ld(M.dc, e);
   const int na = P.fct ? ld(M.nAdv, e) : 0;
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     const double ue = ld(R.u, o), hee = ld(R.he, o), pvee = ld(R.pve, o);
     double pb = 0.0, s1 = 0.0, s2 = 0.0;
@@ -521,7 +546,9 @@ __global__ __launch_bounds__(BS) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ pveb,
                                                   const double *__restrict__ tb_u, double *vT,
                                                   double *hvT) {
-  const int v = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int v = level_item(M.nL, k0, k1, lane_);
   if (v >= M.nV) return;
   const uint32_t L = M.nL, V = M.nV;
   int x[3];
@@ -533,7 +560,7 @@ __global__ __launch_bounds__(BS) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
     kt[i] = ld(M.kite, i * V + v);
   }
   const double at = ld(M.areaTri, v);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     double pvvb = 0.0;
     for (int i = 0; i < 3; i++) pvvb += 0.5 * ld(pveb, v2e_id(x[i]) * L + k);
     double hv, pvv;
@@ -568,15 +595,16 @@ __global__ __launch_bounds__(BS) void k_ad_cell(
     const double *__restrict__ hbf, double *keT, double *divT,
     double *xb_h, double *xb_tr, const double *__restrict__ aw_h,
     const double *__restrict__ aw_tr, double *tb_h, double *tb_tr) {
-  const int item = xcd_item();
-  const int c = item / kCellLanes, lane = item % kCellLanes;
+  uint32_t k0, k1;
+  int lane;
+  const int c = level_item<kCellLanes>(M.nL, k0, k1, lane);
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
   const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
   const uint8_t neg = ld(M.sgnNeg, c);
   const double ar = ld(M.area, c);
   constexpr int Q = kChunk / kCellLanes;  // entries per lane and round
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
     double g1 = 0.0, hsum = 0.0, Tb[8] = {};
     for (int j = lane; j < n; j += kCellLanes) {
@@ -637,13 +665,15 @@ __global__ __launch_bounds__(BS) void k_ad_edge_b(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ divT,
                                                   const double *__restrict__ vT, double *xb_u,
                                                   const double *__restrict__ aw_u, double *tb_u) {
-  const int e = xcd_item();
+  uint32_t k0, k1;
+  int lane_;
+  const int e = level_item(M.nL, k0, k1, lane_);
   if (e >= M.nE) return;
   const uint32_t L = M.nL;
   const int2 cc = ld(M.e2c, e), vv = ld(M.e2v, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
   const double dc = ld(M.dc, e), dv = ld(M.dv, e);
-  for (uint32_t k = 0; k < L; k++) {
+  for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     double ub = ld(ubA, o) +
                 0.5 * dc * dv * ld(R.u, o) * (ld(keT, c1 * L + k) + ld(keT, c2 * L + k));
@@ -659,6 +689,10 @@ __global__ __launch_bounds__(BS) void k_ad_edge_b(DMesh M, AdjParams P, Rec R,
 int dalloc(AdjState &A, size_t n, double **out) {
   *out = nullptr;
   if (!n) return 0;
+  // the kernels address every array with 32-bit byte offsets
+  if (n > size_t(UINT32_MAX) / sizeof(double))
+    MPAS_FAIL(MPAS_ESHAPE, "an adjoint tape array of " + std::to_string(sizeof(double) * n) +
+                               " bytes exceeds the 4 GiB the kernels address");
   void *p = nullptr;
   if (hipMalloc(&p, sizeof(double) * n) != hipSuccess)
     MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(double) * n) +
@@ -873,7 +907,8 @@ AdjParams adj_params(const mpas_ctx *ctx, int s, double dt) {
 #define ADJ_LAUNCH(kernel, n, ...)                                                         \
   do {                                                                                     \
     if (int rc_ = prof_begin(ctx, #kernel, 0.0)) return rc_;                               \
-    hipLaunchKernelGGL(kernel, dim3(xcd_grid((n), BS)), dim3(BS), 0, ctx->stream,          \
+    hipLaunchKernelGGL(kernel, dim3(xcd_grid(int64_t(n) * M.nL, BS)), dim3(BS), 0,         \
+                       ctx->stream,                                                        \
                        __VA_ARGS__);                                                       \
     if (int rc_ = after_launch(#kernel)) return rc_;                                       \
     if (int rc_ = prof_end(ctx)) return rc_;                                               \

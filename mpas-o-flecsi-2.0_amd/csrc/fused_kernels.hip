@@ -60,14 +60,17 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 #endif
 #define MPAS_WAVES(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1, (n) > 0 ? (n) : 8)))
 
-template <int WE>
+// LOL: 0 for one entity per lane (all its levels), 1 for one level per
+// lane with an entity's levels on adjacent lanes (contexts of more than one
+// level: the accesses of a wave are then contiguous runs across levels).
+template <int WE, int LOL>
 __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
                                                                                StageIO S) {
-  stage_edges<WE>(M, S, xcd_item());
+  stage_edges<WE, LOL>(M, S, xcd_item());
 }
-template <int NT, int WA>
+template <int NT, int WA, int LOL>
 __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
-  stage_ho<NT, WA>(M, S, xcd_item());
+  stage_ho<NT, WA, LOL>(M, S, xcd_item());
 }
 // Tracers per lane of the cell kernels (0: pairs when the count is even).
 #ifndef MPAS_TPL_CELLS
@@ -84,29 +87,45 @@ template <int NT>
 constexpr int tpl_of(int x) {
   return x && NT % x == 0 ? x : 0;
 }
-template <int NT, int W>
+template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
                                                                                StageIO S) {
-  stage_cells<NT, W, 0, tpl_of<NT>(MPAS_TPL_CELLS)>(M, S, xcd_item());
+  stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS)>(M, S, xcd_item());
 }
-template <int NT, int W>
+template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
-  stage_tracer<NT, W, 0, tpl_of<NT>(MPAS_TPL_TRACER)>(M, S, xcd_item());
+  stage_tracer<NT, W, LOL, tpl_of<NT>(MPAS_TPL_TRACER)>(M, S, xcd_item());
 }
 
-__global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, const double *__restrict__ h,
-                                                    const double *__restrict__ u,
-                                                    double *vorticity, double *pv_vertex) {
-  const int v = xcd_item();
-  if (v < M.nV) diag_vertex(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex);
+template <int LOL>
+__global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, StageIO S,
+                                                         const double *__restrict__ h,
+                                                         const double *__restrict__ u,
+                                                         double *vorticity, double *pv_vertex) {
+  int sub;
+  uint32_t k0, k1;
+  const int v = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
+  if (v < M.nV) diag_vertex(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex, k0, k1);
 }
 
-__global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, const double *__restrict__ h,
-                                                  const double *__restrict__ pv_vertex,
-                                                  double *h_edge, double *pv_edge) {
-  const int e = xcd_item();
-  if (e < M.nE) diag_edge(M, e, h, pv_vertex, h_edge, pv_edge);
+template <int LOL>
+__global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, StageIO S,
+                                                       const double *__restrict__ h,
+                                                       const double *__restrict__ pv_vertex,
+                                                       double *h_edge, double *pv_edge) {
+  int sub;
+  uint32_t k0, k1;
+  const int e = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
+  if (e < M.nE) diag_edge(M, e, h, pv_vertex, h_edge, pv_edge, k0, k1);
+}
+
+// The layout of a context's launches and the StageIO fields it reads.
+int layout_of(const DMesh &M) { return M.nL > 1 ? 1 : 0; }
+StageIO with_layout(StageIO S, const DMesh &M) {
+  S.lpl = 1;
+  S.nchunk = M.nL;
+  return S;
 }
 
 // ke and divergence of a state (the cell part of the initial diagnostics)
@@ -186,10 +205,12 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
 // stage's output.  (Running the independent pairs on two streams at once was
 // measured slower on the level-8 workload: the pairs contend for L2 and HBM.)
 // ctx (may be null) only times the launches while profiling is on.
-template <int NT, int W, int WE>
-int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO &S,
+template <int NT, int W, int WE, int LOL>
+int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO &S0,
                    const Bytes &b, double *vort_y, double *pv_vertex, double *h_edge_y,
                    double *pv_edge_y) {
+  const StageIO S = LOL ? with_layout(S0, M) : S0;
+  const int64_t lv = LOL ? M.nL : 1;  // lanes per entity and tracer lane group
   constexpr int NPL = NT > 0 ? NT / tracers_per_lane<NT>() : 1;  // lanes per entity
   const bool fct = NT > 0 && S.scheme == MPAS_TRACER_FCT;
 #define MPAS_STEP(expr) \
@@ -198,33 +219,31 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   } while (0)
   if (!S.ocean) {
     MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
-    hipLaunchKernelGGL((k_stage_edges<WE>), dim3(xcd_grid(M.nE, BS_EDGES)), dim3(BS_EDGES), 0, st,
-                       M, S);
+    hipLaunchKernelGGL((k_stage_edges<WE, LOL>), dim3(xcd_grid(M.nE * lv, BS_EDGES)),
+                       dim3(BS_EDGES), 0, st, M, S);
     MPAS_STEP(after_launch("k_stage_edges"));
     MPAS_STEP(prof_end(ctx));
   }
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
-      hipLaunchKernelGGL((k_stage_ho<NT, WE>), dim3(xcd_grid(int64_t(M.nE) * NPL, BS_HO)),
-                         dim3(BS_HO),
-                         0, st, M, S);
+      hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>), dim3(xcd_grid(M.nE * lv * NPL, BS_HO)),
+                         dim3(BS_HO), 0, st, M, S);
       MPAS_STEP(after_launch("k_stage_ho"));
       MPAS_STEP(prof_end(ctx));
     }
   }
   MPAS_STEP(prof_begin(ctx, "k_stage_cells", b.cells));
-  hipLaunchKernelGGL((k_stage_cells<NT, W>),
-                     dim3(xcd_grid(int64_t(M.nC) * lanes_per_cell<NT, MPAS_TPL_CELLS>(), BS_CELLS)),
-                     dim3(BS_CELLS),
-                     0, st, M, S);
+  hipLaunchKernelGGL((k_stage_cells<NT, W, LOL>),
+                     dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_CELLS>(), BS_CELLS)),
+                     dim3(BS_CELLS), 0, st, M, S);
   MPAS_STEP(after_launch("k_stage_cells"));
   MPAS_STEP(prof_end(ctx));
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
-      hipLaunchKernelGGL((k_stage_tracer<NT, W>),
-                         dim3(xcd_grid(int64_t(M.nC) * lanes_per_cell<NT, MPAS_TPL_TRACER>(),
+      hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>),
+                         dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_TRACER>(),
                                        BS_TRACER)),
                          dim3(BS_TRACER), 0, st, M, S);
       MPAS_STEP(after_launch("k_stage_tracer"));
@@ -233,13 +252,13 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   }
   if (S.ocean) return 0;  // u and h do not change: no diagnostics
   MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
-  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS_DIAG)), dim3(BS_DIAG), 0, st, M, S.yh, S.yu,
-                     vort_y, pv_vertex);
+  hipLaunchKernelGGL(k_diag_vertex<LOL>, dim3(xcd_grid(M.nV * lv, BS_DIAG)), dim3(BS_DIAG), 0,
+                     st, M, S, S.yh, S.yu, vort_y, pv_vertex);
   MPAS_STEP(after_launch("k_diag_vertex"));
   MPAS_STEP(prof_end(ctx));
   MPAS_STEP(prof_begin(ctx, "k_diag_edge", b.edge_diag));
-  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS_DIAG)), dim3(BS_DIAG), 0, st, M, S.yh, pv_vertex,
-                     h_edge_y, pv_edge_y);
+  hipLaunchKernelGGL(k_diag_edge<LOL>, dim3(xcd_grid(M.nE * lv, BS_DIAG)), dim3(BS_DIAG), 0, st,
+                     M, S, S.yh, pv_vertex, h_edge_y, pv_edge_y);
   MPAS_STEP(after_launch("k_diag_edge"));
   MPAS_STEP(prof_end(ctx));
 #undef MPAS_STEP
@@ -251,8 +270,12 @@ template <int NT>
 int launch_stage_w(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO &S,
                    const Bytes &b, double *vort_y, double *pv_vertex, double *h_edge_y,
                    double *pv_edge_y) {
-  if (M.narrow) return launch_stage_t<NT, 6, 10>(MPAS_STAGE_ARGS);
-  return launch_stage_t<NT, 8, 16>(MPAS_STAGE_ARGS);
+  if (layout_of(M)) {
+    if (M.narrow) return launch_stage_t<NT, 6, 10, 1>(MPAS_STAGE_ARGS);
+    return launch_stage_t<NT, 8, 16, 1>(MPAS_STAGE_ARGS);
+  }
+  if (M.narrow) return launch_stage_t<NT, 6, 10, 0>(MPAS_STAGE_ARGS);
+  return launch_stage_t<NT, 8, 16, 0>(MPAS_STAGE_ARGS);
 }
 
 int launch_stage_nt(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO &S,
@@ -309,11 +332,22 @@ int diagnostics_dm(const DMesh &M, hipStream_t st, int use_visc, const double *u
   hipLaunchKernelGGL(k_diag_cell, dim3(xcd_grid(M.nC, BS)), dim3(BS), 0, st, M, u, ke, div,
                      use_visc);
   if (int rc = after_launch("k_diag_cell")) return rc;
-  hipLaunchKernelGGL(k_diag_vertex, dim3(xcd_grid(M.nV, BS_DIAG)), dim3(BS_DIAG), 0, st, M, h, u,
-                     vort, pv_vertex);
+  const StageIO S = with_layout(StageIO{}, M);
+  if (layout_of(M)) {
+    hipLaunchKernelGGL(k_diag_vertex<1>, dim3(xcd_grid(int64_t(M.nV) * M.nL, BS_DIAG)),
+                       dim3(BS_DIAG), 0, st, M, S, h, u, vort, pv_vertex);
+  } else {
+    hipLaunchKernelGGL(k_diag_vertex<0>, dim3(xcd_grid(M.nV, BS_DIAG)), dim3(BS_DIAG), 0, st, M,
+                       S, h, u, vort, pv_vertex);
+  }
   if (int rc = after_launch("k_diag_vertex")) return rc;
-  hipLaunchKernelGGL(k_diag_edge, dim3(xcd_grid(M.nE, BS_DIAG)), dim3(BS_DIAG), 0, st, M, h,
-                     pv_vertex, h_edge, pv_edge);
+  if (layout_of(M)) {
+    hipLaunchKernelGGL(k_diag_edge<1>, dim3(xcd_grid(int64_t(M.nE) * M.nL, BS_DIAG)),
+                       dim3(BS_DIAG), 0, st, M, S, h, pv_vertex, h_edge, pv_edge);
+  } else {
+    hipLaunchKernelGGL(k_diag_edge<0>, dim3(xcd_grid(M.nE, BS_DIAG)), dim3(BS_DIAG), 0, st, M, S,
+                       h, pv_vertex, h_edge, pv_edge);
+  }
   return after_launch("k_diag_edge");
 }
 

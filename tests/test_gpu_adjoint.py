@@ -40,7 +40,8 @@ def _setup(level, nT, K, visc2=1e5, kappa2=1e3, seed=1):
 
 
 def _rand_state(ctx, rng, scale=(1.0, 1.0, 0.01)):
-    n = (ctx.nEdges, ctx.nCells, ctx.nCells * ctx.nTracers)
+    L = ctx.nLevels
+    n = (ctx.nEdges * L, ctx.nCells * L, ctx.nCells * L * ctx.nTracers)
     host = [scale[i] * rng.standard_normal(n[i]) for i in range(3)]
     dev = ctx.state_arrays()
     for i in range(3):
@@ -193,3 +194,84 @@ def test_adjoint_error_paths():
     with pytest.raises(MpasError):
         ctx.adj_init(0)
     ctx.close()
+
+
+def _pack_members(m, members, nT):
+    """[entity][member] (u, h) and [cell][member][tracer] tracers."""
+    U = np.stack([x[0] for x in members], axis=1).ravel()
+    H = np.stack([x[1] for x in members], axis=1).ravel()
+    T = np.stack([x[2].reshape(m.nCells, nT) for x in members], axis=1).ravel()
+    return U, H, T
+
+
+def test_members_as_levels_equal_single_runs():
+    """Levels of one context are independent, so a context whose levels are
+    ensemble members (the batched adjoint) gives every member the final
+    state and gradient of its own single-level run, bit for bit; the
+    levels run one per lane (adjacent lanes, adjacent members)."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(3)
+    nM, nT, K, nsteps, dt = 6, 2, 3, 7, 600.0
+    kw = dict(nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    members = []
+    for g in range(nM):
+        u, h, tr = sw_workload_state(m, nT, seed=20 + g, u0=12.0 + 2 * g, bump_amp=6.0 + g)
+        members.append((u, h, tr.ravel()))
+    U, H, T = _pack_members(m, members, nT)
+    cm = Context(m, nLevels=nM, **kw)
+    cm.sw_upload(H, U, T, np.full(m.nCells, 4500.0))
+    cm.adj_init(K)
+    x0 = cm.state_arrays()
+    for a, v in zip(x0, (U, H, T)):
+        a.upload(v)
+    x1 = cm.state_arrays()
+    cm.adj_forward(dt, nsteps, x0, x1)
+    rng = np.random.default_rng(11)
+    E, C = m.nEdges, m.nCells
+    yb = [rng.standard_normal(E * nM), rng.standard_normal(C * nM),
+          0.01 * rng.standard_normal(C * nM * nT)]
+    y = cm.state_arrays()
+    for a, v in zip(y, yb):
+        a.upload(v)
+    xb = cm.state_arrays()
+    cm.adj_backward(y, xb)
+    cm.sync()
+    f = [x1[0].numpy().reshape(E, nM), x1[1].numpy().reshape(C, nM),
+         x1[2].numpy().reshape(C, nM, nT)]
+    g = [xb[0].numpy().reshape(E, nM), xb[1].numpy().reshape(C, nM),
+         xb[2].numpy().reshape(C, nM, nT)]
+    c1 = Context(m, **kw)
+    c1.sw_upload(members[0][1], members[0][0], members[0][2], np.full(C, 4500.0))
+    c1.adj_init(K)
+    for k, (u, h, tr) in enumerate(members):
+        a0 = c1.state_arrays()
+        for a, v in zip(a0, (u, h, tr)):
+            a.upload(v)
+        a1 = c1.state_arrays()
+        c1.adj_forward(dt, nsteps, a0, a1)
+        yk = c1.state_arrays()
+        yk[0].upload(yb[0].reshape(E, nM)[:, k].copy())
+        yk[1].upload(yb[1].reshape(C, nM)[:, k].copy())
+        yk[2].upload(yb[2].reshape(C, nM, nT)[:, k, :].ravel().copy())
+        bk = c1.state_arrays()
+        c1.adj_backward(yk, bk)
+        c1.sync()
+        assert bitwise_equal(a1[0].numpy(), f[0][:, k].copy()), k
+        assert bitwise_equal(a1[1].numpy(), f[1][:, k].copy()), k
+        assert bitwise_equal(a1[2].numpy(), f[2][:, k, :].ravel().copy()), k
+        assert bitwise_equal(bk[0].numpy(), g[0][:, k].copy()), k
+        assert bitwise_equal(bk[1].numpy(), g[1][:, k].copy()), k
+        assert bitwise_equal(bk[2].numpy(), g[2][:, k, :].ravel().copy()), k
+    # the dot-product test on the multi-level tape
+    dx_h, dx = _rand_state(cm, rng)
+    dxN = cm.state_arrays()
+    cm.tlm_forward(dt, nsteps, x0, dx, dx1=dxN)
+    cm.adj_forward(dt, nsteps, x0)
+    xb2 = cm.state_arrays()
+    cm.adj_backward(y, xb2)
+    cm.sync()
+    lhs = _dot(_get(cm, dxN), yb)
+    rhs = _dot(dx_h, _get(cm, xb2))
+    assert abs(lhs - rhs) / abs(lhs) < DOT_TOL
+    c1.close()
+    cm.close()
