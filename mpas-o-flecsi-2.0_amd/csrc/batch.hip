@@ -111,42 +111,58 @@ __device__ __forceinline__ int batch_block(const BatchArgs &A) {
   const int item = t.y * int(blockDim.x) + int(threadIdx.x)
 
 // MODE: the groups' layout (1: levels on lanes, 2: the same with one
-// entity per wave); the groups of each layout have their own launch.
+// entity per wave); the groups of each layout have their own launch.  The
+// group's descriptors are copied to locals first: read through the pointer
+// they would be reloaded after every store the bodies make (the stores
+// could alias them).
 template <int WE, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_edges<WE, MODE>(g.M, g.S[s], item);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  stage_edges<WE, MODE>(M, S, item);
 }
 template <int NT, int WA, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_ho<NT, WA, MODE>(g.M, g.S[s], item);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  stage_ho<NT, WA, MODE>(M, S, item);
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_cells<NT, W, MODE>(g.M, g.S[s], item);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  stage_cells<NT, W, MODE>(M, S, item);
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
   BATCH_PROLOGUE;
-  stage_tracer<NT, W, MODE>(g.M, g.S[s], item);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  stage_tracer<NT, W, MODE>(M, S, item);
 }
 template <int MODE>
 __device__ __forceinline__ void diag_vertex_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
-  const int v = split_levels<MODE, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
-  if (v < g.M.nV)
-    diag_vertex(g.M, v, g.S[s].yh, g.S[s].yu, nullptr, nullptr, g.vort_y[s], g.pv_vertex, k0, k1);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  double *const vort = g.vort_y[s], *const pvv = g.pv_vertex;
+  const int v = split_levels<MODE, 1>(S, M.nL, item, sub, k0, k1);
+  if (v < M.nV) diag_vertex(M, v, S.yh, S.yu, nullptr, nullptr, vort, pvv, k0, k1);
 }
 template <int MODE>
 __device__ __forceinline__ void diag_edge_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
-  const int e = split_levels<MODE, 1>(g.S[s], g.M.nL, item, sub, k0, k1);
-  if (e < g.M.nE)
-    diag_edge(g.M, e, g.S[s].yh, g.pv_vertex, g.h_edge_y[s], g.pv_edge_y[s], k0, k1);
+  const DMesh M = g.M;
+  const StageIO S = g.S[s];
+  const double *const pvv = g.pv_vertex;
+  double *const he = g.h_edge_y[s], *const pve = g.pv_edge_y[s];
+  const int e = split_levels<MODE, 1>(S, M.nL, item, sub, k0, k1);
+  if (e < M.nE) diag_edge(M, e, S.yh, pvv, he, pve, k0, k1);
 }
 template <int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_vertex(BatchArgs A, int s) {

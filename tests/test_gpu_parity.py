@@ -180,6 +180,10 @@ FUSED_CASES = {
     "sw_centred2": (3, 1, 2, 1, 0.0, 1e3, False, 600.0, 4),
     "sw_fct_2levels": (3, 2, 2, 2, 1e5, 0.0, False, 600.0, 3),
     "case1_frozen": (3, 1, 1, 1, 0.0, 0.0, True, 40.0, 5),
+    # one level per lane with many levels (the layout of multi-level contexts)
+    "sw_fct_8levels": (3, 8, 2, 2, 1e5, 1e3, False, 600.0, 3),
+    # the plain CPU-parity run: level 4, dt 960 s, viscosity 1e5, 100 steps
+    "plain_L4_100": (4, 1, 0, 0, 1e5, 0.0, False, 960.0, 100),
     # the single-instance workload's configuration on a level-4 mesh, run
     # for the 1,000 steps the accuracy bound (1e-9 after 1,000 steps) names
     "workload_L4_1000": (4, 1, 4, 2, 1e5, 1e3, False, 240.0, 1000),
