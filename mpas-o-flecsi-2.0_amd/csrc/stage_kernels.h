@@ -258,8 +258,10 @@ __device__ __forceinline__ void tracer_update(const StageIO &S, uint32_t ol, uin
     st(S.ytr, ot, (h0 * t0 + S.w_sub * te) / hy);
     stn<NT>(S.ntr, ot, t0 * h0 + S.w_rk * te);
   } else if (S.stage < 3) {
-    st(S.ytr, ot, (ldo<NT>(S.h0, ol) * ldo<NT>(S.tr0, ot) + S.w_sub * te) / hy);
-    stn<NT>(S.ntr, ot, ldo<NT>(S.ntr, ot) + S.w_rk * te);
+    // every load before the first store (the stores could alias them)
+    const double h0 = ldo<NT>(S.h0, ol), t0 = ldo<NT>(S.tr0, ot), q = ldo<NT>(S.ntr, ot);
+    st(S.ytr, ot, (h0 * t0 + S.w_sub * te) / hy);
+    stn<NT>(S.ntr, ot, q + S.w_rk * te);
   } else {
     st(S.ytr, ot, (ldo<NT>(S.ntr, ot) + S.w_rk * te) / hy);
   }
@@ -313,6 +315,11 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
+    // the non-tracer outputs are stored after the tracer part's loads: the
+    // stores could alias them and would hold the loads back
+    double ke_v = 0.0, dvg_v = 0.0, nh_v = 0.0;
+    const bool w_ke = lead && !S.ocean;
+    bool w_nh = false;
     if (lead && !S.ocean) {
       // ke (solve.cc:63-77) and divergence of the new state
       double yu[W], dc[W];
@@ -329,8 +336,8 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
         ke = (j < n) ? kx : ke;
         dvg = (j < n) ? dx : dvg;
       }
-      st(S.ke_y, ol, ke / ar);
-      if (S.use_visc) st(S.div_y, ol, dvg / ar);
+      ke_v = ke / ar;
+      dvg_v = dvg / ar;
     }
     double xu[W], he[W];
 #pragma unroll
@@ -354,14 +361,15 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       if (S.stage == 0) {
         const double h0 = ldo<NTS>(S.h0, ol);
         hy = h0 + S.w_sub * th;
-        if (lead) stn<NTS>(S.nh, ol, h0 + S.w_rk * th);
+        nh_v = h0 + S.w_rk * th;
+        w_nh = lead;
       } else if (S.stage < 3) {
         hy = ldo<NTS>(S.h0, ol) + S.w_sub * th;
-        if (lead) stn<NTS>(S.nh, ol, ldo<NTS>(S.nh, ol) + S.w_rk * th);
+        nh_v = ldo<NTS>(S.nh, ol) + S.w_rk * th;
+        w_nh = lead;
       } else {
         hy = ldo<NTS>(S.nh, ol) + S.w_rk * th;
       }
-      if (lead) st(S.yh, ol, hy);
     }
 
     if constexpr (NT > 0) {
@@ -466,6 +474,12 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
         st_group<TPL>(S.tlo, ot, tl);
       }
     }
+    if (w_ke) {
+      st(S.ke_y, ol, ke_v);
+      if (S.use_visc) st(S.div_y, ol, dvg_v);
+    }
+    if (w_nh) stn<LOL == 0 && kNtNext>(S.nh, ol, nh_v);
+    if (lead && !S.ocean) st(S.yh, ol, hy);
   }
 }
 
