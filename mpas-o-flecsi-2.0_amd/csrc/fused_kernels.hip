@@ -68,11 +68,12 @@ __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_e
                                                                                StageIO S) {
   stage_edges<WE, LOL>(M, S, xcd_item());
 }
-template <int NT, int WA, int LOL>
-__global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
-  stage_ho<NT, WA, LOL>(M, S, xcd_item());
-}
-// Tracers per lane of the cell kernels (0: pairs when the count is even).
+// Tracers per lane of the high-order flux and cell kernels (0: pairs when
+// the count is even).  The high-order flux takes four where the count
+// allows: each lane streams its edge's stencil once for all of them.
+#ifndef MPAS_TPL_HO
+#define MPAS_TPL_HO 4
+#endif
 #ifndef MPAS_TPL_CELLS
 #define MPAS_TPL_CELLS 0
 #endif
@@ -86,6 +87,10 @@ constexpr int lanes_per_cell() {
 template <int NT>
 constexpr int tpl_of(int x) {
   return x && NT % x == 0 ? x : 0;
+}
+template <int NT, int WA, int LOL>
+__global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
+  stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO)>(M, S, xcd_item());
 }
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
@@ -211,7 +216,6 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
                    double *pv_edge_y) {
   const StageIO S = LOL ? with_layout(S0, M) : S0;
   const int64_t lv = LOL ? M.nL : 1;  // lanes per entity and tracer lane group
-  constexpr int NPL = NT > 0 ? NT / tracers_per_lane<NT>() : 1;  // lanes per entity
   const bool fct = NT > 0 && S.scheme == MPAS_TRACER_FCT;
 #define MPAS_STEP(expr) \
   do {                  \
@@ -227,7 +231,8 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
-      hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>), dim3(xcd_grid(M.nE * lv * NPL, BS_HO)),
+      hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>),
+                         dim3(xcd_grid(M.nE * lv * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
                          dim3(BS_HO), 0, st, M, S);
       MPAS_STEP(after_launch("k_stage_ho"));
       MPAS_STEP(prof_end(ctx));

@@ -16,14 +16,17 @@ __device__ __forceinline__ void load_tr(const double *__restrict__ p, uint32_t i
 }
 
 // TPL consecutive tracers of one entity: one 16-byte load / store per pair
-// (the offset i is even whenever TPL == 2: NT is even then).
+// (the offset i is even whenever TPL is: NT is a multiple of TPL then).
 template <int TPL>
 __device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t i,
                                          double (&v)[TPL]) {
-  if constexpr (TPL == 2) {
-    const double2 x = ld(reinterpret_cast<const double2 *>(p), i / 2);
-    v[0] = x.x;
-    v[1] = x.y;
+  if constexpr (TPL % 2 == 0) {
+#pragma unroll
+    for (int g = 0; g < TPL; g += 2) {
+      const double2 x = ld(reinterpret_cast<const double2 *>(p), (i + g) / 2);
+      v[g] = x.x;
+      v[g + 1] = x.y;
+    }
   } else {
 #pragma unroll
     for (int g = 0; g < TPL; g++) v[g] = ld(p, i + g);
@@ -31,8 +34,10 @@ __device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t 
 }
 template <int TPL>
 __device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v)[TPL]) {
-  if constexpr (TPL == 2) {
-    st(reinterpret_cast<double2 *>(p), i / 2, make_double2(v[0], v[1]));
+  if constexpr (TPL % 2 == 0) {
+#pragma unroll
+    for (int g = 0; g < TPL; g += 2)
+      st(reinterpret_cast<double2 *>(p), (i + g) / 2, make_double2(v[g], v[g + 1]));
   } else {
 #pragma unroll
     for (int g = 0; g < TPL; g++) st(p, i + g, v[g]);
@@ -189,9 +194,9 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // calc_normal_thickness_flux, the high-order flux and its correction by the
 // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
 // lane.
-template <int NT, int WA, int LOL = 0>
+template <int NT, int WA, int LOL = 0, int TPLX = 0>
 __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
-  constexpr int TPL = tracers_per_lane<NT>(), NPL = NT / TPL;
+  constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
   uint32_t k0, k1;
   const int e = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
