@@ -551,6 +551,20 @@ int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers, const 
     B->idx_in_grp[m] = int(B->groups[q].members.size());
     B->groups[q].members.push_back(m);
   }
+  // the kernels address each group array with 32-bit byte offsets; the
+  // largest are the antidiffusive fluxes (edges x members x tracers), the
+  // limiter factors (2 x cells x members x tracers) and the edge fields
+  for (const GroupInfo &G : B->groups) {
+    const DMesh &M = B->meshes[G.ctx]->dm;
+    const size_t m = G.members.size(), nt = size_t(std::max(B->nT, 1));
+    const size_t largest = std::max({size_t(M.nE) * m * nt, 2 * size_t(M.nC) * m * nt,
+                                     size_t(M.nE) * m});
+    if (largest > size_t(UINT32_MAX) / sizeof(double))
+      MPAS_FAIL(MPAS_ESHAPE, "mpas_batch_create: " + std::to_string(m) +
+                                 " members of one mesh and time step make a " +
+                                 std::to_string(largest * sizeof(double)) +
+                                 "-byte array; the kernels address at most 4 GiB per array");
+  }
   Carver sizer{nullptr};
   for (GroupInfo &G : B->groups) {
     GroupBufs dummy;

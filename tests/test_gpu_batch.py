@@ -135,3 +135,10 @@ def test_batch_error_paths():
     b = Batch([d], [0], [600.0])  # centred tracers batch as well
     b.close()
     d.close()
+    # 2,200 members of 8 tracers on level 5: the antidiffusive flux array
+    # (edges x members x tracers) would pass the 4 GiB the kernels address
+    big = Context(mesh(5), nTracers=8, tracer_scheme=TRACER_FCT)
+    with pytest.raises(MpasError) as e:
+        Batch([big], [0] * 2200, [600.0] * 2200)
+    assert "4 GiB" in str(e.value)
+    big.close()
