@@ -412,7 +412,8 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
               const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
               const double flux = he[j] * turb * rho_scaling;
               const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
-              te = (j < n) ? x : te;
+              // levels below the edge's maxLevelEdgeTop only (ocean/solve.cc:344-363)
+              te = (j < n && int(k) < lvl_edge(M, int(e[j]))) ? x : te;
             }
           }
           tracer_update<LOL == 0 && kNtNext>(S, ol, ot + g, te, hy);
@@ -566,7 +567,8 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
           const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
           const double flux = he[j] * turb * rho[j];
           const double y = te[g] - (first ? -1.0 : 1.0) * flux * invAreaCell1;
-          te[g] = (j < n) ? y : te[g];
+          // levels below the edge's maxLevelEdgeTop only (ocean/solve.cc:344-363)
+          te[g] = (j < n && int(k) < lvl_edge(M, int(e[j]))) ? y : te[g];
         }
       }
     }
