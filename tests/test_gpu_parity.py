@@ -177,6 +177,9 @@ FUSED_CASES = {
     # name: (level, nL, nT, scheme, visc2, kappa2, freeze_u, dt, steps)
     "sw_visc": (4, 1, 0, 0, 1e5, 0.0, False, 960.0, 5),
     "sw_fct4": (3, 1, 4, 2, 1e5, 1e3, False, 600.0, 5),
+    # the most tracers the fused step takes (two lanes of four per edge in
+    # the high-order flux)
+    "sw_fct8": (3, 1, 8, 2, 1e5, 1e3, False, 600.0, 3),
     "sw_centred2": (3, 1, 2, 1, 0.0, 1e3, False, 600.0, 4),
     "sw_fct_2levels": (3, 2, 2, 2, 1e5, 0.0, False, 600.0, 3),
     "case1_frozen": (3, 1, 1, 1, 0.0, 0.0, True, 40.0, 5),
@@ -272,6 +275,40 @@ def test_fused_partial_levels_centred_mixing_bitwise(mesh3):
     o.set("h_s", h_s)
     o.init_diagnostics()
     o.run(4, 600.0)
+    for g, r, nm in zip(got, (o.get("h"), o.get("u"), o.get("tracers")), ("h", "u", "tr")):
+        assert bitwise_equal(g, r), (nm, max_rel(g, r))
+    ctx.close()
+
+
+@pytest.mark.parametrize("nL", [1, 2])
+def test_fused_fct_boundary_mask_bitwise(mesh3, nL):
+    """boundaryCell switches the third-order flux off next to boundaries
+    (highOrderAdvMask, init_functions.cc:330-447): the fused FCT step with a
+    random mask, one and two levels, against the oracle bit for bit."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    from oracle.oracle import OracleSW
+    m, nT = mesh3, 2
+    rng = np.random.default_rng(31 + nL)
+    bnd = (rng.random(m.nCells * nL) < 0.15).astype(np.int32)
+    u, h, tr = sw_workload_state(m, nT, seed=6)
+    if nL > 1:
+        h = np.stack([h * (1 + 0.01 * k) for k in range(nL)], axis=1)
+        u = np.stack([u * (1 - 0.1 * k) for k in range(nL)], axis=1)
+        tr = np.stack([tr] * nL, axis=1)
+    h_s = np.full(m.nCells, 4500.0)
+    ctx = Context(m, nLevels=nL, nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3,
+                  boundaryCell=bnd)
+    ctx.sw_upload(h, u, tr, h_s)
+    ctx.rk4_step(600.0, 3)
+    got = ctx.sw_download()
+    o = OracleSW(_omesh(m, nL=nL, boundaryCell=bnd), nL=nL, nT=nT, tracer_adv=TRACER_FCT,
+                 visc2=1e5, kappa2=1e3)
+    o.set("h", h)
+    o.set("u", u)
+    o.set("tracers", tr)
+    o.set("h_s", h_s)
+    o.init_diagnostics()
+    o.run(3, 600.0)
     for g, r, nm in zip(got, (o.get("h"), o.get("u"), o.get("tracers")), ("h", "u", "tr")):
         assert bitwise_equal(g, r), (nm, max_rel(g, r))
     ctx.close()
