@@ -63,10 +63,15 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 // LOL: 0 for one entity per lane (all its levels), 1 for one level per
 // lane with an entity's levels on adjacent lanes (contexts of more than one
 // level: the accesses of a wave are then contiguous runs across levels).
-template <int WE, int LOL>
+// Levels per lane of the momentum kernel in multi-level contexts with an
+// even level count (pairs: one 16-byte access per level pair).
+#ifndef MPAS_LP_EDGES
+#define MPAS_LP_EDGES 1
+#endif
+template <int WE, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
                                                                                StageIO S) {
-  stage_edges<WE, LOL>(M, S, xcd_item());
+  stage_edges<WE, LOL, LP>(M, S, xcd_item());
 }
 // Tracers per lane of the high-order flux and cell kernels (0: pairs when
 // the count is even).  The high-order flux takes four where the count
@@ -223,8 +228,16 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   } while (0)
   if (!S.ocean) {
     MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
-    hipLaunchKernelGGL((k_stage_edges<WE, LOL>), dim3(xcd_grid(M.nE * lv, BS_EDGES)),
-                       dim3(BS_EDGES), 0, st, M, S);
+    if (LOL && MPAS_LP_EDGES == 2 && M.nL % 2 == 0) {
+      StageIO S2 = S;
+      S2.lpl = 2;
+      S2.nchunk = M.nL / 2;
+      hipLaunchKernelGGL((k_stage_edges<WE, LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_EDGES)),
+                         dim3(BS_EDGES), 0, st, M, S2);
+    } else {
+      hipLaunchKernelGGL((k_stage_edges<WE, LOL>), dim3(xcd_grid(M.nE * lv, BS_EDGES)),
+                         dim3(BS_EDGES), 0, st, M, S);
+    }
     MPAS_STEP(after_launch("k_stage_edges"));
     MPAS_STEP(prof_end(ctx));
   }

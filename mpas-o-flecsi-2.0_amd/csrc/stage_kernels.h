@@ -130,7 +130,19 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-template <int WE, int LOL = 0>
+// LP consecutive levels of one entity from p + i: one 16-byte load / store
+// per level pair (i is even: the level count is, and a lane's first level).
+template <int LP>
+__device__ __forceinline__ void ld_lv(const double *__restrict__ p, uint32_t i,
+                                      double (&v)[LP]) {
+  ld_group<LP>(p, i, v);
+}
+template <int LP>
+__device__ __forceinline__ void st_lv(double *p, uint32_t i, const double (&v)[LP]) {
+  st_group<LP>(p, i, v);
+}
+
+template <int WE, int LOL = 0, int LP = 1>
 __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
   int sub;
   uint32_t k0, k1;
@@ -152,40 +164,73 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
     w[j] = ldn<LOL == 0 && kNtStencil>(M.woe, j * E + e);
   }
   const double dc = ldc(M.dc, e);
-  for (uint32_t k = k0; k < k1; k++) {
+  for (uint32_t k = k0; k < k1; k += LP) {
     const uint32_t o = e * L + k;
-    double uu[WE], pv[WE], he[WE];
+    double uu[WE][LP], pv[WE][LP], he[WE][LP];
 #pragma unroll
     for (int j = 0; j < WE; j++) {
       const uint32_t oj = id[j] * L + k;
-      uu[j] = ld(S.xu, oj);
-      pv[j] = ld(S.pv_edge, oj);
-      he[j] = ld(S.h_edge, oj);
-    }
-    const double pve = ld(S.pv_edge, o);
-    double q = 0;
-#pragma unroll
-    for (int j = 0; j < WE; j++) {
-      const double workpv = 0.5 * (pve + pv[j]);
-      const double qn = q + w[j] * uu[j] * workpv * he[j];
-      q = (j < n) ? qn : q;
+      ld_lv<LP>(S.xu, oj, uu[j]);
+      ld_lv<LP>(S.pv_edge, oj, pv[j]);
+      ld_lv<LP>(S.h_edge, oj, he[j]);
     }
     const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
-    double t = q - (ld(S.ke, o2) - ld(S.ke, o1) +
-                    M.gravity * (ld(S.xh, o2) + ldc(S.h_s, cc.y) - ld(S.xh, o1) -
-                                 ldc(S.h_s, cc.x))) /
-                       dc;
-    if (S.use_visc) t += visc_at(M, e, k, cc, ldc(M.e2v, e), S.div, S.vort, S.visc_coef);
+    double pve[LP], ke1[LP], ke2[LP], xh1[LP], xh2[LP];
+    ld_lv<LP>(S.pv_edge, o, pve);
+    ld_lv<LP>(S.ke, o1, ke1);
+    ld_lv<LP>(S.ke, o2, ke2);
+    ld_lv<LP>(S.xh, o1, xh1);
+    ld_lv<LP>(S.xh, o2, xh2);
+    double visc[LP];
+    if (S.use_visc) {
+      const int2 vv = ldc(M.e2v, e);
+      double d1[LP], d2[LP], v1[LP], v2[LP];
+      ld_lv<LP>(S.div, o1, d1);
+      ld_lv<LP>(S.div, o2, d2);
+      ld_lv<LP>(S.vort, uint32_t(vv.x) * L + k, v1);
+      ld_lv<LP>(S.vort, uint32_t(vv.y) * L + k, v2);
+      const double dv = ldc(M.dv, e);
+#pragma unroll
+      for (int p = 0; p < LP; p++)
+        visc[p] = S.visc_coef * ((d2[p] - d1[p]) / dc - (v2[p] - v1[p]) / dv);  // visc_at
+    }
     constexpr bool NTS = LOL == 0 && kNtNext;
-    if (S.stage == 0) {
-      const double u0 = ldo<NTS>(S.u0, o);
-      st(S.yu, o, u0 + S.w_sub * t);
-      stn<NTS>(S.nu, o, u0 + S.w_rk * t);
-    } else if (S.stage < 3) {
-      st(S.yu, o, ldo<NTS>(S.u0, o) + S.w_sub * t);
-      stn<NTS>(S.nu, o, ldo<NTS>(S.nu, o) + S.w_rk * t);
+    double u0[LP], nu[LP];
+    if constexpr (LP == 1) {
+      if (S.stage < 3) u0[0] = ldo<NTS>(S.u0, o);
+      if (S.stage > 0) nu[0] = ldo<NTS>(S.nu, o);
     } else {
-      st(S.yu, o, ldo<NTS>(S.nu, o) + S.w_rk * t);
+      if (S.stage < 3) ld_lv<LP>(S.u0, o, u0);
+      if (S.stage > 0) ld_lv<LP>(S.nu, o, nu);
+    }
+    double yu[LP], nun[LP];
+#pragma unroll
+    for (int p = 0; p < LP; p++) {
+      double q = 0;
+#pragma unroll
+      for (int j = 0; j < WE; j++) {
+        const double workpv = 0.5 * (pve[p] + pv[j][p]);
+        const double qn = q + w[j] * uu[j][p] * workpv * he[j][p];
+        q = (j < n) ? qn : q;
+      }
+      double t = q - (ke2[p] - ke1[p] +
+                      M.gravity * (xh2[p] + ldc(S.h_s, cc.y) - xh1[p] - ldc(S.h_s, cc.x))) /
+                         dc;
+      if (S.use_visc) t += visc[p];
+      if (S.stage == 0) {
+        yu[p] = u0[p] + S.w_sub * t;
+        nun[p] = u0[p] + S.w_rk * t;
+      } else if (S.stage < 3) {
+        yu[p] = u0[p] + S.w_sub * t;
+        nun[p] = nu[p] + S.w_rk * t;
+      } else {
+        yu[p] = nu[p] + S.w_rk * t;
+      }
+    }
+    st_lv<LP>(S.yu, o, yu);
+    if (S.stage < 3) {
+      if constexpr (LP == 1) stn<NTS>(S.nu, o, nun[0]);
+      else st_lv<LP>(S.nu, o, nun);
     }
   }
 }
@@ -317,6 +362,9 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   for (int j = 0; j < W; j++) e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
 #pragma unroll
   for (int j = 0; j < W; j++) dv[j] = ldc(M.dv, e[j]);
+  int lvE[W];  // maxLevelEdgeTop of the cell's edges (partial columns only)
+#pragma unroll
+  for (int j = 0; j < W; j++) lvE[j] = M.mlet ? ldc(M.mlet, e[j]) : int(L);
 
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
@@ -413,7 +461,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
               const double flux = he[j] * turb * rho_scaling;
               const double x = te - (first ? -1.0 : 1.0) * flux * invArea;
               // levels below the edge's maxLevelEdgeTop only (ocean/solve.cc:344-363)
-              te = (j < n && int(k) < lvl_edge(M, int(e[j]))) ? x : te;
+              te = (j < n && int(k) < lvE[j]) ? x : te;
             }
           }
           tracer_update<LOL == 0 && kNtNext>(S, ol, ot + g, te, hy);
@@ -567,8 +615,9 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
           const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
           const double flux = he[j] * turb * rho[j];
           const double y = te[g] - (first ? -1.0 : 1.0) * flux * invAreaCell1;
-          // levels below the edge's maxLevelEdgeTop only (ocean/solve.cc:344-363)
-          te[g] = (j < n && int(k) < lvl_edge(M, int(e[j]))) ? y : te[g];
+          // every level is below maxLevelEdgeTop here: the fused FCT runs
+          // refuse partial columns (ocean/solve.cc:344-363 stops there)
+          te[g] = (j < n) ? y : te[g];
         }
       }
     }
