@@ -154,6 +154,47 @@ __device__ __forceinline__ void stn(T *p, uint32_t i, T v) {
   st(p, i, v);
 }
 
+// TPL consecutive tracers of one entity: one 16-byte load / store per pair
+// (the offset i is even whenever TPL is: NT is a multiple of TPL then).
+template <int TPL>
+__device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t i,
+                                         double (&v)[TPL]) {
+  if constexpr (TPL % 2 == 0) {
+#pragma unroll
+    for (int g = 0; g < TPL; g += 2) {
+      const double2 x = ld(reinterpret_cast<const double2 *>(p), (i + g) / 2);
+      v[g] = x.x;
+      v[g + 1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) v[g] = ld(p, i + g);
+  }
+}
+template <int TPL>
+__device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v)[TPL]) {
+  if constexpr (TPL % 2 == 0) {
+#pragma unroll
+    for (int g = 0; g < TPL; g += 2)
+      st(reinterpret_cast<double2 *>(p), (i + g) / 2, make_double2(v[g], v[g + 1]));
+  } else {
+#pragma unroll
+    for (int g = 0; g < TPL; g++) st(p, i + g, v[g]);
+  }
+}
+
+// LP consecutive levels of one entity from p + i: one 16-byte load / store
+// per level pair (i is even: the level count is, and a lane's first level).
+template <int LP>
+__device__ __forceinline__ void ld_lv(const double *__restrict__ p, uint32_t i,
+                                      double (&v)[LP]) {
+  ld_group<LP>(p, i, v);
+}
+template <int LP>
+__device__ __forceinline__ void st_lv(double *p, uint32_t i, const double (&v)[LP]) {
+  st_group<LP>(p, i, v);
+}
+
 __device__ __forceinline__ int lvl_cell(const DMesh &M, int c) {
   return M.mlc ? ldc(M.mlc, uint32_t(c)) : M.nL;
 }

@@ -63,10 +63,10 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 // LOL: 0 for one entity per lane (all its levels), 1 for one level per
 // lane with an entity's levels on adjacent lanes (contexts of more than one
 // level: the accesses of a wave are then contiguous runs across levels).
-// Levels per lane of the momentum kernel in multi-level contexts with an
-// even level count (pairs: one 16-byte access per level pair).
-#ifndef MPAS_LP_EDGES
-#define MPAS_LP_EDGES 1
+// Level pairs per lane in multi-level contexts with an even level count
+// (the momentum and diagnostic kernels: one 16-byte access per pair).
+#ifndef MPAS_LEVEL_PAIRS
+#define MPAS_LEVEL_PAIRS 1
 #endif
 template <int WE, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
@@ -93,9 +93,9 @@ template <int NT>
 constexpr int tpl_of(int x) {
   return x && NT % x == 0 ? x : 0;
 }
-template <int NT, int WA, int LOL>
+template <int NT, int WA, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
-  stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO)>(M, S, xcd_item());
+  stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO), LP>(M, S, xcd_item());
 }
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage
   stage_tracer<NT, W, LOL, tpl_of<NT>(MPAS_TPL_TRACER)>(M, S, xcd_item());
 }
 
-template <int LOL>
+template <int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, StageIO S,
                                                          const double *__restrict__ h,
                                                          const double *__restrict__ u,
@@ -116,10 +116,10 @@ __global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, StageIO S,
   int sub;
   uint32_t k0, k1;
   const int v = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
-  if (v < M.nV) diag_vertex(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex, k0, k1);
+  if (v < M.nV) diag_vertex<LP>(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex, k0, k1);
 }
 
-template <int LOL>
+template <int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, StageIO S,
                                                        const double *__restrict__ h,
                                                        const double *__restrict__ pv_vertex,
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, StageIO S,
   int sub;
   uint32_t k0, k1;
   const int e = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
-  if (e < M.nE) diag_edge(M, e, h, pv_vertex, h_edge, pv_edge, k0, k1);
+  if (e < M.nE) diag_edge<LP>(M, e, h, pv_vertex, h_edge, pv_edge, k0, k1);
 }
 
 // The layout of a context's launches and the StageIO fields it reads.
@@ -221,6 +221,11 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
                    double *pv_edge_y) {
   const StageIO S = LOL ? with_layout(S0, M) : S0;
   const int64_t lv = LOL ? M.nL : 1;  // lanes per entity and tracer lane group
+  // level pairs: half the lanes, two levels each
+  const bool pairs = LOL && MPAS_LEVEL_PAIRS && M.nL % 2 == 0;
+  StageIO S2 = S;
+  S2.lpl = 2;
+  S2.nchunk = M.nL / 2;
   const bool fct = NT > 0 && S.scheme == MPAS_TRACER_FCT;
 #define MPAS_STEP(expr) \
   do {                  \
@@ -228,10 +233,7 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   } while (0)
   if (!S.ocean) {
     MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
-    if (LOL && MPAS_LP_EDGES == 2 && M.nL % 2 == 0) {
-      StageIO S2 = S;
-      S2.lpl = 2;
-      S2.nchunk = M.nL / 2;
+    if (pairs) {
       hipLaunchKernelGGL((k_stage_edges<WE, LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_EDGES)),
                          dim3(BS_EDGES), 0, st, M, S2);
     } else {
@@ -244,9 +246,15 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
-      hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>),
-                         dim3(xcd_grid(M.nE * lv * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
-                         dim3(BS_HO), 0, st, M, S);
+      if (pairs)
+        hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL, 2>),
+                           dim3(xcd_grid(M.nE * (lv / 2) * lanes_per_cell<NT, MPAS_TPL_HO>(),
+                                         BS_HO)),
+                           dim3(BS_HO), 0, st, M, S2);
+      else
+        hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>),
+                           dim3(xcd_grid(M.nE * lv * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
+                           dim3(BS_HO), 0, st, M, S);
       MPAS_STEP(after_launch("k_stage_ho"));
       MPAS_STEP(prof_end(ctx));
     }
@@ -270,13 +278,21 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   }
   if (S.ocean) return 0;  // u and h do not change: no diagnostics
   MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
-  hipLaunchKernelGGL(k_diag_vertex<LOL>, dim3(xcd_grid(M.nV * lv, BS_DIAG)), dim3(BS_DIAG), 0,
-                     st, M, S, S.yh, S.yu, vort_y, pv_vertex);
+  if (pairs)
+    hipLaunchKernelGGL((k_diag_vertex<LOL, 2>), dim3(xcd_grid(M.nV * (lv / 2), BS_DIAG)),
+                       dim3(BS_DIAG), 0, st, M, S2, S.yh, S.yu, vort_y, pv_vertex);
+  else
+    hipLaunchKernelGGL((k_diag_vertex<LOL>), dim3(xcd_grid(M.nV * lv, BS_DIAG)), dim3(BS_DIAG),
+                       0, st, M, S, S.yh, S.yu, vort_y, pv_vertex);
   MPAS_STEP(after_launch("k_diag_vertex"));
   MPAS_STEP(prof_end(ctx));
   MPAS_STEP(prof_begin(ctx, "k_diag_edge", b.edge_diag));
-  hipLaunchKernelGGL(k_diag_edge<LOL>, dim3(xcd_grid(M.nE * lv, BS_DIAG)), dim3(BS_DIAG), 0, st,
-                     M, S, S.yh, pv_vertex, h_edge_y, pv_edge_y);
+  if (pairs)
+    hipLaunchKernelGGL((k_diag_edge<LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_DIAG)),
+                       dim3(BS_DIAG), 0, st, M, S2, S.yh, pv_vertex, h_edge_y, pv_edge_y);
+  else
+    hipLaunchKernelGGL((k_diag_edge<LOL>), dim3(xcd_grid(M.nE * lv, BS_DIAG)), dim3(BS_DIAG), 0,
+                       st, M, S, S.yh, pv_vertex, h_edge_y, pv_edge_y);
   MPAS_STEP(after_launch("k_diag_edge"));
   MPAS_STEP(prof_end(ctx));
 #undef MPAS_STEP

@@ -9,7 +9,9 @@ namespace mpasx {
 // 81-92): circulation gathers v1 -= dc*u / v2 += dc*u over the vertex's
 // edges in ascending id; h_vertex pairs the ascending m.cells(v) with the
 // file-order kiteAreasOnVertex (line 86).  circulation / h_vertex may be
-// null (the fused step does not keep them).  Levels [lv0, lv1) (all by default).
+// null (the fused step does not keep them).  Levels [lv0, lv1) (all by
+// default), LP at a time (every load of a group before its stores).
+template <int LP = 1>
 __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double *__restrict__ h,
                                             const double *__restrict__ u, double *h_vertex,
                                             double *circulation, double *vorticity,
@@ -22,27 +24,41 @@ __device__ __forceinline__ void diag_vertex(const DMesh &M, int v, const double 
   const double k0 = ldc(M.kite, v), k1 = ldc(M.kite, V + v), k2 = ldc(M.kite, 2 * V + v);
   const double d0 = ldc(M.dc, e0), d1 = ldc(M.dc, e1), d2 = ldc(M.dc, e2);
   const double at = ldc(M.areaTri, v), f = ldc(M.fV, v);
-  for (uint32_t k = lv0; k < min(lv1, L); k++) {
-    double circ = 0.0;
-    circ = circ + v2e_sign(x0) * (d0 * ld(u, e0 * L + k));
-    circ = circ + v2e_sign(x1) * (d1 * ld(u, e1 * L + k));
-    circ = circ + v2e_sign(x2) * (d2 * ld(u, e2 * L + k));
-    const double vort = circ / at;
-    double hv = 0.0;
-    hv += ld(h, c0 * L + k) * k0;
-    hv += ld(h, c1 * L + k) * k1;
-    hv += ld(h, c2 * L + k) * k2;
-    hv /= at;
+  for (uint32_t k = lv0; k < min(lv1, L); k += LP) {
+    double u0[LP], u1[LP], u2[LP], h0[LP], h1[LP], h2[LP];
+    ld_lv<LP>(u, e0 * L + k, u0);
+    ld_lv<LP>(u, e1 * L + k, u1);
+    ld_lv<LP>(u, e2 * L + k, u2);
+    ld_lv<LP>(h, c0 * L + k, h0);
+    ld_lv<LP>(h, c1 * L + k, h1);
+    ld_lv<LP>(h, c2 * L + k, h2);
+    double circ[LP], vort[LP], hv[LP], pvv[LP];
+#pragma unroll
+    for (int p = 0; p < LP; p++) {
+      double c = 0.0;
+      c = c + v2e_sign(x0) * (d0 * u0[p]);
+      c = c + v2e_sign(x1) * (d1 * u1[p]);
+      c = c + v2e_sign(x2) * (d2 * u2[p]);
+      circ[p] = c;
+      vort[p] = c / at;
+      double s = 0.0;
+      s += h0[p] * k0;
+      s += h1[p] * k1;
+      s += h2[p] * k2;
+      hv[p] = s / at;
+      pvv[p] = (f + vort[p]) / hv[p];
+    }
     const uint32_t o = v * L + k;
-    if (circulation) st(circulation, o, circ);
-    st(vorticity, o, vort);
-    if (h_vertex) st(h_vertex, o, hv);
-    st(pv_vertex, o, (f + vort) / hv);
+    if (circulation) st_lv<LP>(circulation, o, circ);
+    st_lv<LP>(vorticity, o, vort);
+    if (h_vertex) st_lv<LP>(h_vertex, o, hv);
+    st_lv<LP>(pv_vertex, o, pvv);
   }
 }
 
 // compute_solve_diagnostics, edge part (solve.cc:31-39, 95-106): pv_edge
 // starts at zero and takes half of each vertex value, lower vertex id first.
+template <int LP = 1>
 __device__ __forceinline__ void diag_edge(const DMesh &M, int e, const double *__restrict__ h,
                                           const double *__restrict__ pv_vertex,
                                           double *h_edge, double *pv_edge, uint32_t lv0 = 0,
@@ -51,13 +67,24 @@ __device__ __forceinline__ void diag_edge(const DMesh &M, int e, const double *_
   const int2 cc = ldc(M.e2c, e);
   const int2 vv = ldc(M.e2v, e);
   const uint32_t va = min(vv.x, vv.y), vb = max(vv.x, vv.y);
-  for (uint32_t k = lv0; k < min(lv1, L); k++) {
+  for (uint32_t k = lv0; k < min(lv1, L); k += LP) {
     const uint32_t o = e * L + k;
-    st(h_edge, o, 0.5 * (ld(h, uint32_t(cc.x) * L + k) + ld(h, uint32_t(cc.y) * L + k)));
-    double p = 0.0;
-    p += 0.5 * ld(pv_vertex, va * L + k);
-    p += 0.5 * ld(pv_vertex, vb * L + k);
-    st(pv_edge, o, p);
+    double h1[LP], h2[LP], pa[LP], pb[LP];
+    ld_lv<LP>(h, uint32_t(cc.x) * L + k, h1);
+    ld_lv<LP>(h, uint32_t(cc.y) * L + k, h2);
+    ld_lv<LP>(pv_vertex, va * L + k, pa);
+    ld_lv<LP>(pv_vertex, vb * L + k, pb);
+    double he[LP], pe[LP];
+#pragma unroll
+    for (int p = 0; p < LP; p++) {
+      he[p] = 0.5 * (h1[p] + h2[p]);
+      double q = 0.0;
+      q += 0.5 * pa[p];
+      q += 0.5 * pb[p];
+      pe[p] = q;
+    }
+    st_lv<LP>(h_edge, o, he);
+    st_lv<LP>(pv_edge, o, pe);
   }
 }
 

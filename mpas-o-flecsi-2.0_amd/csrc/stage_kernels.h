@@ -15,35 +15,6 @@ __device__ __forceinline__ void load_tr(const double *__restrict__ p, uint32_t i
   for (int t = 0; t < NT; t++) v[t] = ld(p, i0 + t);
 }
 
-// TPL consecutive tracers of one entity: one 16-byte load / store per pair
-// (the offset i is even whenever TPL is: NT is a multiple of TPL then).
-template <int TPL>
-__device__ __forceinline__ void ld_group(const double *__restrict__ p, uint32_t i,
-                                         double (&v)[TPL]) {
-  if constexpr (TPL % 2 == 0) {
-#pragma unroll
-    for (int g = 0; g < TPL; g += 2) {
-      const double2 x = ld(reinterpret_cast<const double2 *>(p), (i + g) / 2);
-      v[g] = x.x;
-      v[g + 1] = x.y;
-    }
-  } else {
-#pragma unroll
-    for (int g = 0; g < TPL; g++) v[g] = ld(p, i + g);
-  }
-}
-template <int TPL>
-__device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v)[TPL]) {
-  if constexpr (TPL % 2 == 0) {
-#pragma unroll
-    for (int g = 0; g < TPL; g += 2)
-      st(reinterpret_cast<double2 *>(p), (i + g) / 2, make_double2(v[g], v[g + 1]));
-  } else {
-#pragma unroll
-    for (int g = 0; g < TPL; g++) st(p, i + g, v[g]);
-  }
-}
-
 // Tracers handled by one lane: pairs when the count is even.
 template <int NT>
 constexpr int tracers_per_lane() {
@@ -130,18 +101,6 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-// LP consecutive levels of one entity from p + i: one 16-byte load / store
-// per level pair (i is even: the level count is, and a lane's first level).
-template <int LP>
-__device__ __forceinline__ void ld_lv(const double *__restrict__ p, uint32_t i,
-                                      double (&v)[LP]) {
-  ld_group<LP>(p, i, v);
-}
-template <int LP>
-__device__ __forceinline__ void st_lv(double *p, uint32_t i, const double (&v)[LP]) {
-  st_group<LP>(p, i, v);
-}
-
 template <int WE, int LOL = 0, int LP = 1>
 __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
   int sub;
@@ -239,7 +198,7 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // calc_normal_thickness_flux, the high-order flux and its correction by the
 // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
 // lane.
-template <int NT, int WA, int LOL = 0, int TPLX = 0>
+template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1>
 __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
@@ -251,50 +210,66 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
   const int2 cc = ldc(M.e2c, e);
   const double dv = ldc(M.dv, e);
   const int na = ldc(M.nAdv, e);
+  const int lve = lvl_edge(M, e);
   uint32_t ic[WA];
   double c1[WA], c3[WA];
+  int lvc[WA];
 #pragma unroll
   for (int i = 0; i < WA; i++) {
     ic[i] = ldn<LOL == 0 && kNtStencil>(M.adv, i * E + e);
     c1[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef, i * E + e);
     c3[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef3, i * E + e) * M.coef3rd;
   }
-  for (uint32_t k = k0; k < k1; k++) {
+#pragma unroll
+  for (int i = 0; i < WA; i++) lvc[i] = lvl_cell(M, ic[i]);
+  for (uint32_t k = k0; k < k1; k += LP) {
     const uint32_t o = e * L + k;
-    const double F = ld(S.xu, o) * ld(S.h_edge, o);
-    const int mask = M.hoMask ? ldc(M.hoMask, o) : 1;
-    const double wgt = F * double(mask);
-    const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
-    double tv[WA][TPL];
+    // every load of the level group first
+    double xu[LP], he[LP], tv[WA][LP][TPL], t1[LP][TPL], t2[LP][TPL];
+    int mask[LP];
+    ld_lv<LP>(S.xu, o, xu);
+    ld_lv<LP>(S.h_edge, o, he);
 #pragma unroll
-    for (int i = 0; i < WA; i++) ld_group<TPL>(S.xtr, (ic[i] * L + k) * NT + t0, tv[i]);
-    double ho[TPL];
+    for (int p = 0; p < LP; p++) {
+      mask[p] = M.hoMask ? ldc(M.hoMask, o + p) : 1;
 #pragma unroll
-    for (int g = 0; g < TPL; g++) ho[g] = 0.0;
+      for (int i = 0; i < WA; i++)
+        ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
+      ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
+      ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k + p) * NT + t0, t2[p]);
+    }
+    double ho[LP][TPL];
 #pragma unroll
-    for (int i = 0; i < WA; i++) {
-      const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
-      const bool live = i < na && int(k) < lvl_cell(M, ic[i]);
+    for (int p = 0; p < LP; p++) {
+      const int kp = int(k) + p;
+      const double F = xu[p] * he[p];
+      const double wgt = F * double(mask[p]);
+      const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
 #pragma unroll
-      for (int g = 0; g < TPL; g++) {
-        const double x = ho[g] + tv[i][g] * weightFactor;
-        ho[g] = live ? x : ho[g];
+      for (int g = 0; g < TPL; g++) ho[p][g] = 0.0;
+#pragma unroll
+      for (int i = 0; i < WA; i++) {
+        const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
+        const bool live = i < na && kp < lvc[i];
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          const double x = ho[p][g] + tv[i][p][g] * weightFactor;
+          ho[p][g] = live ? x : ho[p][g];
+        }
+      }
+      if (kp < lve) {
+        const double tracerWeight = double((mask[p] + 1) & 1) * (dv * 0.5) * F;
+        const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          const double lo = dv * (ntf1 * t1[p][g] + ntf2 * t2[p][g]);
+          ho[p][g] += tracerWeight * (t1[p][g] + t2[p][g]);
+          ho[p][g] -= lo;
+        }
       }
     }
-    if (int(k) < lvl_edge(M, e)) {
-      double t1[TPL], t2[TPL];
-      ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k) * NT + t0, t1);
-      ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k) * NT + t0, t2);
-      const double tracerWeight = double((mask + 1) & 1) * (dv * 0.5) * F;
-      const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
 #pragma unroll
-      for (int g = 0; g < TPL; g++) {
-        const double lo = dv * (ntf1 * t1[g] + ntf2 * t2[g]);
-        ho[g] += tracerWeight * (t1[g] + t2[g]);
-        ho[g] -= lo;
-      }
-    }
-    st_group<TPL>(S.anti, o * NT + t0, ho);
+    for (int p = 0; p < LP; p++) st_group<TPL>(S.anti, (o + p) * NT + t0, ho[p]);
   }
 }
 
