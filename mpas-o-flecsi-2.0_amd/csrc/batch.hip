@@ -87,8 +87,8 @@ struct mpas_batch {
   int xcd = 1;
   // workgroup tables per kernel and layout (index mode - 1); grid 0: no
   // group of that layout
-  int2 *d_tab[mpasx::kBatchKernels][2] = {};
-  int grid[mpasx::kBatchKernels][2] = {};
+  int2 *d_tab[mpasx::kBatchKernels][4] = {};
+  int grid[mpasx::kBatchKernels][4] = {};
   bool tables_ok = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -111,58 +111,72 @@ __device__ __forceinline__ int batch_block(const BatchArgs &A) {
   const int item = t.y * int(blockDim.x) + int(threadIdx.x)
 
 // MODE: the groups' layout (1: levels on lanes, 2: the same with one
-// entity per wave); the groups of each layout have their own launch.  The
+// entity per wave; 3 and 4: 1 and 2 with two levels per lane, for the
+// kernels in level-group form); the groups of each layout have their own
+// launch.  The
 // group's descriptors are copied to locals first: read through the pointer
 // they would be reloaded after every store the bodies make (the stores
 // could alias them).
+constexpr int lol_of(int mode) { return mode > 2 ? mode - 2 : mode; }
+constexpr int lp_of(int mode) { return mode > 2 ? 2 : 1; }
+// The stage descriptor of layout MODE: level pairs halve the chunks.
+template <int MODE>
+__device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
+  StageIO S = g.S[s];
+  if (MODE > 2) {
+    S.lpl = 2;
+    S.nchunk = S.nchunk / 2;
+  }
+  return S;
+}
 template <int WE, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
-  const StageIO S = g.S[s];
-  stage_edges<WE, MODE>(M, S, item);
+  const StageIO S = stage_of<MODE>(g, s);
+  stage_edges<WE, lol_of(MODE), lp_of(MODE)>(M, S, item);
 }
 template <int NT, int WA, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
-  const StageIO S = g.S[s];
-  stage_ho<NT, WA, MODE>(M, S, item);
+  const StageIO S = stage_of<MODE>(g, s);
+  stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE)>(M, S, item);
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = g.S[s];
-  stage_cells<NT, W, MODE>(M, S, item);
+  stage_cells<NT, W, lol_of(MODE)>(M, S, item);  // never launched with pairs
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = g.S[s];
-  stage_tracer<NT, W, MODE>(M, S, item);
+  stage_tracer<NT, W, lol_of(MODE)>(M, S, item);  // never launched with pairs
 }
 template <int MODE>
 __device__ __forceinline__ void diag_vertex_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
   const DMesh M = g.M;
-  const StageIO S = g.S[s];
+  const StageIO S = stage_of<MODE>(g, s);
   double *const vort = g.vort_y[s], *const pvv = g.pv_vertex;
-  const int v = split_levels<MODE, 1>(S, M.nL, item, sub, k0, k1);
-  if (v < M.nV) diag_vertex(M, v, S.yh, S.yu, nullptr, nullptr, vort, pvv, k0, k1);
+  const int v = split_levels<lol_of(MODE), 1>(S, M.nL, item, sub, k0, k1);
+  if (v < M.nV) diag_vertex<lp_of(MODE)>(M, v, S.yh, S.yu, nullptr, nullptr, vort, pvv, k0, k1);
 }
 template <int MODE>
 __device__ __forceinline__ void diag_edge_lanes(const BatchGroup &g, int s, int item) {
   int sub;
   uint32_t k0, k1;
   const DMesh M = g.M;
-  const StageIO S = g.S[s];
+  const StageIO S = stage_of<MODE>(g, s);
   const double *const pvv = g.pv_vertex;
   double *const he = g.h_edge_y[s], *const pve = g.pv_edge_y[s];
-  const int e = split_levels<MODE, 1>(S, M.nL, item, sub, k0, k1);
-  if (e < M.nE) diag_edge(M, e, S.yh, pvv, he, pve, k0, k1);
+  const int e = split_levels<lol_of(MODE), 1>(S, M.nL, item, sub, k0, k1);
+  if (e < M.nE) diag_edge<lp_of(MODE)>(M, e, S.yh, pvv, he, pve, k0, k1);
 }
 template <int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_vertex(BatchArgs A, int s) {
@@ -195,41 +209,58 @@ __global__ __launch_bounds__(256) void kb_member_gather(double *dst, const doubl
 
 int lanes_per_entity(int nT) { return nT > 0 ? nT / (nT % 2 == 0 ? 2 : 1) : 1; }
 
-// Whether group q runs one entity per wave.
-bool group_wave_entity(const mpas_batch *B, int q) {
+// Whether group q runs one entity per wave when each lane takes `per_lane`
+// times the members it takes otherwise: when the entity's lanes fill at
+// least three quarters of their waves.
+bool group_wave_entity(const mpas_batch *B, int q, int per_lane) {
   if (B->wave_entity != 1) return B->wave_entity == 2;
-  const int64_t nch = (int64_t(B->groups[q].members.size()) + B->lpl - 1) / B->lpl;
+  const int64_t nch = (int64_t(B->groups[q].members.size()) + B->lpl - 1) / B->lpl / per_lane;
   const int64_t n = nch * lanes_per_entity(B->nT);
   return 4 * n >= 3 * ((n + 63) / 64 * 64);
 }
 
-// Lanes per entity of kernel k (tracer lane groups npl when it has them):
-// nchunk * npl, padded to a multiple of the wave size with one entity per
-// wave.
-int64_t entity_lanes(const mpas_batch *B, int q, bool tracer_lanes) {
+// The layout of group q in kernel k (0 momentum, 1 high-order flux, 2
+// cells, 3 tracer update, 4 vertex and 5 edge diagnostics): levels on
+// lanes (1) or one entity per wave (2), and for the kernels in level-group
+// form the same with two members per lane (3 and 4: an even member count,
+// one member per lane otherwise).
+int group_mode(const mpas_batch *B, int q, int k) {
+  const bool grouped = k == 0 || k == 1 || k == 4 || k == 5;
+  if (grouped && MPAS_LEVEL_PAIRS && B->lpl == 1 && B->groups[q].members.size() % 2 == 0)
+    return group_wave_entity(B, q, 2) ? 4 : 3;
+  return group_wave_entity(B, q, 1) ? 2 : 1;
+}
+
+// Lanes per entity of group q in kernel k (tracer lane groups npl when it
+// has them): nchunk * npl (half the chunks with level pairs), padded to a
+// multiple of the wave size with one entity per wave.
+int64_t entity_lanes(const mpas_batch *B, int q, int k) {
   const GroupInfo &G = B->groups[q];
+  const bool tracer_lanes = k >= 1 && k <= 3;
   const int64_t npl = tracer_lanes ? lanes_per_entity(B->nT) : 1;
-  const int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
+  int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
+  const int mode = group_mode(B, q, k);
+  if (mode > 2) nch /= 2;
   const int64_t n = nch * npl;
-  return group_wave_entity(B, q) ? (n + 63) / 64 * 64 : n;
+  return mode % 2 == 0 ? (n + 63) / 64 * 64 : n;
 }
 
 // Work items of kernel k for group q.
 int64_t kernel_items(const mpas_batch *B, int k, int q) {
   const DMesh &M = B->meshes[B->groups[q].ctx]->dm;
   switch (k) {
-    case 0: return M.nE * entity_lanes(B, q, false);
-    case 1: return B->nT ? M.nE * entity_lanes(B, q, true) : 0;
-    case 2: return M.nC * entity_lanes(B, q, true);
-    case 3: return B->nT ? M.nC * entity_lanes(B, q, true) : 0;
-    case 4: return M.nV * entity_lanes(B, q, false);
-    default: return M.nE * entity_lanes(B, q, false);
+    case 0: return M.nE * entity_lanes(B, q, k);
+    case 1: return B->nT ? M.nE * entity_lanes(B, q, k) : 0;
+    case 2: return M.nC * entity_lanes(B, q, k);
+    case 3: return B->nT ? M.nC * entity_lanes(B, q, k) : 0;
+    case 4: return M.nV * entity_lanes(B, q, k);
+    default: return M.nE * entity_lanes(B, q, k);
   }
 }
 
 void free_tables(mpas_batch *B) {
   for (int k = 0; k < kBatchKernels; k++)
-    for (int m = 0; m < 2; m++) {
+    for (int m = 0; m < 4; m++) {
       if (B->d_tab[k][m]) (void)hipFree(B->d_tab[k][m]);
       B->d_tab[k][m] = nullptr;
       B->grid[k][m] = 0;
@@ -299,12 +330,11 @@ int build_tables(mpas_batch *B) {
                      hipMemcpyHostToDevice));
   const int ng = int(B->groups.size());
   for (int k = 0; k < kBatchKernels; k++)
-    for (int mode = 1; mode <= 2; mode++) {
+    for (int mode = 1; mode <= 4; mode++) {
       const int bs = B->block[k];
       std::vector<int> nb(ng, 0);
       for (int q = 0; q < ng; q++)
-        if (int(group_wave_entity(B, q)) + 1 == mode)
-          nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
+        if (group_mode(B, q, k) == mode) nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
       std::vector<int2> tab;
       if (B->order == 1) {  // round-robin over the groups' blocks
         for (int j = 0, more = 1; more; j++) {
@@ -344,6 +374,18 @@ BatchArgs args_of(const mpas_batch *B, int k, int mode) {
       constexpr int MODE = 2;                                                             \
       hipLaunchKernelGGL(kern, dim3(B->grid[k][1]), dim3(B->block[k]), 0, st,             \
                          args_of(B, k, 2), s);                                            \
+      if (int rc = after_launch(#kern)) return rc;                                        \
+    }                                                                                     \
+    if (B->grid[k][2]) {                                                                  \
+      constexpr int MODE = 3;                                                             \
+      hipLaunchKernelGGL(kern, dim3(B->grid[k][2]), dim3(B->block[k]), 0, st,             \
+                         args_of(B, k, 3), s);                                            \
+      if (int rc = after_launch(#kern)) return rc;                                        \
+    }                                                                                     \
+    if (B->grid[k][3]) {                                                                  \
+      constexpr int MODE = 4;                                                             \
+      hipLaunchKernelGGL(kern, dim3(B->grid[k][3]), dim3(B->block[k]), 0, st,             \
+                         args_of(B, k, 4), s);                                            \
       if (int rc = after_launch(#kern)) return rc;                                        \
     }                                                                                     \
   } while (0)

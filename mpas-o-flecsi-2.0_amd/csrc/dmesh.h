@@ -183,6 +183,14 @@ __device__ __forceinline__ void st_group(double *p, uint32_t i, const double (&v
   }
 }
 
+// Level pairs per lane where an entity's levels (or ensemble members) lie
+// on adjacent lanes and their count is even: the momentum, high-order flux
+// and diagnostic kernels take two levels per lane, one 16-byte access per
+// pair.
+#ifndef MPAS_LEVEL_PAIRS
+#define MPAS_LEVEL_PAIRS 1
+#endif
+
 // LP consecutive levels of one entity from p + i: one 16-byte load / store
 // per level pair (i is even: the level count is, and a lane's first level).
 template <int LP>

@@ -63,11 +63,6 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 // LOL: 0 for one entity per lane (all its levels), 1 for one level per
 // lane with an entity's levels on adjacent lanes (contexts of more than one
 // level: the accesses of a wave are then contiguous runs across levels).
-// Level pairs per lane in multi-level contexts with an even level count
-// (the momentum and diagnostic kernels: one 16-byte access per pair).
-#ifndef MPAS_LEVEL_PAIRS
-#define MPAS_LEVEL_PAIRS 1
-#endif
 template <int WE, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
                                                                                StageIO S) {

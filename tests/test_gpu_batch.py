@@ -89,6 +89,26 @@ def test_configurations_keep_results():
     _close(ctxs, b)
 
 
+@pytest.mark.parametrize("nT", [2, 3])
+def test_member_pairs_keep_results(nT):
+    """Groups with an even member count run the momentum, high-order flux and
+    diagnostic kernels two members per lane (levels on lanes, or one entity
+    per wave); odd groups and more members per lane do not."""
+    levels = (3, 4, 3, 4, 4, 3, 4, 4, 5, 3, 5)  # groups of 4, 5 and 2 members
+    dts = tuple({3: 600.0, 4: 300.0, 5: 150.0}[L] for L in levels)
+    ctxs, b, states, mesh_of = _ensemble(levels, nT, dts)
+    ref = _singles(ctxs, states, mesh_of, 3, dts)
+    for lpl, we in ((1, 0), (1, 1), (1, 2), (2, 1)):
+        b.config([256] * 6, lpl, 0, 1, we)
+        _load(b, states)
+        b.step(3)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (nT, lpl, we, i)
+    _close(ctxs, b)
+
+
 def test_autotune_keeps_results_and_state():
     ctxs, b, states, mesh_of = _ensemble()
     ref = _singles(ctxs, states, mesh_of, 3)
