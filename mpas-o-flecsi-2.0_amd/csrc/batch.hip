@@ -143,12 +143,12 @@ __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s
   const StageIO S = stage_of<MODE>(g, s);
   stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE)>(M, S, item);
 }
-template <int NT, int W, int MODE>
+template <int NT, int W, int MODE, int SCH>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = g.S[s];
-  stage_cells<NT, W, lol_of(MODE)>(M, S, item);  // never launched with pairs
+  stage_cells<NT, W, lol_of(MODE), 0, SCH>(M, S, item);  // never launched with pairs
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
@@ -398,7 +398,14 @@ int launch_batch_stage_t(mpas_batch *B, int s) {
   if constexpr (NT > 0) {
     if (fct) BL(1, (kb_stage_ho<NT, WE, MODE>));
   }
-  BL(2, (kb_stage_cells<NT, W, MODE>));
+  // the tracer scheme is fixed per instantiation (one set of registers each)
+  const int sch = B->desc[0].S[0].scheme;
+  if (NT == 0 || sch == MPAS_TRACER_NONE)
+    BL(2, (kb_stage_cells<NT, W, MODE, MPAS_TRACER_NONE>));
+  else if (sch == MPAS_TRACER_CENTRED)
+    BL(2, (kb_stage_cells<NT, W, MODE, MPAS_TRACER_CENTRED>));
+  else
+    BL(2, (kb_stage_cells<NT, W, MODE, MPAS_TRACER_FCT>));
   if constexpr (NT > 0) {
     if (fct) BL(3, (kb_stage_tracer<NT, W, MODE>));
   }

@@ -92,10 +92,13 @@ template <int NT, int WA, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
   stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO), LP>(M, S, xcd_item());
 }
-template <int NT, int W, int LOL>
+// SCH: the tracer scheme, fixed per instantiation so that the level-8 FCT
+// kernel does not carry the centred branch's registers (238 VGPRs with
+// both, 117 with FCT alone: twice the waves per SIMD).
+template <int NT, int W, int LOL, int SCH>
 __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
                                                                                StageIO S) {
-  stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS)>(M, S, xcd_item());
+  stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS), SCH>(M, S, xcd_item());
 }
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
@@ -214,6 +217,8 @@ template <int NT, int W, int WE, int LOL>
 int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO &S0,
                    const Bytes &b, double *vort_y, double *pv_vertex, double *h_edge_y,
                    double *pv_edge_y) {
+  if (!LOL && M.nL != 1)  // the one-level kernels take level 0 only
+    MPAS_FAIL(MPAS_EINVAL, "one-level stage kernels launched for a multi-level context");
   const StageIO S = LOL ? with_layout(S0, M) : S0;
   const int64_t lv = LOL ? M.nL : 1;  // lanes per entity and tracer lane group
   // level pairs: half the lanes, two levels each
@@ -255,9 +260,18 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
     }
   }
   MPAS_STEP(prof_begin(ctx, "k_stage_cells", b.cells));
-  hipLaunchKernelGGL((k_stage_cells<NT, W, LOL>),
-                     dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_CELLS>(), BS_CELLS)),
-                     dim3(BS_CELLS), 0, st, M, S);
+  {
+    const dim3 grid(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_CELLS>(), BS_CELLS));
+    if (NT == 0 || S.scheme == MPAS_TRACER_NONE)
+      hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_NONE>), grid, dim3(BS_CELLS), 0,
+                         st, M, S);
+    else if (S.scheme == MPAS_TRACER_CENTRED)
+      hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_CENTRED>), grid, dim3(BS_CELLS),
+                         0, st, M, S);
+    else
+      hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_FCT>), grid, dim3(BS_CELLS), 0,
+                         st, M, S);
+  }
   MPAS_STEP(after_launch("k_stage_cells"));
   MPAS_STEP(prof_end(ctx));
   if constexpr (NT > 0) {

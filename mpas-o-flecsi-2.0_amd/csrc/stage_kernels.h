@@ -57,9 +57,11 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 // [cell][level][tracer] record per pair); the lanes of an entity are
 // adjacent and the first of them writes the entity's non-tracer outputs.
 
-// Levels of a work item.  The single-run kernels loop over every level in
-// each lane (LOL = 0: the static stencil data of an entity is loaded once
-// and reused for all levels).  With levels on lanes (LOL = 1) a lane takes
+// Levels of a work item.  LOL = 0 serves one-level contexts: a lane takes
+// an entity's one level (k0 = 0, k1 = 1, so the compiler sees a single
+// pass and keeps the 32-bit offset addressing; a level loop would be
+// strength-reduced into 64-bit pointers held in registers).  Contexts of
+// more levels put levels on lanes.  With levels on lanes (LOL = 1) a lane takes
 // S.lpl consecutive levels of one entity, and the lanes of an entity cover
 // its S.nchunk chunks of levels side by side: when the levels are ensemble
 // members (every path kernel treats levels independently), neighbouring
@@ -77,7 +79,7 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
     const int q = item / NPL;
     sub = item - q * NPL;
     k0 = 0;
-    k1 = L;
+    k1 = 1;
     return q;
   } else if constexpr (LOL == 1) {
     const int q = item / NPL;
@@ -316,7 +318,7 @@ __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, doub
 // cell's first lane stores them), and per tracer either the centred
 // tendency with its update or the FCT bounds, upwind tendency and limiter
 // factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
-template <int NT, int W, int LOL = 0, int TPLX = 0>
+template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1>
 __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = NT > 0 ? (TPLX ? TPLX : tracers_per_lane<NT>()) : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
@@ -401,6 +403,9 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
     }
 
     if constexpr (NT > 0) {
+      // the tracer scheme: SCH when the launch fixes it (each scheme then
+      // gets its own registers), else the descriptor's
+      const int scheme = SCH >= 0 ? SCH : S.scheme;
       const double invArea = 1.0 / ar;
       const uint32_t ot = ol * NT + t0;
       double tc[TPL];
@@ -413,7 +418,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
         for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
       }
-      if (S.scheme == MPAS_TRACER_CENTRED) {
+      if (scheme == MPAS_TRACER_CENTRED) {
         // compute_scalar_tend (solve.cc:237-270), ascending edge id, then
         // tracer_hmix_del2_tend (ocean/solve.cc:328-366), file order
 #pragma unroll
@@ -441,7 +446,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
           }
           tracer_update<LOL == 0 && kNtNext>(S, ol, ot + g, te, hy);
         }
-      } else if (S.scheme == MPAS_TRACER_FCT) {
+      } else if (scheme == MPAS_TRACER_FCT) {
         // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
         const double dtInvAreaCell = S.dt / ar;
         const double lt = ld(S.xh, ol);
