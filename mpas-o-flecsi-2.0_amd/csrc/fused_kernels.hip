@@ -55,6 +55,22 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 #ifndef MPAS_WPE_CELLS
 #define MPAS_WPE_CELLS 0
 #endif
+// Resident workgroups per CU of the tracer-update kernel.  Its 68 VGPRs
+// would allow 7 waves per SIMD, but its gathers of the neighbours'
+// limiter factors ran faster with at most 4 workgroups of 256 lanes per CU
+// (83 against 86 us per launch on the level-8 workload, 4 tracers).  A
+// dynamic LDS reservation the kernel never touches sets that limit: the
+// CU's 160 KiB hold no more than WG such reservations.  0: no limit.
+#ifndef MPAS_WG_PER_CU_TRACER
+#define MPAS_WG_PER_CU_TRACER 4
+#endif
+constexpr unsigned kLdsPerCu = 160u * 1024u;
+constexpr unsigned occupancy_lds(int wg) {
+  return wg > 0 ? (kLdsPerCu / unsigned(wg + 1) + kLdsPerCu / unsigned(wg)) / 2 : 0u;
+}
+static_assert(kLdsPerCu / occupancy_lds(4) == 4 && kLdsPerCu / occupancy_lds(5) == 5 &&
+                  kLdsPerCu / occupancy_lds(6) == 6,
+              "occupancy_lds picks WG workgroups per CU");
 #ifndef MPAS_WPE_TRACER
 #define MPAS_WPE_TRACER 0
 #endif
@@ -280,7 +296,7 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
       hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>),
                          dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_TRACER>(),
                                        BS_TRACER)),
-                         dim3(BS_TRACER), 0, st, M, S);
+                         dim3(BS_TRACER), occupancy_lds(MPAS_WG_PER_CU_TRACER), st, M, S);
       MPAS_STEP(after_launch("k_stage_tracer"));
       MPAS_STEP(prof_end(ctx));
     }

@@ -45,6 +45,20 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
   return acc;
 }
 
+// A phase boundary for the instruction scheduler: the loads written before
+// it issue together (index loads, then the gathers they feed) instead of
+// being interleaved with their uses, which would serialise the memory round
+// trips (the one-level kernels are one basic block, and the scheduler
+// otherwise trades that latency for registers).
+#ifndef MPAS_SCHED_PHASES
+#define MPAS_SCHED_PHASES 1
+#endif
+#if MPAS_SCHED_PHASES
+#define MPAS_PHASE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define MPAS_PHASE() ((void)0)
+#endif
+
 // The stage kernels read every slot of a ragged list up front, with widths
 // fixed at compile time (WE edgesOnEdge, WA advection cells, W edges of a
 // cell): all index loads issue together, then all gathers, so a lane waits
@@ -536,6 +550,7 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
     op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
   }
+  MPAS_PHASE();
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k, ot = ol * NT + t0;
     double te[TPL], rs[2 * TPL];
@@ -560,6 +575,7 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
         ro[j][2 * g + 1] = pair[1];
       }
     }
+    MPAS_PHASE();
     // rescale (ocean/solve.cc:270-282) and accumulate (301-316), file order
 #pragma unroll
     for (int j = 0; j < W; j++) {
@@ -587,6 +603,7 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
         he[j] = ld(S.h_edge, e[j] * L + k);
         rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
       }
+      MPAS_PHASE();
 #pragma unroll
       for (int j = 0; j < W; j++) {
         const bool first = (neg >> j) & 1;
