@@ -54,6 +54,7 @@ struct GroupInfo {
 };
 
 constexpr int kBatchKernels = 6;  // edges, ho, cells, tracer, vertex, edge_diag
+constexpr int kBatchModes = 6;    // layouts of a group (see MODE below)
 constexpr int kBatchMaxBlock = 512;
 
 struct BatchArgs {
@@ -87,8 +88,8 @@ struct mpas_batch {
   int xcd = 1;
   // workgroup tables per kernel and layout (index mode - 1); grid 0: no
   // group of that layout
-  int2 *d_tab[mpasx::kBatchKernels][4] = {};
-  int grid[mpasx::kBatchKernels][4] = {};
+  int2 *d_tab[mpasx::kBatchKernels][mpasx::kBatchModes] = {};
+  int grid[mpasx::kBatchKernels][mpasx::kBatchModes] = {};
   bool tables_ok = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -110,20 +111,25 @@ __device__ __forceinline__ int batch_block(const BatchArgs &A) {
   const BatchGroup &g = A.grp[t.x];                                       \
   const int item = t.y * int(blockDim.x) + int(threadIdx.x)
 
-// MODE: the groups' layout (1: levels on lanes, 2: the same with one
-// entity per wave; 3 and 4: 1 and 2 with two levels per lane, for the
-// kernels in level-group form); the groups of each layout have their own
-// launch.  The
+// MODE: the groups' layout, each with its own launch:
+//   1  levels on lanes, one member per lane;
+//   2  the same with one entity per wave;
+//   3, 4  1 and 2 with two members per lane, for the kernels in
+//         level-group form (an even member count);
+//   5, 6  1 and 2 with members_per_lane > 1 (the last chunk may be short).
+// Layouts 1-4 fix the members per lane at compile time (one pass of the
+// level loop, fewer registers); 5 and 6 read it from the descriptor.  The
 // group's descriptors are copied to locals first: read through the pointer
 // they would be reloaded after every store the bodies make (the stores
 // could alias them).
-constexpr int lol_of(int mode) { return mode > 2 ? mode - 2 : mode; }
-constexpr int lp_of(int mode) { return mode > 2 ? 2 : 1; }
+constexpr int lol_of(int mode) { return mode % 2 == 0 ? 2 : 1; }
+constexpr int lp_of(int mode) { return mode == 3 || mode == 4 ? 2 : 1; }
+constexpr int fix_of(int mode) { return mode <= 4 ? lp_of(mode) : 0; }
 // The stage descriptor of layout MODE: level pairs halve the chunks.
 template <int MODE>
 __device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
   StageIO S = g.S[s];
-  if (MODE > 2) {
+  if (lp_of(MODE) == 2) {
     S.lpl = 2;
     S.nchunk = S.nchunk / 2;
   }
@@ -134,28 +140,32 @@ __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, in
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
-  stage_edges<WE, lol_of(MODE), lp_of(MODE)>(M, S, item);
+  stage_edges<WE, lol_of(MODE), lp_of(MODE), fix_of(MODE)>(M, S, item);
 }
 template <int NT, int WA, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
-  stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE)>(M, S, item);
+  stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE), fix_of(MODE)>(M, S, item);
 }
 template <int NT, int W, int MODE, int SCH>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
-  const DMesh M = g.M;
-  const StageIO S = g.S[s];
-  stage_cells<NT, W, lol_of(MODE), 0, SCH>(M, S, item);  // never launched with pairs
+  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
+    BATCH_PROLOGUE;
+    const DMesh M = g.M;
+    const StageIO S = g.S[s];
+    stage_cells<NT, W, lol_of(MODE), 0, SCH, fix_of(MODE)>(M, S, item);
+  }
 }
 template <int NT, int W, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
-  const DMesh M = g.M;
-  const StageIO S = g.S[s];
-  stage_tracer<NT, W, lol_of(MODE)>(M, S, item);  // never launched with pairs
+  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
+    BATCH_PROLOGUE;
+    const DMesh M = g.M;
+    const StageIO S = g.S[s];
+    stage_tracer<NT, W, lol_of(MODE), 0, fix_of(MODE)>(M, S, item);
+  }
 }
 template <int MODE>
 __device__ __forceinline__ void diag_vertex_lanes(const BatchGroup &g, int s, int item) {
@@ -164,7 +174,7 @@ __device__ __forceinline__ void diag_vertex_lanes(const BatchGroup &g, int s, in
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
   double *const vort = g.vort_y[s], *const pvv = g.pv_vertex;
-  const int v = split_levels<lol_of(MODE), 1>(S, M.nL, item, sub, k0, k1);
+  const int v = split_levels<lol_of(MODE), 1, fix_of(MODE)>(S, M.nL, item, sub, k0, k1);
   if (v < M.nV) diag_vertex<lp_of(MODE)>(M, v, S.yh, S.yu, nullptr, nullptr, vort, pvv, k0, k1);
 }
 template <int MODE>
@@ -175,7 +185,7 @@ __device__ __forceinline__ void diag_edge_lanes(const BatchGroup &g, int s, int 
   const StageIO S = stage_of<MODE>(g, s);
   const double *const pvv = g.pv_vertex;
   double *const he = g.h_edge_y[s], *const pve = g.pv_edge_y[s];
-  const int e = split_levels<lol_of(MODE), 1>(S, M.nL, item, sub, k0, k1);
+  const int e = split_levels<lol_of(MODE), 1, fix_of(MODE)>(S, M.nL, item, sub, k0, k1);
   if (e < M.nE) diag_edge<lp_of(MODE)>(M, e, S.yh, pvv, he, pve, k0, k1);
 }
 template <int MODE>
@@ -221,12 +231,14 @@ bool group_wave_entity(const mpas_batch *B, int q, int per_lane) {
 
 // The layout of group q in kernel k (0 momentum, 1 high-order flux, 2
 // cells, 3 tracer update, 4 vertex and 5 edge diagnostics): levels on
-// lanes (1) or one entity per wave (2), and for the kernels in level-group
-// form the same with two members per lane (3 and 4: an even member count,
-// one member per lane otherwise).
+// lanes (odd) or one entity per wave (even); with one member per lane
+// 1 and 2, and for the kernels in level-group form 3 and 4 (two members
+// per lane) when the group's member count is even; with more members per
+// lane 5 and 6.
 int group_mode(const mpas_batch *B, int q, int k) {
+  if (B->lpl > 1) return group_wave_entity(B, q, 1) ? 6 : 5;
   const bool grouped = k == 0 || k == 1 || k == 4 || k == 5;
-  if (grouped && MPAS_LEVEL_PAIRS && B->lpl == 1 && B->groups[q].members.size() % 2 == 0)
+  if (grouped && MPAS_LEVEL_PAIRS && B->groups[q].members.size() % 2 == 0)
     return group_wave_entity(B, q, 2) ? 4 : 3;
   return group_wave_entity(B, q, 1) ? 2 : 1;
 }
@@ -240,7 +252,7 @@ int64_t entity_lanes(const mpas_batch *B, int q, int k) {
   const int64_t npl = tracer_lanes ? lanes_per_entity(B->nT) : 1;
   int64_t nch = (int64_t(G.members.size()) + B->lpl - 1) / B->lpl;
   const int mode = group_mode(B, q, k);
-  if (mode > 2) nch /= 2;
+  if (lp_of(mode) == 2) nch /= 2;
   const int64_t n = nch * npl;
   return mode % 2 == 0 ? (n + 63) / 64 * 64 : n;
 }
@@ -260,7 +272,7 @@ int64_t kernel_items(const mpas_batch *B, int k, int q) {
 
 void free_tables(mpas_batch *B) {
   for (int k = 0; k < kBatchKernels; k++)
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < kBatchModes; m++) {
       if (B->d_tab[k][m]) (void)hipFree(B->d_tab[k][m]);
       B->d_tab[k][m] = nullptr;
       B->grid[k][m] = 0;
@@ -330,7 +342,7 @@ int build_tables(mpas_batch *B) {
                      hipMemcpyHostToDevice));
   const int ng = int(B->groups.size());
   for (int k = 0; k < kBatchKernels; k++)
-    for (int mode = 1; mode <= 4; mode++) {
+    for (int mode = 1; mode <= kBatchModes; mode++) {
       const int bs = B->block[k];
       std::vector<int> nb(ng, 0);
       for (int q = 0; q < ng; q++)
@@ -362,33 +374,19 @@ BatchArgs args_of(const mpas_batch *B, int k, int mode) {
 }
 
 // One phase: the launch of the groups of each layout.
+#define BL1(k, kern, m)                                                                   \
+  if (B->grid[k][m - 1]) {                                                                \
+    constexpr int MODE = m;                                                               \
+    hipLaunchKernelGGL(kern, dim3(B->grid[k][m - 1]), dim3(B->block[k]), 0, st,           \
+                       args_of(B, k, m), s);                                              \
+    if (int rc = after_launch(#kern)) return rc;                                          \
+  }
 #define BL(k, kern)                                                                       \
   do {                                                                                    \
-    if (B->grid[k][0]) {                                                                  \
-      constexpr int MODE = 1;                                                             \
-      hipLaunchKernelGGL(kern, dim3(B->grid[k][0]), dim3(B->block[k]), 0, st,             \
-                         args_of(B, k, 1), s);                                            \
-      if (int rc = after_launch(#kern)) return rc;                                        \
-    }                                                                                     \
-    if (B->grid[k][1]) {                                                                  \
-      constexpr int MODE = 2;                                                             \
-      hipLaunchKernelGGL(kern, dim3(B->grid[k][1]), dim3(B->block[k]), 0, st,             \
-                         args_of(B, k, 2), s);                                            \
-      if (int rc = after_launch(#kern)) return rc;                                        \
-    }                                                                                     \
-    if (B->grid[k][2]) {                                                                  \
-      constexpr int MODE = 3;                                                             \
-      hipLaunchKernelGGL(kern, dim3(B->grid[k][2]), dim3(B->block[k]), 0, st,             \
-                         args_of(B, k, 3), s);                                            \
-      if (int rc = after_launch(#kern)) return rc;                                        \
-    }                                                                                     \
-    if (B->grid[k][3]) {                                                                  \
-      constexpr int MODE = 4;                                                             \
-      hipLaunchKernelGGL(kern, dim3(B->grid[k][3]), dim3(B->block[k]), 0, st,             \
-                         args_of(B, k, 4), s);                                            \
-      if (int rc = after_launch(#kern)) return rc;                                        \
-    }                                                                                     \
+    BL1(k, kern, 1) BL1(k, kern, 2) BL1(k, kern, 3) BL1(k, kern, 4) BL1(k, kern, 5)       \
+    BL1(k, kern, 6)                                                                       \
   } while (0)
+static_assert(kBatchModes == 6, "BL launches modes 1-6");
 
 template <int NT, int W, int WE>
 int launch_batch_stage_t(mpas_batch *B, int s) {
@@ -414,6 +412,7 @@ int launch_batch_stage_t(mpas_batch *B, int s) {
   return 0;
 }
 #undef BL
+#undef BL1
 
 template <int NT>
 int launch_batch_stage_w(mpas_batch *B, int s) {

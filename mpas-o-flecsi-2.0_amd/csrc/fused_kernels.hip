@@ -82,7 +82,7 @@ static_assert(kLdsPerCu / occupancy_lds(4) == 4 && kLdsPerCu / occupancy_lds(5) 
 template <int WE, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges(DMesh M,
                                                                                StageIO S) {
-  stage_edges<WE, LOL, LP>(M, S, xcd_item());
+  stage_edges<WE, LOL, LP, LP>(M, S, xcd_item());
 }
 // Tracers per lane of the high-order flux and cell kernels (0: pairs when
 // the count is even).  The high-order flux takes four where the count
@@ -106,7 +106,7 @@ constexpr int tpl_of(int x) {
 }
 template <int NT, int WA, int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMesh M, StageIO S) {
-  stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO), LP>(M, S, xcd_item());
+  stage_ho<NT, WA, LOL, tpl_of<NT>(MPAS_TPL_HO), LP, LP>(M, S, xcd_item());
 }
 // SCH: the tracer scheme, fixed per instantiation so that the level-8 FCT
 // kernel does not carry the centred branch's registers (238 VGPRs with
@@ -114,12 +114,12 @@ __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho(DMes
 template <int NT, int W, int LOL, int SCH>
 __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells(DMesh M,
                                                                                StageIO S) {
-  stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS), SCH>(M, S, xcd_item());
+  stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS), SCH, 1>(M, S, xcd_item());
 }
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
-  stage_tracer<NT, W, LOL, tpl_of<NT>(MPAS_TPL_TRACER)>(M, S, xcd_item());
+  stage_tracer<NT, W, LOL, tpl_of<NT>(MPAS_TPL_TRACER), 1>(M, S, xcd_item());
 }
 
 template <int LOL, int LP = 1>
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, StageIO S,
                                                          double *vorticity, double *pv_vertex) {
   int sub;
   uint32_t k0, k1;
-  const int v = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
+  const int v = split_levels<LOL, 1, LP>(S, M.nL, xcd_item(), sub, k0, k1);
   if (v < M.nV) diag_vertex<LP>(M, v, h, u, nullptr, nullptr, vorticity, pv_vertex, k0, k1);
 }
 
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BS_DIAG) void k_diag_edge(DMesh M, StageIO S,
                                                        double *h_edge, double *pv_edge) {
   int sub;
   uint32_t k0, k1;
-  const int e = split_levels<LOL, 1>(S, M.nL, xcd_item(), sub, k0, k1);
+  const int e = split_levels<LOL, 1, LP>(S, M.nL, xcd_item(), sub, k0, k1);
   if (e < M.nE) diag_edge<LP>(M, e, h, pv_vertex, h_edge, pv_edge, k0, k1);
 }
 

@@ -85,8 +85,12 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 // multiple of the wave size, so all lanes of a wave share one entity: the
 // entity index is then wave-uniform (taken from the wave's first lane) and
 // every static mesh load becomes a scalar load; padding lanes get no
-// levels.
-template <int LOL, int NPL>
+// levels.  FIX > 0: the launch guarantees S.lpl == FIX (and FIX divides the
+// level count), so a lane's range is one pass of the level loop that the
+// compiler can see (it then keeps the 32-bit offset addressing and fewer
+// registers, as for one level); FIX = 0: S.lpl levels, the last chunk
+// possibly shorter.
+template <int LOL, int NPL, int FIX = 0>
 __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int item, int &sub,
                                             uint32_t &k0, uint32_t &k1) {
   if constexpr (LOL == 0) {
@@ -99,16 +103,26 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
     const int q = item / NPL;
     sub = item - q * NPL;
     const int ent = q / S.nchunk, ch = q - ent * S.nchunk;
-    k0 = uint32_t(ch) * uint32_t(S.lpl);
-    k1 = min(k0 + uint32_t(S.lpl), L);
+    if constexpr (FIX > 0) {
+      k0 = uint32_t(ch) * uint32_t(FIX);
+      k1 = k0 + uint32_t(FIX);
+    } else {
+      k0 = uint32_t(ch) * uint32_t(S.lpl);
+      k1 = min(k0 + uint32_t(S.lpl), L);
+    }
     return ent;
   } else {
     const int pe = (S.nchunk * NPL + 63) / 64 * 64;
     const int ent = __builtin_amdgcn_readfirstlane(item) / pe;
     const int r = item - ent * pe, q = r / NPL;
     sub = r - q * NPL;
-    k0 = uint32_t(q) * uint32_t(S.lpl);
-    k1 = q < S.nchunk ? min(k0 + uint32_t(S.lpl), L) : k0;
+    if constexpr (FIX > 0) {
+      k0 = uint32_t(q) * uint32_t(FIX);
+      k1 = q < S.nchunk ? k0 + uint32_t(FIX) : k0;
+    } else {
+      k0 = uint32_t(q) * uint32_t(S.lpl);
+      k1 = q < S.nchunk ? min(k0 + uint32_t(S.lpl), L) : k0;
+    }
     return ent;
   }
 }
@@ -117,11 +131,11 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-template <int WE, int LOL = 0, int LP = 1>
+template <int WE, int LOL = 0, int LP = 1, int FIX = 0>
 __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
   int sub;
   uint32_t k0, k1;
-  const int e = split_levels<LOL, 1>(S, M.nL, item, sub, k0, k1);
+  const int e = split_levels<LOL, 1, FIX>(S, M.nL, item, sub, k0, k1);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
   if (S.freeze_u) {
@@ -214,12 +228,12 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // calc_normal_thickness_flux, the high-order flux and its correction by the
 // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
 // lane.
-template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1>
+template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1, int FIX = 0>
 __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
   uint32_t k0, k1;
-  const int e = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int e = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
   const int t0 = sub * TPL;
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
@@ -332,13 +346,13 @@ __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, doub
 // cell's first lane stores them), and per tracer either the centred
 // tendency with its update or the FCT bounds, upwind tendency and limiter
 // factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
-template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1>
+template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1, int FIX = 0>
 __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = NT > 0 ? (TPLX ? TPLX : tracers_per_lane<NT>()) : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
   int sub;
   uint32_t k0, k1;
-  const int c = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int c = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
@@ -532,12 +546,12 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 }
 
 // ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
-template <int NT, int W, int LOL = 0, int TPLX = 0>
+template <int NT, int W, int LOL = 0, int TPLX = 0, int FIX = 0>
 __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
   int sub;
   uint32_t k0, k1;
-  const int c = split_levels<LOL, NPL>(S, M.nL, item, sub, k0, k1);
+  const int c = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
