@@ -59,6 +59,12 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 #define MPAS_PHASE() ((void)0)
 #endif
 
+// Slots of the momentum kernel's edgesOnEdge gathers per scheduling phase
+// with levels on lanes (0: all at once).
+#ifndef MPAS_QCH
+#define MPAS_QCH 5
+#endif
+
 // The stage kernels read every slot of a ragged list up front, with widths
 // fixed at compile time (WE edgesOnEdge, WA advection cells, W edges of a
 // cell): all index loads issue together, then all gathers, so a lane waits
@@ -155,14 +161,6 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
   const double dc = ldc(M.dc, e);
   for (uint32_t k = k0; k < k1; k += LP) {
     const uint32_t o = e * L + k;
-    double uu[WE][LP], pv[WE][LP], he[WE][LP];
-#pragma unroll
-    for (int j = 0; j < WE; j++) {
-      const uint32_t oj = id[j] * L + k;
-      ld_lv<LP>(S.xu, oj, uu[j]);
-      ld_lv<LP>(S.pv_edge, oj, pv[j]);
-      ld_lv<LP>(S.h_edge, oj, he[j]);
-    }
     const uint32_t o1 = uint32_t(cc.x) * L + k, o2 = uint32_t(cc.y) * L + k;
     double pve[LP], ke1[LP], ke2[LP], xh1[LP], xh2[LP];
     ld_lv<LP>(S.pv_edge, o, pve);
@@ -192,16 +190,41 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
       if (S.stage < 3) ld_lv<LP>(S.u0, o, u0);
       if (S.stage > 0) ld_lv<LP>(S.nu, o, nu);
     }
+    // q over the edgesOnEdge slots in file order, gathered QCH slots at a
+    // time (a scheduling phase each, so that the gathers of one chunk only
+    // are live at once): with levels on lanes the freed registers buy
+    // occupancy (level 7, 32 levels: 426 -> 409 us), one-level launches
+    // were faster with every gather in flight at once (92 against 95 us)
+    constexpr int QCH = LOL && MPAS_QCH > 0 && MPAS_QCH < WE ? MPAS_QCH : WE;
+    double qs[LP];
+#pragma unroll
+    for (int p = 0; p < LP; p++) qs[p] = 0.0;
+#pragma unroll
+    for (int j0 = 0; j0 < WE; j0 += QCH) {
+      double uu[QCH][LP], pv[QCH][LP], he[QCH][LP];
+#pragma unroll
+      for (int jj = 0; jj < QCH && j0 + jj < WE; jj++) {
+        const uint32_t oj = id[j0 + jj] * L + k;
+        ld_lv<LP>(S.xu, oj, uu[jj]);
+        ld_lv<LP>(S.pv_edge, oj, pv[jj]);
+        ld_lv<LP>(S.h_edge, oj, he[jj]);
+      }
+#pragma unroll
+      for (int p = 0; p < LP; p++) {
+#pragma unroll
+        for (int jj = 0; jj < QCH && j0 + jj < WE; jj++) {
+          const int j = j0 + jj;
+          const double workpv = 0.5 * (pve[p] + pv[jj][p]);
+          const double qn = qs[p] + w[j] * uu[jj][p] * workpv * he[jj][p];
+          qs[p] = (j < n) ? qn : qs[p];
+        }
+      }
+      if (j0 + QCH < WE) MPAS_PHASE();
+    }
     double yu[LP], nun[LP];
 #pragma unroll
     for (int p = 0; p < LP; p++) {
-      double q = 0;
-#pragma unroll
-      for (int j = 0; j < WE; j++) {
-        const double workpv = 0.5 * (pve[p] + pv[j][p]);
-        const double qn = q + w[j] * uu[j][p] * workpv * he[j][p];
-        q = (j < n) ? qn : q;
-      }
+      const double q = qs[p];
       double t = q - (ke2[p] - ke1[p] +
                       M.gravity * (xh2[p] + ldc(S.h_s, cc.y) - xh1[p] - ldc(S.h_s, cc.x))) /
                          dc;
