@@ -64,6 +64,9 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 #ifndef MPAS_WG_PER_CU_TRACER
 #define MPAS_WG_PER_CU_TRACER 4
 #endif
+#ifndef MPAS_WG_PER_CU_TRACER_LANES  // the same with levels on lanes
+#define MPAS_WG_PER_CU_TRACER_LANES MPAS_WG_PER_CU_TRACER
+#endif
 constexpr unsigned kLdsPerCu = 160u * 1024u;
 constexpr unsigned occupancy_lds(int wg) {
   return wg > 0 ? (kLdsPerCu / unsigned(wg + 1) + kLdsPerCu / unsigned(wg)) / 2 : 0u;
@@ -96,6 +99,14 @@ __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_e
 #ifndef MPAS_TPL_TRACER
 #define MPAS_TPL_TRACER 1
 #endif
+// With levels on lanes the tracer update takes pairs: the lanes of a cell
+// then cover its levels x tracer pairs, and one lane per pair halves the
+// cell's index and neighbour-factor gathers (level 7, 32 levels, 2
+// tracers: 346 -> 276 us); one-level launches keep one tracer per lane.
+#ifndef MPAS_TPL_TRACER_LANES
+#define MPAS_TPL_TRACER_LANES 2
+#endif
+constexpr int tpl_tracer(int lol) { return lol ? MPAS_TPL_TRACER_LANES : MPAS_TPL_TRACER; }
 template <int NT, int X>
 constexpr int lanes_per_cell() {
   return NT > 0 ? NT / (X && NT % X == 0 ? X : tracers_per_lane<NT>()) : 1;
@@ -119,7 +130,7 @@ __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_c
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
-  stage_tracer<NT, W, LOL, tpl_of<NT>(MPAS_TPL_TRACER), 1>(M, S, xcd_item());
+  stage_tracer<NT, W, LOL, tpl_of<NT>(tpl_tracer(LOL)), 1>(M, S, xcd_item());
 }
 
 template <int LOL, int LP = 1>
@@ -294,9 +305,11 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
       hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>),
-                         dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, MPAS_TPL_TRACER>(),
+                         dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, tpl_tracer(LOL)>(),
                                        BS_TRACER)),
-                         dim3(BS_TRACER), occupancy_lds(MPAS_WG_PER_CU_TRACER), st, M, S);
+                         dim3(BS_TRACER),
+                         occupancy_lds(LOL ? MPAS_WG_PER_CU_TRACER_LANES : MPAS_WG_PER_CU_TRACER),
+                         st, M, S);
       MPAS_STEP(after_launch("k_stage_tracer"));
       MPAS_STEP(prof_end(ctx));
     }
