@@ -59,6 +59,34 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 #define MPAS_PHASE() ((void)0)
 #endif
 
+// Static rows shared across an entity's lanes.  With levels on lanes the G
+// lanes of an entity (G = chunks x tracer lane groups) would each load the
+// same W slots of its stencil row: W vector loads per lane for one row.
+// When the G lanes form an aligned block of a wave (G divides 64; they do,
+// since an entity's lanes are adjacent items and workgroups are whole
+// waves) and G >= W, lane r of the block loads slot r only and every lane
+// takes slot j from lane base + j by a cross-lane permute (ds_bpermute,
+// no memory access).  Every lane of the block belongs to the same entity,
+// so a lane that returned early (past the last entity) is never read.
+#ifndef MPAS_SHARE_ROWS
+#define MPAS_SHARE_ROWS 1
+#endif
+template <int W>
+__device__ __forceinline__ bool share_rows(int G) {
+  return MPAS_SHARE_ROWS && G >= W && G <= 64 && (64 % G) == 0;
+}
+// This lane's slot (r, clamped to the row) and the block's first lane.
+__device__ __forceinline__ int row_slot(int item, int G, int W, int &base) {
+  const int r = item % G;
+  base = int(threadIdx.x & 63u) - r;
+  return r < W ? r : W - 1;
+}
+template <int W, class T>
+__device__ __forceinline__ void bcast_row(T (&out)[W], T mine, int base) {
+#pragma unroll
+  for (int j = 0; j < W; j++) out[j] = __shfl(mine, base + j, 64);
+}
+
 // Slots of the momentum kernel's edgesOnEdge gathers per scheduling phase
 // with levels on lanes (0: all at once).
 #ifndef MPAS_QCH
@@ -153,10 +181,17 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
   const int n = ldc(M.nEoE, e);
   uint32_t id[WE];
   double w[WE];
+  if (LOL == 1 && share_rows<WE>(S.nchunk)) {
+    int base;
+    const int j = row_slot(item, S.nchunk, WE, base);
+    bcast_row<WE, uint32_t>(id, uint32_t(ld(M.eoe, j * E + e)), base);
+    bcast_row<WE, double>(w, ld(M.woe, j * E + e), base);
+  } else {
 #pragma unroll
-  for (int j = 0; j < WE; j++) {
-    id[j] = ldn<LOL == 0 && kNtStencil>(M.eoe, j * E + e);
-    w[j] = ldn<LOL == 0 && kNtStencil>(M.woe, j * E + e);
+    for (int j = 0; j < WE; j++) {
+      id[j] = ldn<LOL == 0 && kNtStencil>(M.eoe, j * E + e);
+      w[j] = ldn<LOL == 0 && kNtStencil>(M.woe, j * E + e);
+    }
   }
   const double dc = ldc(M.dc, e);
   for (uint32_t k = k0; k < k1; k += LP) {
@@ -267,11 +302,21 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
   uint32_t ic[WA];
   double c1[WA], c3[WA];
   int lvc[WA];
+  if (LOL == 1 && share_rows<WA>(S.nchunk * NPL)) {
+    int base;
+    const int i = row_slot(item, S.nchunk * NPL, WA, base);
+    bcast_row<WA, uint32_t>(ic, uint32_t(ld(M.adv, i * E + e)), base);
+    bcast_row<WA, double>(c1, ld(M.advCoef, i * E + e), base);
+    bcast_row<WA, double>(c3, ld(M.advCoef3, i * E + e), base);
 #pragma unroll
-  for (int i = 0; i < WA; i++) {
-    ic[i] = ldn<LOL == 0 && kNtStencil>(M.adv, i * E + e);
-    c1[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef, i * E + e);
-    c3[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef3, i * E + e) * M.coef3rd;
+    for (int j = 0; j < WA; j++) c3[j] = c3[j] * M.coef3rd;
+  } else {
+#pragma unroll
+    for (int i = 0; i < WA; i++) {
+      ic[i] = ldn<LOL == 0 && kNtStencil>(M.adv, i * E + e);
+      c1[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef, i * E + e);
+      c3[i] = ldn<LOL == 0 && kNtStencil>(M.advCoef3, i * E + e) * M.coef3rd;
+    }
   }
 #pragma unroll
   for (int i = 0; i < WA; i++) lvc[i] = lvl_cell(M, ic[i]);
@@ -386,10 +431,21 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   const bool lead = t0 == 0;
   uint32_t e[W];
   double dv[W];
+  // with levels on lanes lane r of the cell's block loads slot r's rows
+  const bool shared = LOL == 1 && share_rows<W>(S.nchunk * NPL);
+  int base = 0, js = 0;
+  uint32_t ej = 0;  // this lane's slot of the row (no run-time index into e[])
+  if (shared) {
+    js = row_slot(item, S.nchunk * NPL, W, base);
+    ej = ld(M.c2e, js * C + c);
+    bcast_row<W, uint32_t>(e, ej, base);
+    bcast_row<W, double>(dv, ldc(M.dv, ej), base);
+  } else {
 #pragma unroll
-  for (int j = 0; j < W; j++) e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
+    for (int j = 0; j < W; j++) e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
 #pragma unroll
-  for (int j = 0; j < W; j++) dv[j] = ldc(M.dv, e[j]);
+    for (int j = 0; j < W; j++) dv[j] = ldc(M.dv, e[j]);
+  }
   int lvE[W];  // maxLevelEdgeTop of the cell's edges (partial columns only)
 #pragma unroll
   for (int j = 0; j < W; j++) lvE[j] = M.mlet ? ldc(M.mlet, e[j]) : int(L);
@@ -405,9 +461,12 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       // ke (solve.cc:63-77) and divergence of the new state
       double yu[W], dc[W];
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        yu[j] = ld(S.yu, e[j] * L + k);
-        dc[j] = ldc(M.dc, e[j]);
+      for (int j = 0; j < W; j++) yu[j] = ld(S.yu, e[j] * L + k);
+      if (shared) {
+        bcast_row<W, double>(dc, ldc(M.dc, ej), base);
+      } else {
+#pragma unroll
+        for (int j = 0; j < W; j++) dc[j] = ldc(M.dc, e[j]);
       }
       double ke = 0.0, dvg = 0.0;
 #pragma unroll
@@ -464,8 +523,12 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       double tn[W][TPL];
       {
         uint32_t op[W];
+        if (shared) {
+          bcast_row<W, uint32_t>(op, ld(M.cOpp, js * C + c), base);
+        } else {
 #pragma unroll
-        for (int j = 0; j < W; j++) op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
+          for (int j = 0; j < W; j++) op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
+        }
 #pragma unroll
         for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
       }
@@ -582,10 +645,21 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   const uint8_t neg = ldc(M.sgnNeg, c);
   const double invAreaCell1 = 1.0 / ldc(M.area, c);
   uint32_t e[W], op[W];
+  // with levels on lanes lane r of the cell's block loads slot r's rows
+  const bool shared = LOL == 1 && share_rows<W>(S.nchunk * NPL);
+  int base = 0, js = 0;
+  uint32_t ej = 0;  // this lane's slot of the row (no run-time index into e[])
+  if (shared) {
+    js = row_slot(item, S.nchunk * NPL, W, base);
+    ej = ld(M.c2e, js * C + c);
+    bcast_row<W, uint32_t>(e, ej, base);
+    bcast_row<W, uint32_t>(op, ld(M.cOpp, js * C + c), base);
+  } else {
 #pragma unroll
-  for (int j = 0; j < W; j++) {
-    e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
-    op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
+    for (int j = 0; j < W; j++) {
+      e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
+      op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
+    }
   }
   MPAS_PHASE();
   for (uint32_t k = k0; k < k1; k++) {
@@ -638,7 +712,12 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
       for (int j = 0; j < W; j++) {
         ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
         he[j] = ld(S.h_edge, e[j] * L + k);
-        rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
+      }
+      if (shared) {
+        bcast_row<W, double>(rho, M.msd * S.kappa * ldc(M.dv, ej) / ldc(M.dc, ej), base);
+      } else {
+#pragma unroll
+        for (int j = 0; j < W; j++) rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
       }
       MPAS_PHASE();
 #pragma unroll
