@@ -135,8 +135,11 @@ __device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
   }
   return S;
 }
+// At least 3 waves per SIMD for the momentum kernel: with two members per
+// lane it needs 171 VGPRs, 3 over the 3-wave budget.
 template <int WE, int MODE>
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_edges(BatchArgs A, int s) {
+__global__ __launch_bounds__(kBatchMaxBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void
+kb_stage_edges(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
