@@ -435,7 +435,25 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   const bool shared = LOL == 1 && share_rows<W>(S.nchunk * NPL);
   int base = 0, js = 0;
   uint32_t ej = 0;  // this lane's slot of the row (no run-time index into e[])
-  if (shared) {
+  // one level: the cell's NPL tracer lanes (an aligned group of a wave when
+  // NPL divides 64) load slots sub, sub + NPL, ... and share them likewise
+  constexpr bool GRP = LOL == 0 && NPL > 1 && 64 % NPL == 0 && MPAS_SHARE_ROWS;
+  constexpr int MS = (W + NPL - 1) / NPL;
+  uint32_t me[MS];
+  if constexpr (GRP) {
+    base = int(threadIdx.x & 63u) - sub;
+    double dm[MS];
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      me[m] = ldn<kNtCells>(M.c2e, min(sub + m * NPL, W - 1) * C + c);
+      dm[m] = ldc(M.dv, me[m]);
+    }
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      e[j] = __shfl(me[j / NPL], base + j % NPL, 64);
+      dv[j] = __shfl(dm[j / NPL], base + j % NPL, 64);
+    }
+  } else if (shared) {
     js = row_slot(item, S.nchunk * NPL, W, base);
     ej = ld(M.c2e, js * C + c);
     bcast_row<W, uint32_t>(e, ej, base);
@@ -480,10 +498,24 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       dvg_v = dvg / ar;
     }
     double xu[W], he[W];
+    if constexpr (GRP) {
+      double xm[MS], hm[MS];
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-      xu[j] = ld(S.xu, e[j] * L + k);
-      he[j] = ld(S.h_edge, e[j] * L + k);
+      for (int m = 0; m < MS; m++) {
+        xm[m] = ld(S.xu, me[m] * L + k);
+        hm[m] = ld(S.h_edge, me[m] * L + k);
+      }
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        xu[j] = __shfl(xm[j / NPL], base + j % NPL, 64);
+        he[j] = __shfl(hm[j / NPL], base + j % NPL, 64);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        xu[j] = ld(S.xu, e[j] * L + k);
+        he[j] = ld(S.h_edge, e[j] * L + k);
+      }
     }
     // compute_tend, thickness part: edges in ascending id (solve.cc:191-205);
     // the tracer-only (ocean) run keeps the thickness fixed
@@ -523,7 +555,14 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       double tn[W][TPL];
       {
         uint32_t op[W];
-        if (shared) {
+        if constexpr (GRP) {
+          uint32_t om[MS];
+#pragma unroll
+          for (int m = 0; m < MS; m++)
+            om[m] = ldn<kNtCells>(M.cOpp, min(sub + m * NPL, W - 1) * C + c);
+#pragma unroll
+          for (int j = 0; j < W; j++) op[j] = __shfl(om[j / NPL], base + j % NPL, 64);
+        } else if (shared) {
           bcast_row<W, uint32_t>(op, ld(M.cOpp, js * C + c), base);
         } else {
 #pragma unroll
@@ -649,7 +688,26 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   const bool shared = LOL == 1 && share_rows<W>(S.nchunk * NPL);
   int base = 0, js = 0;
   uint32_t ej = 0;  // this lane's slot of the row (no run-time index into e[])
-  if (shared) {
+  // one level: the cell's NPL tracer lanes (an aligned group of a wave when
+  // NPL divides 64) load slots sub, sub + NPL, ... and share them likewise
+  constexpr bool GRP = LOL == 0 && NPL > 1 && 64 % NPL == 0 && MPAS_SHARE_ROWS;
+  constexpr int MS = (W + NPL - 1) / NPL;
+  uint32_t me[MS];
+  if constexpr (GRP) {
+    base = int(threadIdx.x & 63u) - sub;
+    uint32_t mo[MS];
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      const int j = min(sub + m * NPL, W - 1);
+      me[m] = ldn<kNtCells>(M.c2e, j * C + c);
+      mo[m] = ldn<kNtCells>(M.cOpp, j * C + c);
+    }
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      e[j] = __shfl(me[j / NPL], base + j % NPL, 64);
+      op[j] = __shfl(mo[j / NPL], base + j % NPL, 64);
+    }
+  } else if (shared) {
     js = row_slot(item, S.nchunk * NPL, W, base);
     ej = ld(M.c2e, js * C + c);
     bcast_row<W, uint32_t>(e, ej, base);
@@ -708,16 +766,34 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
       // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
       double tc[TPL], tn[W][TPL], he[W], rho[W];
       ld_group<TPL>(S.xtr, ot, tc);
+      if constexpr (GRP) {
+        double hm[MS], rm[MS];
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
-        he[j] = ld(S.h_edge, e[j] * L + k);
-      }
-      if (shared) {
-        bcast_row<W, double>(rho, M.msd * S.kappa * ldc(M.dv, ej) / ldc(M.dc, ej), base);
+        for (int m = 0; m < MS; m++) {
+          hm[m] = ld(S.h_edge, me[m] * L + k);
+          rm[m] = M.msd * S.kappa * ldc(M.dv, me[m]) / ldc(M.dc, me[m]);
+        }
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+          he[j] = __shfl(hm[j / NPL], base + j % NPL, 64);
+          rho[j] = __shfl(rm[j / NPL], base + j % NPL, 64);
+        }
       } else {
 #pragma unroll
-        for (int j = 0; j < W; j++) rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
+        for (int j = 0; j < W; j++) {
+          ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+          he[j] = ld(S.h_edge, e[j] * L + k);
+        }
+      }
+      if constexpr (!GRP) {
+        if (shared) {
+          bcast_row<W, double>(rho, M.msd * S.kappa * ldc(M.dv, ej) / ldc(M.dc, ej), base);
+        } else {
+#pragma unroll
+          for (int j = 0; j < W; j++)
+            rho[j] = M.msd * S.kappa * ldc(M.dv, e[j]) / ldc(M.dc, e[j]);
+        }
       }
       MPAS_PHASE();
 #pragma unroll
