@@ -109,6 +109,26 @@ def test_member_pairs_keep_results(nT):
     _close(ctxs, b)
 
 
+def test_shared_stencil_rows_keep_results():
+    """A group of 32 members: with two members per lane an edge's members
+    fill 16 lanes and a cell's 32, so the kernels load each stencil row once
+    per lane block and share it across lanes; every member still equals its
+    single run."""
+    levels = (3,) * 32 + (4,) * 3
+    dts = tuple(600.0 if L == 3 else 300.0 for L in levels)
+    ctxs, b, states, mesh_of = _ensemble(levels, 2, dts)
+    ref = _singles(ctxs, states, mesh_of, 2, dts)
+    for we in (0, 1):
+        b.config([256] * 6, 1, 0, 1, we)
+        _load(b, states)
+        b.step(2)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (we, i)
+    _close(ctxs, b)
+
+
 def test_autotune_keeps_results_and_state():
     ctxs, b, states, mesh_of = _ensemble()
     ref = _singles(ctxs, states, mesh_of, 3)

@@ -185,6 +185,9 @@ FUSED_CASES = {
     "case1_frozen": (3, 1, 1, 1, 0.0, 0.0, True, 40.0, 5),
     # one level per lane with many levels (the layout of multi-level contexts)
     "sw_fct_8levels": (3, 8, 2, 2, 1e5, 1e3, False, 600.0, 3),
+    # 32 levels: level pairs put an edge's 32 levels on 16 lanes, enough for
+    # every stencil row to be loaded once per lane block and shared
+    "sw_fct_32levels": (3, 32, 2, 2, 1e5, 1e3, False, 600.0, 2),
     # the plain CPU-parity run: level 4, dt 960 s, viscosity 1e5, 100 steps
     "plain_L4_100": (4, 1, 0, 0, 1e5, 0.0, False, 960.0, 100),
     # the single-instance workload's configuration on a level-4 mesh, run
@@ -318,7 +321,7 @@ def test_fused_wide_kernels_bitwise(monkeypatch):
     """The 8 / 16-wide stage kernels (cells with more than 6 edges) give the
     same bits as the oracle on the same workloads."""
     monkeypatch.setenv("MPAS_HIP_WIDE_KERNELS", "1")
-    for name in ("sw_fct4", "sw_centred2"):
+    for name in ("sw_fct4", "sw_centred2", "sw_fct_32levels"):
         got, ref, diag = _run_pair(name, steps=2, seed=7)
         for g, r in zip(got, ref):
             assert bitwise_equal(g, r), name
