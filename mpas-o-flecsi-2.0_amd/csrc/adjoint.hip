@@ -410,6 +410,8 @@ __global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Re
 // of the sums), so every load of a chunk is in bounds and issues before the
 // first value is needed, instead of one dependent round trip per slot.
 constexpr int kChunk = 16;
+// the advection stencil has at most 16 slots (maxEdges2): one chunk
+static_assert(kChunk >= 16, "the edge part reads a cell's advection slots in one chunk");
 
 // Edge part of the stage adjoint: from the tendency adjoints to the partial
 // u adjoint (thickness flux, PV flux, tracer flux paths), the h_edge and
@@ -487,6 +489,16 @@ __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams 
       const double rho = M.msd * P.kappa * dv / dc;
       const double F = ue * hee, sgn = fsign(F);
       const double Fp = F > 0.0 ? F : 0.0, Fm = F < 0.0 ? F : 0.0;
+      // the lane's advection slots (at most 16 = kChunk slots: one chunk),
+      // loaded once for every tracer
+      uint32_t ib[Q];
+      double cf[Q];
+#pragma unroll
+      for (int j = 0; j < Q; j++) {
+        const int i = max(min(lane + j * kEdgeLanes, na - 1), 0);
+        ib[j] = uint32_t(ld(M.adv, i * E + e)) * L + k;
+        cf[j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
+      }
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
         const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
@@ -494,22 +506,12 @@ __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams 
         const double lob = (1.0 - lam) * phib, hob = lam * phib;
         const double hf = hob * F;
         double hs = 0.0;
-        for (int i0 = lane; i0 < na; i0 += kChunk) {
-          uint32_t ii[Q];
-          double cf[Q];
 #pragma unroll
-          for (int j = 0; j < Q; j++) {
-            const int i = min(i0 + j * kEdgeLanes, na - 1);
-            ii[j] = (uint32_t(ld(M.adv, i * E + e)) * L + k) * NT + t;
-            cf[j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
-          }
-#pragma unroll
-          for (int j = 0; j < Q; j++) {
-            const int i = i0 + j * kEdgeLanes;
-            if (i < na) {
-              hs += ld(R.tr, ii[j]) * cf[j];
-              st(hbf, ((uint32_t(i) * E + e) * L + k) * NT + t, hf * cf[j]);
-            }
+        for (int j = 0; j < Q; j++) {
+          const int i = lane + j * kEdgeLanes;
+          if (i < na) {
+            hs += ld(R.tr, ib[j] * NT + t) * cf[j];
+            st(hbf, ((uint32_t(i) * E + e) * L + k) * NT + t, hf * cf[j]);
           }
         }
         hs += __shfl_xor(hs, 1);
