@@ -32,6 +32,18 @@ int after_launch(const char *what) {
 
 namespace {
 
+// A slot-major ELL array [j][E] of width pad (even) as slot pairs,
+// [j/2][E][2].
+template <class T>
+std::vector<T> slot_pairs(const std::vector<T> &a, int pad, int E) {
+  std::vector<T> out(a.size());
+  for (int p = 0; p < pad / 2; p++)
+    for (int e = 0; e < E; e++)
+      for (int s = 0; s < 2; s++)
+        out[(size_t(p) * E + e) * 2 + s] = a[size_t(2 * p + s) * E + e];
+  return out;
+}
+
 template <class T>
 int upload(mpas_ctx *ctx, const std::vector<T> &v, const T **out) {
   *out = nullptr;
@@ -101,8 +113,10 @@ int build_device_mesh(mpas_ctx *ctx) {
       n[e] = uint8_t(m.e2e.size(e));
       for (int j = 0; j < m.e2e.size(e); j++) w[size_t(j) * E + e] = m.weightsOnEdge[size_t(e) * 16 + j];
     }
-    if ((rc = upload(ctx, n, &M.nEoE)) || (rc = upload(ctx, to_ell(m.e2e, padE), &M.eoe)) ||
-        (rc = upload(ctx, w, &M.woe)))
+    const auto ids = to_ell(m.e2e, padE);
+    if ((rc = upload(ctx, n, &M.nEoE)) || (rc = upload(ctx, ids, &M.eoe)) ||
+        (rc = upload(ctx, w, &M.woe)) || (rc = upload(ctx, slot_pairs(ids, padE, E), &M.eoeP)) ||
+        (rc = upload(ctx, slot_pairs(w, padE, E), &M.woeP)))
       return rc;
   }
   // cells
@@ -184,7 +198,10 @@ int build_device_mesh(mpas_ctx *ctx) {
       }
     }
     if ((rc = upload(ctx, na, &M.nAdv)) || (rc = upload(ctx, adv, &M.adv)) ||
-        (rc = upload(ctx, c1, &M.advCoef)) || (rc = upload(ctx, c3, &M.advCoef3)))
+        (rc = upload(ctx, c1, &M.advCoef)) || (rc = upload(ctx, c3, &M.advCoef3)) ||
+        (rc = upload(ctx, slot_pairs(adv, padE, E), &M.advP)) ||
+        (rc = upload(ctx, slot_pairs(c1, padE, E), &M.advCoefP)) ||
+        (rc = upload(ctx, slot_pairs(c3, padE, E), &M.advCoef3P)))
       return rc;
     bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
                             [&](int x) { return x == m.nL; });

@@ -30,6 +30,10 @@ struct DMesh {
   const uint8_t *nEoE;
   const int *eoe;     // [j][E] m.edges(e), file order
   const double *woe;  // [j][E] weightsOnEdge
+  // the same two slots at a time, [j/2][E][2] (the one-level kernels read a
+  // slot pair with one access)
+  const int *eoeP;
+  const double *woeP;
   // cells
   const double *area;
   const uint8_t *nEoC;
@@ -49,6 +53,8 @@ struct DMesh {
   const uint8_t *nAdv;
   const int *adv;                       // [j][E] advCellsForEdge
   const double *advCoef, *advCoef3;     // [j][E] advCoef, advCoef_3rd
+  const int *advP;                      // advCellsForEdge, [j/2][E][2]
+  const double *advCoefP, *advCoef3P;   // advCoef, advCoef_3rd, [j/2][E][2]
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
@@ -126,6 +132,20 @@ __device__ __forceinline__ T ldn(const T *__restrict__ p, uint32_t i) {
   }
 #endif
   return ldc(p, i);
+}
+// Record i of V-sized records (V a native 2-vector of the element type),
+// non-temporal with NT: one access for two consecutive slots of a
+// slot-paired row.
+typedef int mpas_i2v __attribute__((ext_vector_type(2)));
+typedef double mpas_d2v __attribute__((ext_vector_type(2)));
+template <bool NT, class V, class S>
+__device__ __forceinline__ V ld_vec(const S *__restrict__ p, uint32_t i) {
+  const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);
+  const MPAS_GLOBAL V *q = (const MPAS_GLOBAL V *)(b + i * uint32_t(sizeof(V)));
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (NT) return __builtin_nontemporal_load(q);
+#endif
+  return *q;
 }
 template <bool NT, class T>
 __device__ __forceinline__ T ldo(const T *__restrict__ p, uint32_t i) {

@@ -98,6 +98,12 @@ __device__ __forceinline__ void grp_row(T (&out)[W], const T (&mine)[MS], int ba
   for (int j = 0; j < W; j++) out[j] = __shfl(mine[j / G], base + j % G, 64);
 }
 
+// The one-level momentum and high-order flux kernels read their stencils
+// from the slot-paired copies (one 8- or 16-byte access per slot pair).
+#ifndef MPAS_PAIRED_STENCIL
+#define MPAS_PAIRED_STENCIL 1
+#endif
+
 // Slots of the momentum kernel's edgesOnEdge gathers per scheduling phase
 // with levels on lanes (0: all at once).
 #ifndef MPAS_QCH
@@ -197,6 +203,17 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
     const int j = row_slot(item, S.nchunk, WE, base);
     bcast_row<WE, uint32_t>(id, uint32_t(ld(M.eoe, j * E + e)), base);
     bcast_row<WE, double>(w, ld(M.woe, j * E + e), base);
+  } else if (LOL == 0 && MPAS_PAIRED_STENCIL && M.eoeP) {
+    // one access per slot pair
+#pragma unroll
+    for (int p = 0; p < WE / 2; p++) {
+      const mpas_i2v ij = ld_vec<kNtStencil, mpas_i2v>(M.eoeP, p * E + e);
+      const mpas_d2v wj = ld_vec<kNtStencil, mpas_d2v>(M.woeP, p * E + e);
+      id[2 * p] = uint32_t(ij.x);
+      id[2 * p + 1] = uint32_t(ij.y);
+      w[2 * p] = wj.x;
+      w[2 * p + 1] = wj.y;
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < WE; j++) {
@@ -321,6 +338,20 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     bcast_row<WA, double>(c3, ld(M.advCoef3, i * E + e), base);
 #pragma unroll
     for (int j = 0; j < WA; j++) c3[j] = c3[j] * M.coef3rd;
+  } else if (LOL == 0 && MPAS_PAIRED_STENCIL && M.advP) {
+    // one access per slot pair
+#pragma unroll
+    for (int p = 0; p < WA / 2; p++) {
+      const mpas_i2v ij = ld_vec<kNtStencil, mpas_i2v>(M.advP, p * E + e);
+      const mpas_d2v a1 = ld_vec<kNtStencil, mpas_d2v>(M.advCoefP, p * E + e);
+      const mpas_d2v a3 = ld_vec<kNtStencil, mpas_d2v>(M.advCoef3P, p * E + e);
+      ic[2 * p] = uint32_t(ij.x);
+      ic[2 * p + 1] = uint32_t(ij.y);
+      c1[2 * p] = a1.x;
+      c1[2 * p + 1] = a1.y;
+      c3[2 * p] = a3.x * M.coef3rd;
+      c3[2 * p + 1] = a3.y * M.coef3rd;
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < WA; i++) {
