@@ -181,6 +181,9 @@ FUSED_CASES = {
     # the high-order flux)
     "sw_fct8": (3, 1, 8, 2, 1e5, 1e3, False, 600.0, 3),
     "sw_centred2": (3, 1, 2, 1, 0.0, 1e3, False, 600.0, 4),
+    # four centred tracers: the cells kernel's two tracer-pair lanes share
+    # the cell's edge rows
+    "sw_centred4": (3, 1, 4, 1, 1e5, 1e3, False, 600.0, 3),
     "sw_fct_2levels": (3, 2, 2, 2, 1e5, 0.0, False, 600.0, 3),
     "case1_frozen": (3, 1, 1, 1, 0.0, 0.0, True, 40.0, 5),
     # one level per lane with many levels (the layout of multi-level contexts)
