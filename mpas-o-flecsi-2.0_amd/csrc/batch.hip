@@ -135,15 +135,28 @@ __device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
   }
   return S;
 }
-// At least 3 waves per SIMD for the momentum kernel: with two members per
-// lane it needs 171 VGPRs, 3 over the 3-wave budget.
+// The momentum kernel gathers its edgesOnEdge slots in phases of 4 (the
+// one-context kernels take 5): with two members per lane it then needs 157
+// VGPRs instead of 171, within the 3-wave budget (ensemble 12.24 -> 12.18
+// ms/step); the 3-wave hint stays as a guard.
+#ifndef MPAS_BATCH_EDGES_WPE
+#define MPAS_BATCH_EDGES_WPE 3
+#endif
+#if MPAS_BATCH_EDGES_WPE > 0
+#define MPAS_BATCH_EDGES_WAVES __attribute__((amdgpu_waves_per_eu(MPAS_BATCH_EDGES_WPE, 8)))
+#else
+#define MPAS_BATCH_EDGES_WAVES
+#endif
+#ifndef MPAS_BATCH_QCH
+#define MPAS_BATCH_QCH 4
+#endif
 template <int WE, int MODE>
-__global__ __launch_bounds__(kBatchMaxBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void
+__global__ __launch_bounds__(kBatchMaxBlock) MPAS_BATCH_EDGES_WAVES void
 kb_stage_edges(BatchArgs A, int s) {
   BATCH_PROLOGUE;
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
-  stage_edges<WE, lol_of(MODE), lp_of(MODE), fix_of(MODE)>(M, S, item);
+  stage_edges<WE, lol_of(MODE), lp_of(MODE), fix_of(MODE), MPAS_BATCH_QCH>(M, S, item);
 }
 template <int NT, int WA, int MODE>
 __global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {

@@ -182,7 +182,7 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-template <int WE, int LOL = 0, int LP = 1, int FIX = 0>
+template <int WE, int LOL = 0, int LP = 1, int FIX = 0, int QCX = MPAS_QCH>
 __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
   int sub;
   uint32_t k0, k1;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
     // are live at once): with levels on lanes the freed registers buy
     // occupancy (level 7, 32 levels: 426 -> 409 us), one-level launches
     // were faster with every gather in flight at once (92 against 95 us)
-    constexpr int QCH = LOL && MPAS_QCH > 0 && MPAS_QCH < WE ? MPAS_QCH : WE;
+    constexpr int QCH = LOL && QCX > 0 && QCX < WE ? QCX : WE;
     double qs[LP];
 #pragma unroll
     for (int p = 0; p < LP; p++) qs[p] = 0.0;
