@@ -197,6 +197,11 @@ int build_device_mesh(mpas_ctx *ctx) {
         c3[size_t(i) * E + e] = live ? m.advCoef_3rd[size_t(e) * 16 + i] : 0.0;
       }
     }
+    // init_tracer_adv_coeff lists the edge's two cells first
+    M.advC12 = 1;
+    for (int e = 0; e < E && M.advC12; e++)
+      if (na[e] < 2 || adv[e] != m.e2c.at(e, 0) || adv[size_t(E) + e] != m.e2c.at(e, 1))
+        M.advC12 = 0;
     if ((rc = upload(ctx, na, &M.nAdv)) || (rc = upload(ctx, adv, &M.adv)) ||
         (rc = upload(ctx, c1, &M.advCoef)) || (rc = upload(ctx, c3, &M.advCoef3)) ||
         (rc = upload(ctx, slot_pairs(adv, padE, E), &M.advP)) ||

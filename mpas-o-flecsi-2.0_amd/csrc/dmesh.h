@@ -54,6 +54,7 @@ struct DMesh {
   const int *adv;                       // [j][E] advCellsForEdge
   const double *advCoef, *advCoef3;     // [j][E] advCoef, advCoef_3rd
   const int *advP;                      // advCellsForEdge, [j/2][E][2]
+  int advC12;  // 1: every edge's advection cells start with cells(e)[0], [1]
   const double *advCoefP, *advCoef3P;   // advCoef, advCoef_3rd, [j/2][E][2]
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere

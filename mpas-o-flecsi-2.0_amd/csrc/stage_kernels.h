@@ -375,8 +375,16 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
 #pragma unroll
       for (int i = 0; i < WA; i++)
         ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
-      ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
-      ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k + p) * NT + t0, t2[p]);
+      if (M.advC12) {  // the stencil's first two cells are the edge's cells
+#pragma unroll
+        for (int g = 0; g < TPL; g++) {
+          t1[p][g] = tv[0][p][g];
+          t2[p][g] = tv[1][p][g];
+        }
+      } else {
+        ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
+        ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k + p) * NT + t0, t2[p]);
+      }
     }
     double ho[LP][TPL];
 #pragma unroll
