@@ -187,7 +187,7 @@ def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3
 
 
 def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
-                   groups=((5, 96, 480.0), (6, 128, 240.0), (7, 32, 120.0))):
+                   groups=((5, 96, 480.0), (6, 128, 240.0), (7, 32, 120.0)), trace_steps=0):
     """256-member ensemble on meshes of uneven size (96 x level 5 at dt 480 s,
     128 x level 6 at dt 240 s, 32 x level 7 at dt 120 s), 2 FCT tracers, each
     member with a seeded bump amplitude in [5, 20] m and u0 in [10, 30] m/s.
@@ -197,7 +197,11 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
     mesh's context (mpas_rk4_step: the single-run launches), member after
     member, timed over seq_steps steps per member (its cost per step is
     constant).  The batched members equal the single runs bit for bit
-    (checked on one member per mesh)."""
+    (checked on one member per mesh).
+
+    trace_steps > 0: a kernel-trace run instead, with the default launch
+    configuration (no autotuning) and trace_steps direct-launch steps (one
+    launch per phase and layout, no graph), nothing else."""
     from mpas_hip import BATCH_GRAPH, BATCH_LAUNCH, TRACER_FCT, Batch, Context, Mesh, sw_workload_state
     t0 = time.perf_counter()
     meshes = [Mesh(L) for L, _, _ in groups]
@@ -222,6 +226,18 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
     load()
     setup_s = time.perf_counter() - t0
     log(f"  ensemble: {len(mesh_of)} members in {b.groups()} groups set up in {setup_s:.1f} s")
+    if trace_steps > 0:
+        b.step(2, BATCH_LAUNCH)
+        b.sync()
+        t = time.perf_counter()
+        b.step(trace_steps, BATCH_LAUNCH)
+        b.sync()
+        ms = 1e3 * (time.perf_counter() - t) / trace_steps
+        cfg = b.get_config()
+        b.close()
+        for c in ctxs:
+            c.close()
+        return {"trace_steps": trace_steps, "ms_per_step_batched_launch": ms, "config": cfg}
     trials = b.autotune(trial_steps=tune_steps)
     cfg = b.get_config()
     log(f"  ensemble: autotuned {cfg} over {len(trials)} trials")
@@ -486,6 +502,9 @@ def main():
                     help="batched steps of the ensemble measurement (its workload has 500)")
     ap.add_argument("--ens-seq-steps", type=int, default=20,
                     help="steps per member of the member-after-member baseline")
+    ap.add_argument("--ens-trace-steps", type=int, default=0,
+                    help="with --ensemble-only: only this many direct-launch batched steps "
+                         "in the default configuration (for a kernel trace)")
     ap.add_argument("--no-adjoint-ensemble", action="store_true",
                     help="skip the batched adjoint ensemble (64 x level 7 + 64 x level 5)")
     ap.add_argument("--adjoint-ensemble-only", action="store_true")
@@ -511,8 +530,8 @@ def main():
         print(json.dumps(adjoint_ensemble_bench()), flush=True)
         return
     if args.ensemble_only:
-        print(json.dumps(ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps)),
-              flush=True)
+        print(json.dumps(ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps,
+                                        trace_steps=args.ens_trace_steps)), flush=True)
         return
 
     t0 = time.perf_counter()
