@@ -186,7 +186,7 @@ def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3
     }
 
 
-def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=3,
+def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
                    groups=((5, 96, 480.0), (6, 128, 240.0), (7, 32, 120.0)), trace_steps=0):
     """256-member ensemble on meshes of uneven size (96 x level 5 at dt 480 s,
     128 x level 6 at dt 240 s, 32 x level 7 at dt 120 s), 2 FCT tracers, each
