@@ -79,6 +79,8 @@ struct mpas_batch {
   mpasx::BatchGroup *d_grp = nullptr;
   // launch configuration and its tables
   int block[mpasx::kBatchKernels] = {256, 256, 256, 256, 256, 256};
+  // per kernel and layout, set by the autotuner only (0: block[k])
+  int mblock[mpasx::kBatchKernels][mpasx::kBatchModes] = {};
   int lpl = 1;    // members per lane
   // one entity per wave (members padded to whole waves): 0 for no group,
   // 1 for the groups that keep at least 3/4 of the padded lanes busy, 2 for
@@ -286,6 +288,12 @@ int64_t kernel_items(const mpas_batch *B, int k, int q) {
   }
 }
 
+// Workgroup size of kernel k in layout mode.
+int block_of(const mpas_batch *B, int k, int mode) {
+  const int b = B->mblock[k][mode - 1];
+  return b ? b : B->block[k];
+}
+
 void free_tables(mpas_batch *B) {
   for (int k = 0; k < kBatchKernels; k++)
     for (int m = 0; m < kBatchModes; m++) {
@@ -359,7 +367,7 @@ int build_tables(mpas_batch *B) {
   const int ng = int(B->groups.size());
   for (int k = 0; k < kBatchKernels; k++)
     for (int mode = 1; mode <= kBatchModes; mode++) {
-      const int bs = B->block[k];
+      const int bs = block_of(B, k, mode);
       std::vector<int> nb(ng, 0);
       for (int q = 0; q < ng; q++)
         if (group_mode(B, q, k) == mode) nb[q] = int((kernel_items(B, k, q) + bs - 1) / bs);
@@ -393,7 +401,7 @@ BatchArgs args_of(const mpas_batch *B, int k, int mode) {
 #define BL1(k, kern, m)                                                                   \
   if (B->grid[k][m - 1]) {                                                                \
     constexpr int MODE = m;                                                               \
-    hipLaunchKernelGGL(kern, dim3(B->grid[k][m - 1]), dim3(B->block[k]), 0, st,           \
+    hipLaunchKernelGGL(kern, dim3(B->grid[k][m - 1]), dim3(block_of(B, k, m)), 0, st,      \
                        args_of(B, k, m), s);                                              \
     if (int rc = after_launch(#kern)) return rc;                                          \
   }
@@ -754,6 +762,8 @@ int mpas_batch_config(mpas_batch *B, const int32_t *block, int members_per_lane,
         MPAS_FAIL(MPAS_EINVAL, "mpas_batch_config: block sizes are multiples of 64 up to 512");
   MPAS_HIP(hipStreamSynchronize(B->stream));
   if (block) std::copy(block, block + kBatchKernels, B->block);
+  for (auto &row : B->mblock)  // an explicit configuration applies to every layout
+    for (int &b : row) b = 0;
   B->lpl = members_per_lane;
   B->order = order;
   B->xcd = xcd != 0;
@@ -839,6 +849,23 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
       if (t < best) best = t, pick = bs;
     }
     B->block[k] = pick;
+  }
+  // then per layout, where a kernel runs in more than one
+  for (int k = 0; k < kBatchKernels; k++) {
+    int present = 0;
+    for (int m = 0; m < kBatchModes; m++) present += B->grid[k][m] > 0;
+    if (present < 2) continue;
+    for (int m = 0; m < kBatchModes; m++) {
+      if (!B->grid[k][m]) continue;
+      int pick = 0;
+      for (int bs : sizes) {
+        if (bs == B->block[k]) continue;
+        B->mblock[k][m] = bs;
+        const double t = trial("layout block");
+        if (t < best) best = t, pick = bs;
+      }
+      B->mblock[k][m] = pick;
+    }
   }
   if (!rc) rc = build_tables(B);
   (void)hipMemcpyAsync(B->arena, save, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream);
