@@ -62,12 +62,14 @@ __device__ __forceinline__ double sum_ascending(double acc, const double (&v)[W]
 // Static rows shared across an entity's lanes.  With levels on lanes the G
 // lanes of an entity (G = chunks x tracer lane groups) would each load the
 // same W slots of its stencil row: W vector loads per lane for one row.
-// When the G lanes form an aligned block of a wave (G divides 64; they do,
-// since an entity's lanes are adjacent items and workgroups are whole
-// waves) and G >= W, lane r of the block loads slot r only and every lane
-// takes slot j from lane base + j by a cross-lane permute (ds_bpermute,
-// no memory access).  Every lane of the block belongs to the same entity,
-// so a lane that returned early (past the last entity) is never read.
+// When G divides 64 the G lanes form an aligned block of a wave (an
+// entity's lanes are adjacent items, entities start at multiples of G and
+// workgroups are whole waves); if also G >= W, lane r of the block loads
+// slot r only and every lane takes slot j from lane base + j by a
+// cross-lane permute (ds_bpermute, no memory access).  Every lane of the
+// block belongs to the same entity, so a lane that returned early (past
+// the last entity) is never read.  The permutes run outside divergent
+// code: an inactive source lane would not provide its value.
 #ifndef MPAS_SHARE_ROWS
 #define MPAS_SHARE_ROWS 1
 #endif
