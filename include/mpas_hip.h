@@ -377,10 +377,17 @@ int mpas_batch_config(mpas_batch *batch, const int32_t *block, int members_per_l
                       int order, int xcd, int wave_entity);
 int mpas_batch_get_config(const mpas_batch *batch, int32_t *block, int *members_per_lane,
                           int *order, int *xcd, int *wave_entity);
+/* Workgroup sizes per kernel and group layout, for kernels that run in more
+   than one layout: 6 kernels x 6 layouts (MPAS_BATCH_LAYOUTS), row-major by
+   kernel; 0 takes the kernel's size from mpas_batch_config (which clears
+   them all).  The autotuner sets them; get + set reproduce its choice. */
+enum { MPAS_BATCH_KERNELS = 6, MPAS_BATCH_LAYOUTS = 6 };
+int mpas_batch_get_layout_blocks(const mpas_batch *batch, int32_t *blocks);
+int mpas_batch_set_layout_blocks(mpas_batch *batch, const int32_t *blocks);
 /* Times trial_steps graph replays per candidate (entity per wave and
-   members per lane, placement and order, then each kernel's workgroup size), keeps the
-   fastest configuration and restores the members' state; writes the
-   trials as JSON into report. */
+   members per lane, placement and order, then each kernel's workgroup size,
+   then that size per layout), keeps the fastest configuration and restores
+   the members' state; writes the trials as JSON into report. */
 int mpas_batch_autotune(mpas_batch *batch, int trial_steps, char *report, size_t cap);
 
 #ifdef __cplusplus

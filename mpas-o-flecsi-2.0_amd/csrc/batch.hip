@@ -784,12 +784,38 @@ int mpas_batch_get_config(const mpas_batch *B, int32_t *block, int *members_per_
   return 0;
 }
 
+int mpas_batch_get_layout_blocks(const mpas_batch *B, int32_t *blocks) {
+  if (!B || !blocks) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_get_layout_blocks: NULL argument");
+  static_assert(MPAS_BATCH_KERNELS == kBatchKernels && MPAS_BATCH_LAYOUTS == kBatchModes,
+                "ABI sizes of the per-layout table");
+  for (int k = 0; k < kBatchKernels; k++)
+    for (int m = 0; m < kBatchModes; m++) blocks[k * kBatchModes + m] = B->mblock[k][m];
+  return 0;
+}
+
+int mpas_batch_set_layout_blocks(mpas_batch *B, const int32_t *blocks) {
+  if (int rc = check_batch(B)) return rc;
+  if (!blocks) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_set_layout_blocks: blocks is NULL");
+  for (int i = 0; i < kBatchKernels * kBatchModes; i++)
+    if (blocks[i] && (blocks[i] < 64 || blocks[i] > kBatchMaxBlock || blocks[i] % 64))
+      MPAS_FAIL(MPAS_EINVAL,
+                "mpas_batch_set_layout_blocks: sizes are 0 or multiples of 64 up to 512");
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  for (int k = 0; k < kBatchKernels; k++)
+    for (int m = 0; m < kBatchModes; m++) B->mblock[k][m] = blocks[k * kBatchModes + m];
+  return build_tables(B);
+}
+
 int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap) {
   if (int rc = check_batch(B)) return rc;
   if (trial_steps < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_batch_autotune: trial_steps < 1");
   for (int m = 0; m < B->nmem; m++)
     if (!B->ready[m]) MPAS_FAIL(MPAS_ESTATE, "member " + std::to_string(m) + " has no state");
   MPAS_HIP(hipStreamSynchronize(B->stream));
+  // start from the kernels' own sizes: per-layout sizes of an earlier run
+  // would otherwise hide the per-kernel trials below
+  for (auto &row : B->mblock)
+    for (int &b : row) b = 0;
   // the trials advance the members: keep the state and restore it after
   void *save = nullptr;
   if (hipMalloc(&save, B->arena_bytes) != hipSuccess)
@@ -805,11 +831,15 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     std::snprintf(line, sizeof line,
                   "%s{\"trial\": \"%s\", \"block\": [%d, %d, %d, %d, %d, %d], "
                   "\"members_per_lane\": %d, \"wave_entity\": %d, \"order\": %d, \"xcd\": %d, "
-                  "\"ms_per_step\": %.4f}",
+                  "\"ms_per_step\": %.4f, \"layout_block\": [",
                   log.size() > 1 ? ", " : "", what, B->block[0], B->block[1], B->block[2],
                   B->block[3], B->block[4], B->block[5], B->lpl, B->wave_entity, B->order,
                   B->xcd, 1e3 * t / trial_steps);
     log += line;
+    for (int k = 0; k < kBatchKernels; k++)
+      for (int m = 0; m < kBatchModes; m++)
+        log += std::to_string(B->mblock[k][m]) +
+               (k == kBatchKernels - 1 && m == kBatchModes - 1 ? "]}" : ", ");
     return rc ? 1e30 : t;
   };
   double best = trial("start");

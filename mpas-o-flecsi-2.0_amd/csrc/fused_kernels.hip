@@ -466,38 +466,67 @@ int ocean_rk4_steps(mpas_ctx *ctx, double dt, int nsteps) {
   return 0;
 }
 
+// The buffers stage s reads and writes (apps/sw/run.cc:13-70): the
+// reference's copy tasks become these roles, with the ping-pong slot W.cur
+// holding the stage's input diagnostics.
+StageIO sw_stage_io(const mpas_ctx *ctx, const SwState &W, int s, double dt) {
+  const int cur = W.cur, nxt = 1 - W.cur;
+  StageIO S = stage_params(ctx, s, dt);
+  S.xu = s == 0 ? W.u : W.pu[cur];
+  S.xh = s == 0 ? W.h : W.ph[cur];
+  S.xtr = s == 0 ? W.tr : W.ptr[cur];
+  S.h_edge = W.h_edge[cur];
+  S.ke = W.ke[cur];
+  S.pv_edge = W.pv_edge[cur];
+  S.vort = W.vort[cur];
+  S.div = W.div[cur];
+  S.h_s = W.h_s;
+  S.u0 = W.u;
+  S.h0 = W.h;
+  S.tr0 = W.tr;
+  S.nu = W.nu;
+  S.nh = W.nh;
+  S.ntr = W.ntr;
+  S.yu = s < 3 ? W.pu[nxt] : W.u;
+  S.yh = s < 3 ? W.ph[nxt] : W.h;
+  S.ytr = s < 3 ? W.ptr[nxt] : W.tr;
+  S.ke_y = W.ke[nxt];
+  S.div_y = W.div[nxt];
+  S.anti = W.anti;
+  S.rfac = W.rfac;
+  S.tlo = W.tlo;
+  return S;
+}
+
 // nsteps of apps/sw/run.cc:13-70.
 int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps, double *const *frozen_rfac) {
   SwState &W = *ctx->sw;
   for (int step = 0; step < nsteps; step++) {
     for (int s = 0; s < 4; s++) {
-      const int cur = W.cur, nxt = 1 - W.cur;
-      StageIO S = stage_params(ctx, s, dt);
-      S.xu = s == 0 ? W.u : W.pu[cur];
-      S.xh = s == 0 ? W.h : W.ph[cur];
-      S.xtr = s == 0 ? W.tr : W.ptr[cur];
-      S.h_edge = W.h_edge[cur];
-      S.ke = W.ke[cur];
-      S.pv_edge = W.pv_edge[cur];
-      S.vort = W.vort[cur];
-      S.div = W.div[cur];
-      S.h_s = W.h_s;
-      S.u0 = W.u;
-      S.h0 = W.h;
-      S.tr0 = W.tr;
-      S.nu = W.nu;
-      S.nh = W.nh;
-      S.ntr = W.ntr;
-      S.yu = s < 3 ? W.pu[nxt] : W.u;
-      S.yh = s < 3 ? W.ph[nxt] : W.h;
-      S.ytr = s < 3 ? W.ptr[nxt] : W.tr;
-      S.ke_y = W.ke[nxt];
-      S.div_y = W.div[nxt];
-      S.anti = W.anti;
-      S.rfac = frozen_rfac ? frozen_rfac[4 * step + s] : W.rfac;
-      S.keep_rfac = frozen_rfac != nullptr;
-      S.tlo = W.tlo;
+      const int nxt = 1 - W.cur;
+      StageIO S = sw_stage_io(ctx, W, s, dt);
+      if (frozen_rfac) {
+        S.rfac = frozen_rfac[4 * step + s];
+        S.keep_rfac = 1;
+      }
       if (int rc = launch_stage(ctx, S, W.vort[nxt], W.pv_vertex, W.h_edge[nxt], W.pv_edge[nxt]))
+        return rc;
+      W.cur = nxt;
+    }
+    W.elapsed += dt;
+  }
+  return 0;
+}
+
+// The same on buffers the caller describes, without the context's
+// profiling (which creates events): no allocation of any kind.
+int sw_rk4_steps_on(const mpas_ctx *ctx, SwState &W, double dt, int nsteps) {
+  for (int step = 0; step < nsteps; step++) {
+    for (int s = 0; s < 4; s++) {
+      const int nxt = 1 - W.cur;
+      const StageIO S = sw_stage_io(ctx, W, s, dt);
+      if (int rc = launch_stage_dm(ctx->dm, ctx->stream, S, W.vort[nxt], W.pv_vertex,
+                                   W.h_edge[nxt], W.pv_edge[nxt]))
         return rc;
       W.cur = nxt;
     }

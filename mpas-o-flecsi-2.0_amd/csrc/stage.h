@@ -76,4 +76,11 @@ int ocean_rk4_steps(mpas_ctx *ctx, double dt, int nsteps);
 // them (frozen_rfac[4 * step + stage]).
 int sw_rk4_steps(mpas_ctx *ctx, double dt, int nsteps, double *const *frozen_rfac = nullptr);
 
+// The buffer roles of stage s of the fused step over the buffers of W.
+StageIO sw_stage_io(const mpas_ctx *ctx, const SwState &W, int s, double dt);
+
+// nsteps fused RK4 steps over the buffers of W (not the context's state),
+// launched on the context's stream without profiling.
+int sw_rk4_steps_on(const mpas_ctx *ctx, SwState &W, double dt, int nsteps);
+
 }  // namespace mpasx

@@ -492,6 +492,13 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   constexpr bool GRP = LOL == 0 && NPL > 1 && 64 % NPL == 0 && MPAS_SHARE_ROWS;
   constexpr int MS = (W + NPL - 1) / NPL;
   uint32_t me[MS];
+  // shared rows the level loop reads (dcEdge for ke, the other cells of the
+  // edges for the tracers), broadcast here: inside the loop the permutes
+  // would run under divergent control (only the lead lane forms ke, and the
+  // last level chunk may take fewer passes), where an inactive source lane
+  // provides no value
+  double dcS[W];
+  uint32_t opS[W];
   if constexpr (GRP) {
     base = int(threadIdx.x & 63u) - sub;
     double dm[MS];
@@ -507,6 +514,8 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
     ej = ld(M.c2e, js * C + c);
     bcast_row<W, uint32_t>(e, ej, base);
     bcast_row<W, double>(dv, ldc(M.dv, ej), base);
+    bcast_row<W, double>(dcS, ldc(M.dc, ej), base);
+    if constexpr (NT > 0) bcast_row<W, uint32_t>(opS, ld(M.cOpp, js * C + c), base);
   } else {
 #pragma unroll
     for (int j = 0; j < W; j++) e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
@@ -530,7 +539,8 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
       for (int j = 0; j < W; j++) yu[j] = ld(S.yu, e[j] * L + k);
       if (shared) {
-        bcast_row<W, double>(dc, ldc(M.dc, ej), base);
+#pragma unroll
+        for (int j = 0; j < W; j++) dc[j] = dcS[j];
       } else {
 #pragma unroll
         for (int j = 0; j < W; j++) dc[j] = ldc(M.dc, e[j]);
@@ -608,7 +618,8 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
             om[m] = ldn<kNtCells>(M.cOpp, min(sub + m * NPL, W - 1) * C + c);
           grp_row<NPL>(op, om, base);
         } else if (shared) {
-          bcast_row<W, uint32_t>(op, ld(M.cOpp, js * C + c), base);
+#pragma unroll
+          for (int j = 0; j < W; j++) op[j] = opS[j];
         } else {
 #pragma unroll
           for (int j = 0; j < W; j++) op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
@@ -738,6 +749,9 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   constexpr bool GRP = LOL == 0 && NPL > 1 && 64 % NPL == 0 && MPAS_SHARE_ROWS;
   constexpr int MS = (W + NPL - 1) / NPL;
   uint32_t me[MS];
+  // the mixing coefficients of the edges, shared rows broadcast before the
+  // level loop (whose pass count may differ between the level chunks)
+  double rhoS[W];
   if constexpr (GRP) {
     base = int(threadIdx.x & 63u) - sub;
     uint32_t mo[MS];
@@ -754,6 +768,8 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     ej = ld(M.c2e, js * C + c);
     bcast_row<W, uint32_t>(e, ej, base);
     bcast_row<W, uint32_t>(op, ld(M.cOpp, js * C + c), base);
+    if (S.use_hmix)
+      bcast_row<W, double>(rhoS, M.msd * S.kappa * ldc(M.dv, ej) / ldc(M.dc, ej), base);
   } else {
 #pragma unroll
     for (int j = 0; j < W; j++) {
@@ -828,7 +844,8 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
       }
       if constexpr (!GRP) {
         if (shared) {
-          bcast_row<W, double>(rho, M.msd * S.kappa * ldc(M.dv, ej) / ldc(M.dc, ej), base);
+#pragma unroll
+          for (int j = 0; j < W; j++) rho[j] = rhoS[j];
         } else {
 #pragma unroll
           for (int j = 0; j < W; j++)

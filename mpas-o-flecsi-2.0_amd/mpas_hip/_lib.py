@@ -146,6 +146,8 @@ SIGNATURES = {
     "mpas_batch_groups": (c_int, [c_void_p]),
     "mpas_batch_config": (c_int, [c_void_p, P_i32, c_int, c_int, c_int, c_int]),
     "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 4),
+    "mpas_batch_get_layout_blocks": (c_int, [c_void_p, P_i32]),
+    "mpas_batch_set_layout_blocks": (c_int, [c_void_p, P_i32]),
     "mpas_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
 }
 

@@ -468,6 +468,21 @@ class Batch:
         return {"block": blk.tolist(), "members_per_lane": lpl.value, "order": o.value,
                 "xcd": x.value, "wave_entity": we.value}
 
+    def layout_blocks(self):
+        """Workgroup sizes per kernel (rows) and group layout (columns), 0 for
+        the kernel's own size."""
+        b = np.zeros(36, dtype=np.int32)
+        L.check(L.hip().mpas_batch_get_layout_blocks(self._b, b.ctypes.data_as(L.P_i32)),
+                "mpas_batch_get_layout_blocks")
+        return b.reshape(6, 6)
+
+    def set_layout_blocks(self, blocks):
+        b = np.ascontiguousarray(blocks, dtype=np.int32).ravel()
+        if b.size != 36:
+            raise ValueError("6 x 6 workgroup sizes")
+        L.check(L.hip().mpas_batch_set_layout_blocks(self._b, b.ctypes.data_as(L.P_i32)),
+                "mpas_batch_set_layout_blocks")
+
     def autotune(self, trial_steps=3):
         import json
         buf = ctypes.create_string_buffer(1 << 16)

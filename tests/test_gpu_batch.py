@@ -129,6 +129,28 @@ def test_shared_stencil_rows_keep_results():
     _close(ctxs, b)
 
 
+def test_tracer_lanes_share_rows_keep_results():
+    """Four tracers: the cells and tracer-update kernels run two tracer lanes
+    per member, and with levels on lanes a cell's lanes share its edge rows.
+    A group of 4 members (8 lanes per cell) and one of 15 members, whose last
+    level chunk is short with two members per lane (its lanes take fewer
+    passes of the level loop than the others): every member equals its
+    single run bit for bit."""
+    levels = (3,) * 4 + (4,) * 15
+    dts = tuple(600.0 if L == 3 else 300.0 for L in levels)
+    ctxs, b, states, mesh_of = _ensemble(levels, 4, dts)
+    ref = _singles(ctxs, states, mesh_of, 3, dts)
+    for lpl, we in ((1, 0), (2, 0), (2, 1), (1, 2)):
+        b.config([256] * 6, lpl, 0, 1, we)
+        _load(b, states)
+        b.step(3)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (lpl, we, i)
+    _close(ctxs, b)
+
+
 def test_autotune_keeps_results_and_state():
     ctxs, b, states, mesh_of = _ensemble()
     ref = _singles(ctxs, states, mesh_of, 3)
@@ -136,7 +158,18 @@ def test_autotune_keeps_results_and_state():
     before = [b.download(i) for i in range(b.n)]
     trials = b.autotune(trial_steps=2)
     assert len(trials) >= 5 and all(t["ms_per_step"] > 0 for t in trials)
+    assert all(len(t["layout_block"]) == 36 for t in trials)
     print("tuned:", b.get_config())
+    cfg, lb = b.get_config(), b.layout_blocks()
+    # the tuned configuration is reproducible: config + layout blocks
+    b.config(cfg["block"], cfg["members_per_lane"], cfg["order"], cfg["xcd"],
+             cfg["wave_entity"])
+    assert not b.layout_blocks().any()  # an explicit configuration clears them
+    b.set_layout_blocks(lb)
+    assert np.array_equal(b.layout_blocks(), lb)
+    # a second run starts from the kernels' own sizes again
+    trials2 = b.autotune(trial_steps=1)
+    assert not any(trials2[0]["layout_block"])
     for x, y in zip(before, [b.download(i) for i in range(b.n)]):  # the trials are undone
         for a, c in zip(x, y):
             assert bitwise_equal(a, c)

@@ -191,6 +191,12 @@ FUSED_CASES = {
     # 32 levels: level pairs put an edge's 32 levels on 16 lanes, enough for
     # every stencil row to be loaded once per lane block and shared
     "sw_fct_32levels": (3, 32, 2, 2, 1e5, 1e3, False, 600.0, 2),
+    # levels on lanes with several tracer lanes per cell: the cells and
+    # tracer kernels share an entity's rows across 16 (or 8) lanes, of which
+    # only the lead lane of each level forms ke
+    "sw_fct4_8levels": (3, 8, 4, 2, 1e5, 1e3, False, 600.0, 3),
+    "sw_centred4_4levels": (3, 4, 4, 1, 1e5, 1e3, False, 600.0, 3),
+    "sw_fct8_4levels": (3, 4, 8, 2, 1e5, 1e3, False, 600.0, 2),
     # the plain CPU-parity run: level 4, dt 960 s, viscosity 1e5, 100 steps
     "plain_L4_100": (4, 1, 0, 0, 1e5, 0.0, False, 960.0, 100),
     # the single-instance workload's configuration on a level-4 mesh, run
