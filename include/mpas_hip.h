@@ -292,6 +292,23 @@ int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host,
 /* nsteps calls of advance_timestep (apps/sw/run.cc:13-70, loop 97-106),
    with the stage-fused kernels; the result equals the per-task sequence. */
 int mpas_rk4_step(mpas_ctx *ctx, double dt, int nsteps);
+/* ---- allocation-free step ------------------------------------------------
+   No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc): mpas_rk4_step on a state the caller holds in device memory,
+   h [nCells*nLevels] and u [nEdges*nLevels] and tracers
+   [nCells*nLevels*nTracers] (NULL without tracers) advanced in place, h_s
+   [nCells] read.  The initial diagnostics are formed first, so the result
+   equals mpas_sw_state_upload of the same state followed by mpas_rk4_step.
+   Every intermediate lives in one static device array of
+   MPAS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a context
+   whose step needs more returns MPAS_ESHAPE (64 MiB holds a level-6 mesh of
+   40,962 cells with up to 8 tracers).  All calls on one GPU share that
+   array, so calls that overlap in time on one GPU (several streams or
+   threads) race.  Asynchronous on the context's stream (mpas_ctx_sync). */
+#define MPAS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mpas_rk4_step_noalloc(mpas_ctx *ctx, double dt, int nsteps, double *h, double *u,
+                          double *tracers, const double *h_s);
 /* nsteps calls of the tracer-only ocean driver's rk4_timestep
    (apps/ocean/run.cc:10-50, loop 70-72): the uploaded
    u (normalVelocity) and h (layerThickness) stay fixed, layerThicknessEdge

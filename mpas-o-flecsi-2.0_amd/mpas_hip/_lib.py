@@ -131,6 +131,7 @@ SIGNATURES = {
     "mpas_sw_field_download": (c_int, [c_void_p, c_char_p, P_f64, c_int64]),
     "mpas_rk4_step": (c_int, [c_void_p, c_double, c_int]),
     "mpas_ocean_rk4_step": (c_int, [c_void_p, c_double, c_int]),
+    "mpas_rk4_step_noalloc": (c_int, [c_void_p, c_double, c_int, _D, _D, _D, _D]),
     "mpas_adj_init": (c_int, [c_void_p, c_int]),
     "mpas_adj_forward": (c_int, [c_void_p, c_double, c_int] + [_D] * 6),
     "mpas_adj_backward": (c_int, [c_void_p] + [_D] * 6),

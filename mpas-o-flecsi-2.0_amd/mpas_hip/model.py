@@ -346,6 +346,14 @@ class Context:
     def rk4_step(self, dt, nsteps=1):
         L.check(L.hip().mpas_rk4_step(self._h, float(dt), int(nsteps)), "mpas_rk4_step")
 
+    def rk4_step_noalloc(self, dt, nsteps, h, u, tracers, h_s):
+        """mpas_rk4_step_noalloc: nsteps fused steps of the state in the
+        device arrays h, u, tracers (advanced in place; h_s read), with the
+        step's intermediates in the library's fixed scratch array."""
+        L.check(L.hip().mpas_rk4_step_noalloc(self._h, float(dt), int(nsteps), _ptr(h), _ptr(u),
+                                              _ptr(tracers), _ptr(h_s)),
+                "mpas_rk4_step_noalloc")
+
     def ocean_rk4_step(self, dt, nsteps=1):
         """The tracer-only ocean driver (u, h fixed) on the uploaded state."""
         L.check(L.hip().mpas_ocean_rk4_step(self._h, float(dt), int(nsteps)),
