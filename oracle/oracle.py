@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "liboracle.so")
+# MPAS_ORACLE_LIB points the checker at another build of the oracle (a
+# scratch copy with a planted mistake, to see which pin catches it)
+LIB = os.environ.get("MPAS_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 _lib = None
 P_f64 = ctypes.POINTER(ctypes.c_double)
@@ -44,6 +46,7 @@ def lib():
         L.orc_mesh_set_dbl.argtypes = [vp, ctypes.c_char_p, P_f64, i64]
         L.orc_mesh_finalize.argtypes = [vp]
         L.orc_mesh_setup_fct.argtypes = [vp, i32, d]
+        L.orc_migs.argtypes = [i32, P_f64, P_f64]
         L.orc_mesh_cnx.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(P_i32),
                                    ctypes.POINTER(P_i32), ctypes.POINTER(i32)]
         L.orc_mesh_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(i64),

@@ -329,6 +329,10 @@ int orc_mesh_finalize(void *h) {
   return guard([&] { finalize_mesh(*static_cast<Mesh *>(h)); });
 }
 
+int orc_migs(int n, const double *a, double *x) {
+  return guard([&] { migs_dense(n, a, x); });
+}
+
 int orc_mesh_setup_fct(void *h, int nL, double a) {
   return guard([&] { setup_fct(*static_cast<Mesh *>(h), nL, a); });
 }

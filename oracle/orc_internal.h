@@ -157,6 +157,10 @@ void ocn_update_tracers(const Mesh &m, int nL, int nT, const double *trNew,
 // FCT setup (orc_setup.cc)
 void setup_fct(Mesh &m, int nL, double a);
 
+// The inverse of the n x n row-major matrix a by migs (matrix.hh:137-179)
+// into x: exposed so the tests can check it against a library inverse.
+void migs_dense(int n, const double *a, double *x);
+
 // drivers (orc_driver.cc)
 struct SwState;  // opaque
 }  // namespace orc

@@ -321,6 +321,15 @@ void init_tracer_adv_coeff(Mesh &m, int nL) {
 
 }  // namespace
 
+void migs_dense(int n, const double *a, double *x) {
+  Mat m(n, std::vector<double>(n));
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) m[i][j] = a[i * n + j];
+  const Mat inv = migs(m);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) x[i * n + j] = inv[i][j];
+}
+
 // apps/ocean/initialize.cc:273-287 order: maxLevelEdgeTop, (layer thickness
 // edge is a field, set by the caller), meshScalingDel2, edgeSignOnCell,
 // 2nd derivatives, advection coefficients.  maxLevelCell and boundaryCell

@@ -227,6 +227,170 @@ def test_stencil_coefficients_reproduce_constants(mesh4):
     assert n.min() >= 9 and n.max() <= 10
 
 
+# Independent geometry in numpy (not the oracle's spherical helpers): unit
+# positions, the local east / north frame of a cell centre, azimuths of
+# directions in that frame, and the midpoints of the Voronoi edges.
+
+def _unit(x, y, z):
+    p = np.stack([x, y, z], 1)
+    return p / np.linalg.norm(p, axis=1, keepdims=True)
+
+
+def _frame(p):
+    lon = np.arctan2(p[..., 1], p[..., 0])
+    east = np.stack([-np.sin(lon), np.cos(lon), np.zeros_like(lon)], -1)
+    return east, np.cross(p, east)
+
+
+def _azimuth(pc, q, east, north):
+    d = q - np.sum(q * pc, -1, keepdims=True) * pc
+    return np.arctan2(np.sum(d * north, -1), np.sum(d * east, -1))
+
+
+def _edge_midpoints(m):
+    vx = _unit(m["xVertex"], m["yVertex"], m["zVertex"])
+    voe = m["verticesOnEdge"].reshape(-1, 2) - 1
+    mid = vx[voe[:, 0]] + vx[voe[:, 1]]
+    return mid / np.linalg.norm(mid, axis=1, keepdims=True)
+
+
+def _deriv_two_lstsq(m):
+    """derivTwo of init_2nd_deriv (init_functions.cc:83-328) formed
+    independently: the cell and its neighbours placed on the tangent plane by
+    azimuth and great-circle distance (azimuthal equidistant projection), the
+    quadratic fitted by numpy.linalg.lstsq on column-scaled points, and its
+    second derivative along the direction of each edge's midpoint,
+    2 cos^2 b_xx + 2 cos sin b_xy + 2 sin^2 b_yy, as the weights of the n + 1
+    cell values."""
+    R = m.radius
+    pcs = _unit(m["xCell"], m["yCell"], m["zCell"])
+    mids = _edge_midpoints(m)
+    me = m.maxEdges
+    coc = m["cellsOnCell"].reshape(-1, me) - 1
+    eoc = m["edgesOnCell"].reshape(-1, me) - 1
+    c1 = np.sort(m["cellsOnEdge"].reshape(-1, 2) - 1, axis=1)[:, 0]
+    out = np.zeros((m.nEdges, 16, 2))
+    for c in range(m.nCells):
+        n = int((eoc[c] >= 0).sum())
+        pc = pcs[c]
+        east, north = _frame(pc)
+        q = pcs[coc[c, :n]]
+        th = _azimuth(pc, q, east, north)
+        dl = R * 2 * np.arcsin(np.linalg.norm(q - pc, axis=1) / 2)
+        xp, yp = dl * np.cos(th), dl * np.sin(th)
+        A = np.zeros((n + 1, 6))
+        A[0, 0] = 1.0
+        A[1:] = np.stack([np.ones(n), xp, yp, xp * xp, xp * yp, yp * yp], 1)
+        h = dl.mean()
+        S = np.array([1.0, h, h, h * h, h * h, h * h])
+        B = np.linalg.lstsq(A / S, np.eye(n + 1), rcond=None)[0] / S[:, None]
+        for i in range(n):
+            e = eoc[c, i]
+            te = _azimuth(pc, mids[e], east, north)
+            cs, sn = np.cos(te), np.sin(te)
+            out[e, :n + 1, 0 if c == c1[e] else 1] = (2 * cs * cs * B[3] + 2 * cs * sn * B[4] +
+                                                      2 * sn * sn * B[5])
+    return out
+
+
+def test_migs_is_the_matrix_inverse():
+    """migs (matrix.hh:137-179, Pang's scaled partial pivoting) against
+    numpy.linalg.inv: random matrices, and ones whose leading entries are
+    zero or tiny so that the pivots must move."""
+    import ctypes
+    from oracle.oracle import lib
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(17)
+    cases = [rng.standard_normal((n, n)) + n * np.eye(n) for n in (2, 3, 6, 7, 10)]
+    perm = np.eye(6)[[3, 0, 5, 1, 4, 2]]
+    cases.append(perm @ (rng.standard_normal((6, 6)) + 6 * np.eye(6)))  # pivots off the diagonal
+    cases.append(np.array([[0.0, 2.0, 1.0], [1.0, 1.0, 0.0], [3.0, 0.0, 1.0]]))  # zero pivot
+    tiny = rng.standard_normal((5, 5)) + 5 * np.eye(5)
+    tiny[0, 0] = 1e-14  # a pivot that elimination without row exchanges would divide by
+    cases.append(tiny)
+    scaled = rng.standard_normal((6, 6)) + 6 * np.eye(6)
+    cases.append(scaled * np.array([1, 1e3, 1e3, 1e6, 1e6, 1e6]))  # rows of unequal scale
+    for a in cases:
+        a = np.ascontiguousarray(a)
+        n = a.shape[0]
+        x = np.zeros((n, n))
+        assert lib().orc_migs(n, a.ctypes.data_as(P), x.ctypes.data_as(P)) == 0
+        ref = np.linalg.inv(a)
+        assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max(), n
+
+
+@pytest.mark.parametrize("level", [3, 4])
+def test_deriv_two_is_the_least_squares_fit(level):
+    """init_2nd_deriv's weights equal an independent least-squares quadratic
+    fit on the same tangent-plane points (numpy.linalg.lstsq), per edge and
+    side, to 1e-5 of the row's largest weight.  The remaining difference is
+    the conditioning of the reference's normal equations (a few 1e-7);
+    exchanging the sides, the x and y roles or the fit's rows moves the
+    weights by O(1)."""
+    m = mesh(level)
+    d2 = _omesh(m).setup_field("derivTwo").reshape(-1, 16, 2)
+    ref = _deriv_two_lstsq(m)
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (scale > 0).all()
+    err = np.abs(d2 - ref) / scale
+    assert err.max() < 1e-5, err.max()
+
+
+_K = np.array([[0.3, 0.8, 0.52], [0.9, -0.2, 0.4], [-0.1, 0.5, -0.86]])
+
+
+def _psi(p):
+    a, b, c = p @ _K[0], p @ _K[1], p @ _K[2]
+    return np.sin(3 * a) + b * b * c + np.cos(2 * c)
+
+
+def _psi_geodesic_d2(p, t, R):
+    """d2/ds2 of _psi along the great circle through p with unit tangent t:
+    (t^T H t - grad . p) / R^2 (the circle curves towards -p)."""
+    a, b, c = p @ _K[0], p @ _K[1], p @ _K[2]
+    g = ((3 * np.cos(3 * a))[:, None] * _K[0] + (2 * b * c)[:, None] * _K[1] +
+         (b * b - 2 * np.sin(2 * c))[:, None] * _K[2])
+    ta, tb, tc = t @ _K[0], t @ _K[1], t @ _K[2]
+    tht = -9 * np.sin(3 * a) * ta * ta + 2 * c * tb * tb + 4 * b * tb * tc - 4 * np.cos(2 * c) * tc * tc
+    return (tht - np.sum(g * p, 1)) / R ** 2
+
+
+@pytest.mark.parametrize("level", [4, 5])
+def test_third_order_flux_is_upwind_biased(level):
+    """The sign of advCoef_3rd (init_functions.cc:394-420): which cell's
+    curvature the third-order term weights more.  Skamarock & Gassmann
+    (2011, MWR 139, their eq. 11 with beta) write the edge flux as
+        F = u dv [ (psi1 + psi2)/2 - dc^2/12 (d2psi1 + d2psi2)
+                   + beta sign(u) dc^2/12 (d2psi2 - d2psi1) ],
+    u > 0 from cell 1 to cell 2: the upwind cell's curvature enters with
+    less weight (third-order upwind for beta = 1).  So the high-order flux
+    at F = +1 plus that at F = -1 is 2 beta dv dc^2/12 (d2psi2 - d2psi1),
+    d2psi the second derivative along the edge direction, here the
+    analytic one of a smooth field.  A c1 <-> c2 swap of advCoef_3rd's sign
+    gives the slope -1."""
+    m = mesh(level)
+    om = _omesh(m)
+    R, beta = m.radius, 0.25
+    pcs = _unit(m["xCell"], m["yCell"], m["zCell"])
+    mids = _edge_midpoints(m)
+    coe = np.sort(m["cellsOnEdge"].reshape(-1, 2) - 1, axis=1)
+
+    def d2(p):
+        t = mids - np.sum(mids * p, 1, keepdims=True) * p
+        return _psi_geodesic_d2(p, t / np.linalg.norm(t, axis=1, keepdims=True), R)
+
+    expect = d2(pcs[coe[:, 1]]) - d2(pcs[coe[:, 0]])
+    tr = _psi(pcs)
+    hp, hm = np.zeros(m.nEdges), np.zeros(m.nEdges)
+    om.call("orc_ocn_high_order", 1, 1, beta, np.ones(m.nEdges), tr, hp)
+    om.call("orc_ocn_high_order", 1, 1, beta, -np.ones(m.nEdges), tr, hm)
+    got = (hp + hm) / (2 * beta * m["dvEdge"] * m["dcEdge"] ** 2 / 12)
+    slope = np.sum(got * expect) / np.sum(expect * expect)
+    corr = np.corrcoef(got, expect)[0, 1]
+    rms = np.sqrt(np.mean((got - expect) ** 2) / np.mean(expect ** 2))
+    assert 0.98 < slope < 1.02 and corr > 0.999 and rms < 0.04, (slope, corr, rms)
+
+
 def _edge_interp_error(level, corrected=True):
     m = mesh(level)
     om = _omesh(m)
