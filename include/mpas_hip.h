@@ -40,7 +40,8 @@ enum mpas_status {
   MPAS_EHIP = -2,   /* HIP runtime error (no device, launch failure, ...) */
   MPAS_ESHAPE = -3, /* buffer size / shape mismatch */
   MPAS_ESTATE = -4, /* call out of order (e.g. step before state upload) */
-  MPAS_ENOMEM = -5  /* device allocation failed */
+  MPAS_ENOMEM = -5, /* device allocation failed */
+  MPAS_ENONFINITE = -6 /* a state holds NaN or Inf (uploads, mpas_sw_state_check) */
 };
 
 /* Tracer transport inside the shallow-water RK4 stage. */
@@ -285,6 +286,17 @@ int mpas_sw_state_upload(mpas_ctx *ctx, const double *h, const double *u,
                          const double *tracers, const double *h_s);
 int mpas_sw_state_download(mpas_ctx *ctx, double *h, double *u,
                            double *tracers);
+/* Non-finite values: the uploads (mpas_sw_state_upload, mpas_batch_upload)
+   refuse h, u, tracers or h_s holding NaN or Inf with MPAS_ENONFINITE,
+   naming the field and the first index, as the reference's HDF5 reads abort
+   a conversion of NaN or Inf (mpasoflecsi/specialization/definition.hh:
+   165-191).  The task and step kernels do not test values: NaN and Inf
+   propagate through them by IEEE arithmetic as through the reference's task
+   loops (apps/sw/tasks/solve.cc; apps/sw/finalize.cc:24-25 only compares
+   the error).  mpas_sw_state_check counts the non-finite values of the
+   resident h, u and tracers on the device (count in *nonfinite, may be
+   NULL): 0 when every value is finite, else MPAS_ENONFINITE. */
+int mpas_sw_state_check(mpas_ctx *ctx, int64_t *nonfinite);
 /* Host copy of a field of the current state: "h", "u", "tracers",
    "h_edge", "ke", "pv_edge", "vorticity", "pv_vertex", "h_s". */
 int mpas_sw_field_download(mpas_ctx *ctx, const char *name, double *host,

@@ -694,6 +694,14 @@ int mpas_batch_upload(mpas_batch *B, int member, const double *h, const double *
   GroupInfo &G = B->groups[B->grp_of[member]];
   const mpas_ctx *ctx = B->meshes[G.ctx];
   const DMesh &M = ctx->dm;
+  {
+    int rc;
+    if ((rc = check_finite(h, size_t(M.nC), "mpas_batch_upload: h")) ||
+        (rc = check_finite(u, size_t(M.nE), "mpas_batch_upload: u")) ||
+        (rc = check_finite(h_s, size_t(M.nC), "mpas_batch_upload: h_s")) ||
+        (B->nT && (rc = check_finite(tracers, size_t(M.nC) * B->nT, "mpas_batch_upload: tracers"))))
+      return rc;
+  }
   if (G.h_s.empty()) {
     G.h_s.assign(h_s, h_s + M.nC);
     MPAS_HIP(hipMemcpyAsync(G.b.h_s, h_s, sizeof(double) * M.nC, hipMemcpyHostToDevice,

@@ -3,6 +3,7 @@
 // storage FleCSI owns in the reference, mpasoflecsi/specialization/mesh.cc:67-116,
 // apps/sw/state.hh:74-87).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -28,6 +29,25 @@ int after_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) MPAS_FAIL(MPAS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
   return 0;
+}
+
+int check_finite(const double *p, size_t n, const char *what) {
+  for (size_t i = 0; i < n; i++)
+    if (!std::isfinite(p[i]))
+      MPAS_FAIL(MPAS_ENONFINITE, std::string(what) + "[" + std::to_string(i) + "] is " +
+                                     (std::isnan(p[i]) ? "NaN" : "infinite"));
+  return 0;
+}
+
+// Counts the non-finite values of p[0, n): one atomic per wave that holds any.
+__global__ __launch_bounds__(256) void k_count_nonfinite(const double *__restrict__ p, size_t n,
+                                                         unsigned long long *count) {
+  for (size_t i0 = size_t(blockIdx.x) * blockDim.x; i0 < n; i0 += size_t(gridDim.x) * blockDim.x) {
+    const size_t i = i0 + threadIdx.x;
+    const bool bad = i < n && !__builtin_isfinite(p[i]);
+    const unsigned long long b = __ballot(bad);
+    if ((threadIdx.x & 63u) == 0 && b) atomicAdd(count, (unsigned long long)__popcll(b));
+  }
 }
 
 namespace {
@@ -257,6 +277,9 @@ int alloc_state(mpas_ctx *ctx) {
       (rc = alloc_field(ctx, V, &W->pv_vertex)) || (rc = alloc_field(ctx, size_t(M.nC), &W->h_s)) ||
       (rc = alloc_field(ctx, E * M.nT, &W->anti)) || (rc = alloc_field(ctx, 2 * T, &W->rfac)))
     return rc;
+  double *cnt = nullptr;
+  if ((rc = alloc_field(ctx, 1, &cnt))) return rc;
+  W->nonfinite = reinterpret_cast<unsigned long long *>(cnt);
   ctx->sw = std::move(W);
   return 0;
 }
@@ -470,6 +493,15 @@ int mpas_sw_state_upload(mpas_ctx *ctx, const double *h, const double *u, const 
   if (!h || !u || !h_s) MPAS_FAIL(MPAS_EINVAL, "mpas_sw_state_upload: h, u and h_s are required");
   const DMesh &M = ctx->dm;
   if (M.nT && !tracers) MPAS_FAIL(MPAS_EINVAL, "mpas_sw_state_upload: tracers is NULL");
+  {
+    const size_t C = size_t(M.nC) * M.nL, E = size_t(M.nE) * M.nL;
+    int rc;
+    if ((rc = check_finite(h, C, "mpas_sw_state_upload: h")) ||
+        (rc = check_finite(u, E, "mpas_sw_state_upload: u")) ||
+        (rc = check_finite(h_s, size_t(M.nC), "mpas_sw_state_upload: h_s")) ||
+        (M.nT && (rc = check_finite(tracers, C * M.nT, "mpas_sw_state_upload: tracers"))))
+      return rc;
+  }
   if (int rc = alloc_state(ctx)) return rc;
   SwState &W = *ctx->sw;
   const size_t cb = sizeof(double) * M.nC * M.nL, eb = sizeof(double) * M.nE * M.nL;
@@ -484,6 +516,29 @@ int mpas_sw_state_upload(mpas_ctx *ctx, const double *h, const double *u, const 
   if (int rc = sw_diagnostics(ctx, W.u, W.h, W.cur)) return rc;
   MPAS_HIP(hipStreamSynchronize(ctx->stream));
   W.ready = true;
+  return 0;
+}
+
+int mpas_sw_state_check(mpas_ctx *ctx, int64_t *nonfinite) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->sw || !ctx->sw->ready) MPAS_FAIL(MPAS_ESTATE, "no state uploaded");
+  const DMesh &M = ctx->dm;
+  SwState &W = *ctx->sw;
+  const size_t C = size_t(M.nC) * M.nL, E = size_t(M.nE) * M.nL;
+  MPAS_HIP(hipMemsetAsync(W.nonfinite, 0, sizeof(unsigned long long), ctx->stream));
+  const double *arr[3] = {W.h, W.u, W.tr};
+  const size_t len[3] = {C, E, M.nT ? C * M.nT : 0};
+  for (int a = 0; a < 3; a++) {
+    if (!len[a]) continue;
+    hipLaunchKernelGGL(k_count_nonfinite, dim3(std::min<size_t>((len[a] + 255) / 256, 4096)),
+                       dim3(256), 0, ctx->stream, arr[a], len[a], W.nonfinite);
+    if (int rc = after_launch("k_count_nonfinite")) return rc;
+  }
+  unsigned long long n = 0;
+  MPAS_HIP(hipMemcpyAsync(&n, W.nonfinite, sizeof n, hipMemcpyDeviceToHost, ctx->stream));
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  if (nonfinite) *nonfinite = int64_t(n);
+  if (n) MPAS_FAIL(MPAS_ENONFINITE, std::to_string(n) + " non-finite values in h, u and tracers");
   return 0;
 }
 

@@ -47,6 +47,7 @@ struct SwState {
   double *anti = nullptr;   // [E][nL][nT] antidiffusive flux
   double *rfac = nullptr;   // [C][nL][nT][2] limiter factors (in, out)
   double *tlo = nullptr;    // [C][nL][nT] upwind tendency
+  unsigned long long *nonfinite = nullptr;  // counter of mpas_sw_state_check
   int cur = 0;              // which ping-pong slot holds the current state
 };
 
@@ -104,6 +105,10 @@ int check_ctx(const mpas_ctx *ctx);
 
 // Kernel launch epilogue: reports launch errors as MPAS_EHIP.
 int after_launch(const char *what);
+
+// Fails with MPAS_ENONFINITE, naming `what` and the index, when one of the
+// n host values is NaN or Inf.
+int check_finite(const double *p, size_t n, const char *what);
 
 // Event bracket of one launch on stream st while profiling is on (no-ops
 // otherwise, and for a null context; st = null means the context's stream).

@@ -21,7 +21,7 @@ P_i64 = ctypes.POINTER(ctypes.c_int64)
 TRACER_NONE, TRACER_CENTRED, TRACER_FCT = 0, 1, 2
 BATCH_GRAPH, BATCH_LAUNCH = 0, 1
 STATUS = {0: "MPAS_OK", -1: "MPAS_EINVAL", -2: "MPAS_EHIP", -3: "MPAS_ESHAPE",
-          -4: "MPAS_ESTATE", -5: "MPAS_ENOMEM"}
+          -4: "MPAS_ESTATE", -5: "MPAS_ENOMEM", -6: "MPAS_ENONFINITE"}
 
 
 class MeshDesc(ctypes.Structure):
@@ -128,6 +128,7 @@ SIGNATURES = {
     "mpas_fct_update_tracers": (c_int, [c_void_p] + [_D] * 2),
     "mpas_sw_state_upload": (c_int, [c_void_p, P_f64, P_f64, P_f64, P_f64]),
     "mpas_sw_state_download": (c_int, [c_void_p, P_f64, P_f64, P_f64]),
+    "mpas_sw_state_check": (c_int, [c_void_p, ctypes.POINTER(c_int64)]),
     "mpas_sw_field_download": (c_int, [c_void_p, c_char_p, P_f64, c_int64]),
     "mpas_rk4_step": (c_int, [c_void_p, c_double, c_int]),
     "mpas_ocean_rk4_step": (c_int, [c_void_p, c_double, c_int]),

@@ -305,6 +305,15 @@ class Context:
                                              tr.ctypes.data_as(L.P_f64) if tr is not None else None,
                                              h_s.ctypes.data_as(L.P_f64)), "mpas_sw_state_upload")
 
+    def state_check(self):
+        """mpas_sw_state_check: the number of non-finite values in the
+        resident h, u and tracers (0 when every value is finite)."""
+        n = ctypes.c_int64(-1)
+        rc = L.hip().mpas_sw_state_check(self._h, ctypes.byref(n))
+        if rc not in (0, -6):
+            L.check(rc, "mpas_sw_state_check")
+        return n.value
+
     def sw_download(self):
         h = np.empty(self.nCells * self.nLevels)
         u = np.empty(self.nEdges * self.nLevels)
