@@ -84,12 +84,33 @@ struct AdjState {
          *hbf = nullptr, *vT = nullptr, *hvT = nullptr, *keT = nullptr, *divT = nullptr;
   AdjMesh T{};
   std::vector<void *> bufs;
+  // A second tape with its own replay buffers: the backward sweep replays
+  // segment m - 1 into it on a second stream while it sweeps segment m from
+  // the first (the replay needs only the checkpoint; the sweep reads only the
+  // tape and the adjoint buffers), then the two swap.  Allocated at the
+  // first sweep of more than one segment when the device has the memory;
+  // without it the replays run in turn on the context's stream.
+  struct Tape {
+    std::vector<Rec> rec;
+    double *pv_vertex = nullptr, *nu = nullptr, *nh = nullptr, *ntr = nullptr,
+           *tlo = nullptr;
+  } spare;
+  int spare_state = 0;  // 0 not tried, 1 allocated, -1 not enough memory
+  hipStream_t replay_stream = nullptr;
+  hipEvent_t ev_swept = nullptr, ev_replayed = nullptr;
 };
 
 void adj_free(mpas_ctx *ctx) {
   if (!ctx->adj) return;
-  for (void *p : ctx->adj->bufs) (void)hipFree(p);
-  delete ctx->adj;
+  AdjState *A = ctx->adj;
+  if (A->replay_stream) {
+    (void)hipStreamSynchronize(A->replay_stream);
+    (void)hipStreamDestroy(A->replay_stream);
+  }
+  if (A->ev_swept) (void)hipEventDestroy(A->ev_swept);
+  if (A->ev_replayed) (void)hipEventDestroy(A->ev_replayed);
+  for (void *p : A->bufs) (void)hipFree(p);
+  delete A;
   ctx->adj = nullptr;
 }
 
@@ -835,29 +856,58 @@ int adj_alloc(mpas_ctx *ctx, int K) {
   return 0;
 }
 
-int copy_d2d(mpas_ctx *ctx, double *dst, const double *src, size_t n) {
+int copy_d2d(mpas_ctx *ctx, double *dst, const double *src, size_t n,
+             hipStream_t st = nullptr) {
   if (!n) return 0;
-  MPAS_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  MPAS_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                          st ? st : ctx->stream));
   return 0;
 }
 
-// State (u, h, tr) into record r and the diagnostics of it.
-int restore(mpas_ctx *ctx, AdjState &A, Rec &r, const double *u, const double *h,
-            const double *tr) {
+// The tape a replay writes: its records and its own replay buffers.
+struct TapeView {
+  std::vector<Rec> *rec;
+  double *pv_vertex, *nu, *nh, *ntr, *tlo;
+};
+TapeView primary(AdjState &A) { return {&A.rec, A.pv_vertex, A.nu, A.nh, A.ntr, A.tlo}; }
+TapeView spare(AdjState &A) {
+  auto &t = A.spare;
+  return {&t.rec, t.pv_vertex, t.nu, t.nh, t.ntr, t.tlo};
+}
+void swap_tapes(AdjState &A) {
+  std::swap(A.rec, A.spare.rec);
+  std::swap(A.pv_vertex, A.spare.pv_vertex);
+  std::swap(A.nu, A.spare.nu);
+  std::swap(A.nh, A.spare.nh);
+  std::swap(A.ntr, A.spare.ntr);
+  std::swap(A.tlo, A.spare.tlo);
+}
+
+// State (u, h, tr) into record r and the diagnostics of it, on stream st
+// (nullptr: the context's stream).
+int restore(mpas_ctx *ctx, const TapeView &tv, Rec &r, const double *u, const double *h,
+            const double *tr, hipStream_t st = nullptr) {
   const DMesh &M = ctx->dm;
   const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL;
   int rc;
-  if ((rc = copy_d2d(ctx, r.u, u, E)) || (rc = copy_d2d(ctx, r.h, h, C)) ||
-      (rc = copy_d2d(ctx, r.tr, tr, C * M.nT)))
+  if ((rc = copy_d2d(ctx, r.u, u, E, st)) || (rc = copy_d2d(ctx, r.h, h, C, st)) ||
+      (rc = copy_d2d(ctx, r.tr, tr, C * M.nT, st)))
     return rc;
-  return diagnostics_into(ctx, r.u, r.h, r.ke, r.div, r.vort, A.pv_vertex, r.he, r.pve);
+  return diagnostics_dm(M, st ? st : ctx->stream, ctx->params.visc2 != 0.0, r.u, r.h, r.ke,
+                        r.div, r.vort, tv.pv_vertex, r.he, r.pve);
+}
+int restore(mpas_ctx *ctx, AdjState &A, Rec &r, const double *u, const double *h,
+            const double *tr) {
+  return restore(ctx, primary(A), r, u, h, tr);
 }
 
-// One forward step from record 4i into records 4i+1 .. 4i+4.
-int record_step(mpas_ctx *ctx, AdjState &A, int i, double dt) {
-  const Rec &r0 = A.rec[4 * i];
+// One forward step from record 4i into records 4i+1 .. 4i+4 of the tape,
+// on stream st (nullptr: the context's stream, with profiling).
+int record_step(mpas_ctx *ctx, const TapeView &tv, int i, double dt, hipStream_t st = nullptr) {
+  std::vector<Rec> &rec = *tv.rec;
+  const Rec &r0 = rec[4 * i];
   for (int s = 0; s < 4; s++) {
-    const Rec &x = A.rec[4 * i + s], &y = A.rec[4 * i + s + 1];
+    const Rec &x = rec[4 * i + s], &y = rec[4 * i + s + 1];
     StageIO S = stage_params(ctx, s, dt);
     S.xu = x.u;
     S.xh = x.h;
@@ -871,9 +921,9 @@ int record_step(mpas_ctx *ctx, AdjState &A, int i, double dt) {
     S.u0 = r0.u;
     S.h0 = r0.h;
     S.tr0 = r0.tr;
-    S.nu = A.nu;
-    S.nh = A.nh;
-    S.ntr = A.ntr;
+    S.nu = tv.nu;
+    S.nh = tv.nh;
+    S.ntr = tv.ntr;
     S.yu = y.u;
     S.yh = y.h;
     S.ytr = y.tr;
@@ -881,9 +931,52 @@ int record_step(mpas_ctx *ctx, AdjState &A, int i, double dt) {
     S.div_y = y.div;
     S.anti = x.anti;
     S.rfac = x.rfac;
-    S.tlo = A.tlo;
-    if (int rc = launch_stage(ctx, S, y.vort, A.pv_vertex, y.he, y.pve)) return rc;
+    S.tlo = tv.tlo;
+    const int rc = st ? launch_stage_dm(ctx->dm, st, S, y.vort, tv.pv_vertex, y.he, y.pve)
+                      : launch_stage(ctx, S, y.vort, tv.pv_vertex, y.he, y.pve);
+    if (rc) return rc;
   }
+  return 0;
+}
+int record_step(mpas_ctx *ctx, AdjState &A, int i, double dt) {
+  return record_step(ctx, primary(A), i, dt);
+}
+
+// Replays checkpoint segment m (len steps) into a tape.
+int replay(mpas_ctx *ctx, AdjState &A, const TapeView &tv, int m, int len,
+           hipStream_t st = nullptr) {
+  if (int rc = restore(ctx, tv, (*tv.rec)[0], A.ck[3 * m], A.ck[3 * m + 1], A.ck[3 * m + 2], st))
+    return rc;
+  for (int i = 0; i < len; i++)
+    if (int rc = record_step(ctx, tv, i, A.dt, st)) return rc;
+  return 0;
+}
+
+// The spare tape, its stream and events, when the device has room for them
+// (with 8 GiB to spare); returns 0 whether or not it was allocated.
+int try_spare(mpas_ctx *ctx, AdjState &A) {
+  if (A.spare_state) return 0;
+  A.spare_state = -1;
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
+  const size_t T = C * M.nT;
+  const size_t per_rec = 3 * E + 3 * C + V + T + 2 * T + E * M.nT;
+  const size_t need = 8 * (per_rec * A.rec.size() + V + E + C + 2 * T);
+  size_t free_b = 0, total_b = 0;
+  MPAS_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (free_b < need + (size_t(8) << 30)) return 0;
+  A.spare.rec.resize(A.rec.size());
+  int rc;
+  for (Rec &r : A.spare.rec)
+    if ((rc = alloc_rec(A, M, r))) return rc;
+  if ((rc = dalloc(A, V, &A.spare.pv_vertex)) || (rc = dalloc(A, E, &A.spare.nu)) ||
+      (rc = dalloc(A, C, &A.spare.nh)) || (rc = dalloc(A, T, &A.spare.ntr)) ||
+      (rc = dalloc(A, T, &A.spare.tlo)))
+    return rc;
+  MPAS_HIP(hipStreamCreateWithFlags(&A.replay_stream, hipStreamNonBlocking));
+  MPAS_HIP(hipEventCreateWithFlags(&A.ev_swept, hipEventDisableTiming));
+  MPAS_HIP(hipEventCreateWithFlags(&A.ev_replayed, hipEventDisableTiming));
+  A.spare_state = 1;
   return 0;
 }
 
@@ -1074,20 +1167,41 @@ int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
       (rc = copy_d2d(ctx, A.xb_tr[0], tr1_bar, T)))
     return rc;
   const int nseg = (A.nsteps + A.K - 1) / A.K;
+  auto seg_len = [&](int m) { return std::min(A.K, A.nsteps - m * A.K); };
+  if (nseg > 1 && (rc = try_spare(ctx, A))) return rc;
+  const bool overlap = nseg > 1 && A.spare_state == 1;
+  hipStream_t rs = A.replay_stream;
+  if (overlap) {  // the replays read checkpoints the context's stream wrote
+    MPAS_HIP(hipEventRecord(A.ev_swept, ctx->stream));
+    MPAS_HIP(hipStreamWaitEvent(rs, A.ev_swept, 0));
+  }
+  if (A.tape_seg != nseg - 1) {
+    if ((rc = replay(ctx, A, primary(A), nseg - 1, seg_len(nseg - 1)))) return rc;
+    A.tape_seg = nseg - 1;
+  }
   for (int m = nseg - 1; m >= 0; m--) {
-    const int len = std::min(A.K, A.nsteps - m * A.K);
-    if (m != A.tape_seg) {  // replay the segment into the tape
-      if ((rc = restore(ctx, A, A.rec[0], A.ck[3 * m], A.ck[3 * m + 1], A.ck[3 * m + 2])))
-        return rc;
-      for (int i = 0; i < len; i++)
-        if ((rc = record_step(ctx, A, i, A.dt))) return rc;
+    const int len = seg_len(m);
+    if (m != A.tape_seg) {  // no spare tape: replay the segment into the tape
+      if ((rc = replay(ctx, A, primary(A), m, len))) return rc;
       A.tape_seg = m;
     }
+    // the next segment's replay runs beside this segment's sweep; the spare
+    // tape's last reader (the previous sweep) finished before ev_swept
+    const bool ahead = overlap && m > 0;
+    if (ahead && (rc = replay(ctx, A, spare(A), m - 1, seg_len(m - 1), rs))) return rc;
+    if (ahead) MPAS_HIP(hipEventRecord(A.ev_replayed, rs));
     for (int i = len - 1; i >= 0; i--) {
       if ((rc = adj_step(ctx, A, i, A.dt, src))) return rc;
       src = 1 - src;
     }
     A.taped = len;
+    if (ahead) {
+      MPAS_HIP(hipEventRecord(A.ev_swept, ctx->stream));
+      MPAS_HIP(hipStreamWaitEvent(rs, A.ev_swept, 0));
+      MPAS_HIP(hipStreamWaitEvent(ctx->stream, A.ev_replayed, 0));
+      swap_tapes(A);
+      A.tape_seg = m - 1;
+    }
   }
   if ((rc = copy_d2d(ctx, u0_bar, A.xb_u[src], E)) || (rc = copy_d2d(ctx, h0_bar, A.xb_h[src], C)) ||
       (rc = copy_d2d(ctx, tr0_bar, A.xb_tr[src], T)))
