@@ -378,9 +378,15 @@ int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0,
    evolves exactly as mpas_rk4_step would evolve it alone. */
 typedef struct mpas_batch mpas_batch;
 enum {
-  MPAS_BATCH_GRAPH = 0, /* one launch per phase for all groups, the step
-                           captured in a hipGraph and replayed */
-  MPAS_BATCH_LAUNCH = 1 /* the same launches, issued directly */
+  MPAS_BATCH_GRAPH = 0,      /* one launch per phase for all groups, the step
+                                captured in a hipGraph and replayed */
+  MPAS_BATCH_LAUNCH = 1,     /* the same launches, issued directly */
+  MPAS_BATCH_PERSISTENT = 2, /* every phase of up to 20 steps in one
+                                cooperative launch, grid barriers between
+                                phases (meshes of at most 6 edges per cell,
+                                0, 1, 2 or 4 FCT tracers) */
+  MPAS_BATCH_TUNED = 3       /* the mode mpas_batch_autotune found fastest
+                                (MPAS_BATCH_GRAPH before it ran) */
 };
 int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers,
                       const int32_t *mesh_of, const double *dt, mpas_batch **out);
@@ -415,8 +421,11 @@ int mpas_batch_get_layout_blocks(const mpas_batch *batch, int32_t *blocks);
 int mpas_batch_set_layout_blocks(mpas_batch *batch, const int32_t *blocks);
 /* Times trial_steps graph replays per candidate (entity per wave and
    members per lane, placement and order, then each kernel's workgroup size,
-   then that size per layout), keeps the fastest configuration and restores
-   the members' state; writes the trials as JSON into report. */
+   then that size per layout), keeps the fastest configuration, then times
+   the persistent launch against the graph (mpas_batch_tuned_mode returns
+   the faster), and restores the members' state; writes the trials as JSON
+   into report. */
+int mpas_batch_tuned_mode(const mpas_batch *batch);
 int mpas_batch_autotune(mpas_batch *batch, int trial_steps, char *report, size_t cap);
 
 #ifdef __cplusplus

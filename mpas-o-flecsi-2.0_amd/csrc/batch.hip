@@ -25,6 +25,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "ctx.h"
 #include "stage_kernels.h"
 
@@ -92,6 +94,12 @@ struct mpas_batch {
   // group of that layout
   int2 *d_tab[mpasx::kBatchKernels][mpasx::kBatchModes] = {};
   int grid[mpasx::kBatchKernels][mpasx::kBatchModes] = {};
+  // the persistent launch's tables: per kernel, (layout, group, block) of
+  // blocks of 256 items over every layout
+  int4 *d_ptab[mpasx::kBatchKernels] = {};
+  int pn[mpasx::kBatchKernels] = {};
+  int cus = 0;          // compute units of the device
+  int tuned_mode = 0;   // the launch mode the autotuner found fastest
   bool tables_ok = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -152,37 +160,59 @@ __device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
 #ifndef MPAS_BATCH_QCH
 #define MPAS_BATCH_QCH 4
 #endif
+// The phase bodies of layout MODE for one work item of group g, stage s.
 template <int WE, int MODE>
-__global__ __launch_bounds__(kBatchMaxBlock) MPAS_BATCH_EDGES_WAVES void
-kb_stage_edges(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
+__device__ __forceinline__ void run_edges(const BatchGroup &g, int s, int item) {
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
   stage_edges<WE, lol_of(MODE), lp_of(MODE), fix_of(MODE), MPAS_BATCH_QCH>(M, S, item);
 }
 template <int NT, int WA, int MODE>
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
-  BATCH_PROLOGUE;
+__device__ __forceinline__ void run_ho(const BatchGroup &g, int s, int item) {
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
   stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE), fix_of(MODE)>(M, S, item);
 }
 template <int NT, int W, int MODE, int SCH>
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
-  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
-    BATCH_PROLOGUE;
+__device__ __forceinline__ void run_cells(const BatchGroup &g, int s, int item) {
+  if constexpr (lp_of(MODE) == 1) {  // never run with pairs
     const DMesh M = g.M;
     const StageIO S = g.S[s];
     stage_cells<NT, W, lol_of(MODE), 0, SCH, fix_of(MODE)>(M, S, item);
   }
 }
 template <int NT, int W, int MODE>
-__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
-  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
-    BATCH_PROLOGUE;
+__device__ __forceinline__ void run_tracer(const BatchGroup &g, int s, int item) {
+  if constexpr (lp_of(MODE) == 1) {  // never run with pairs
     const DMesh M = g.M;
     const StageIO S = g.S[s];
     stage_tracer<NT, W, lol_of(MODE), 0, fix_of(MODE)>(M, S, item);
+  }
+}
+
+template <int WE, int MODE>
+__global__ __launch_bounds__(kBatchMaxBlock) MPAS_BATCH_EDGES_WAVES void
+kb_stage_edges(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  run_edges<WE, MODE>(g, s, item);
+}
+template <int NT, int WA, int MODE>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_ho(BatchArgs A, int s) {
+  BATCH_PROLOGUE;
+  run_ho<NT, WA, MODE>(g, s, item);
+}
+template <int NT, int W, int MODE, int SCH>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_cells(BatchArgs A, int s) {
+  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
+    BATCH_PROLOGUE;
+    run_cells<NT, W, MODE, SCH>(g, s, item);
+  }
+}
+template <int NT, int W, int MODE>
+__global__ __launch_bounds__(kBatchMaxBlock) void kb_stage_tracer(BatchArgs A, int s) {
+  if constexpr (lp_of(MODE) == 1) {  // never launched with pairs
+    BATCH_PROLOGUE;
+    run_tracer<NT, W, MODE>(g, s, item);
   }
 }
 template <int MODE>
@@ -217,6 +247,92 @@ __global__ __launch_bounds__(kBatchMaxBlock) void kb_diag_edge(BatchArgs A, int 
   diag_edge_lanes<MODE>(g, s, item);
 }
 #undef BATCH_PROLOGUE
+
+// ---- one persistent launch ----------------------------------------------------
+// Every phase of every stage of nsteps steps in one cooperative launch: the
+// workgroups walk phase k's work table (layout, group, block of kPBlock
+// items) block-stride, then wait at a grid barrier for the phase's results.
+// The bodies are those of the phase kernels above, so every member evolves
+// exactly as with the per-phase launches.  Instantiated for the shapes of
+// the ensemble workloads (meshes of at most 6 edges per cell, 0, 1, 2 or 4
+// tracers, FCT).
+constexpr int kPBlock = 256;
+#if MPAS_PERSIST_WPE > 0
+#define MPAS_PERSIST_WAVES __attribute__((amdgpu_waves_per_eu(MPAS_PERSIST_WPE, 8)))
+#else
+#define MPAS_PERSIST_WAVES
+#endif
+struct PersistArgs {
+  const BatchGroup *__restrict__ grp;
+  const int4 *__restrict__ tab[kBatchKernels];  // (mode, group, block, 0)
+  int n[kBatchKernels];                          // entries of each table
+  int xcd;                                       // XCD-contiguous walk
+};
+
+template <int NT, int W, int WE, int SCH>
+__device__ __forceinline__ void run_phase(int k, int mode, const BatchGroup &g, int s, int item) {
+#define MPAS_MODES(call)                  \
+  switch (mode) {                         \
+    case 1: { constexpr int MODE = 1; call; } break; \
+    case 2: { constexpr int MODE = 2; call; } break; \
+    case 3: { constexpr int MODE = 3; call; } break; \
+    case 4: { constexpr int MODE = 4; call; } break; \
+    case 5: { constexpr int MODE = 5; call; } break; \
+    default: { constexpr int MODE = 6; call; } break; \
+  }
+  switch (k) {
+    case 0: MPAS_MODES((run_edges<WE, MODE>(g, s, item))) break;
+    case 1:
+      if constexpr (NT > 0) MPAS_MODES((run_ho<NT, WE, MODE>(g, s, item)))
+      break;
+    case 2: MPAS_MODES((run_cells<NT, W, MODE, SCH>(g, s, item))) break;
+    case 3:
+      if constexpr (NT > 0) MPAS_MODES((run_tracer<NT, W, MODE>(g, s, item)))
+      break;
+    case 4: MPAS_MODES((diag_vertex_lanes<MODE>(g, s, item))) break;
+    default: MPAS_MODES((diag_edge_lanes<MODE>(g, s, item))) break;
+  }
+#undef MPAS_MODES
+}
+
+// Waves per SIMD the persistent kernel is compiled for (its registers are
+// the largest body's; 0: the compiler's choice).
+#ifndef MPAS_PERSIST_WPE
+#define MPAS_PERSIST_WPE 2
+#endif
+template <int NT, int W, int WE, int SCH>
+__global__ __launch_bounds__(kPBlock) MPAS_PERSIST_WAVES void kb_persistent(PersistArgs P,
+                                                                           int nsteps) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  // XCD x (workgroups b with b % 8 == x) walks one contiguous range of each
+  // table, so neighbouring blocks share that XCD's L2
+  const int nb = int(gridDim.x), b = int(blockIdx.x);
+  const int per_xcd = nb / kXcds, x = b % kXcds, j = b / kXcds;
+  for (int step = 0; step < nsteps; step++)
+    for (int s = 0; s < 4; s++)
+      for (int k = 0; k < kBatchKernels; k++) {
+        const int n = P.n[k];
+        if (!n) continue;
+        int hi = n, first = b, stride = nb;
+        if (P.xcd && per_xcd > 0) {
+          if (b < per_xcd * kXcds) {
+            const int chunk = (n + kXcds - 1) / kXcds;
+            first = x * chunk + j;
+            hi = min(n, x * chunk + chunk);
+            stride = per_xcd;
+          } else {
+            first = n;  // workgroups past whole rounds of 8 stay idle
+          }
+        }
+        for (int lb = first; lb < hi; lb += stride) {
+          const int4 t = P.tab[k][lb];
+          if (t.y < 0) continue;
+          run_phase<NT, W, WE, SCH>(k, t.x, P.grp[t.y], s, t.z * kPBlock + int(threadIdx.x));
+        }
+        grid.sync();
+      }
+}
 
 // One member's field between its contiguous copy (n entities x w doubles)
 // and the group layout (n entities x members x w doubles).
@@ -295,12 +411,16 @@ int block_of(const mpas_batch *B, int k, int mode) {
 }
 
 void free_tables(mpas_batch *B) {
-  for (int k = 0; k < kBatchKernels; k++)
+  for (int k = 0; k < kBatchKernels; k++) {
     for (int m = 0; m < kBatchModes; m++) {
       if (B->d_tab[k][m]) (void)hipFree(B->d_tab[k][m]);
       B->d_tab[k][m] = nullptr;
       B->grid[k][m] = 0;
     }
+    if (B->d_ptab[k]) (void)hipFree(B->d_ptab[k]);
+    B->d_ptab[k] = nullptr;
+    B->pn[k] = 0;
+  }
   B->tables_ok = false;
 }
 
@@ -389,6 +509,22 @@ int build_tables(mpas_batch *B) {
       MPAS_HIP(hipMemcpy(B->d_tab[k][mode - 1], tab.data(), sizeof(int2) * tab.size(),
                          hipMemcpyHostToDevice));
     }
+  // the persistent launch: every layout of kernel k in one table, blocks
+  // of kPBlock items, each group's blocks together
+  for (int k = 0; k < kBatchKernels; k++) {
+    std::vector<int4> tab;
+    for (int mode = 1; mode <= kBatchModes; mode++)
+      for (int q = 0; q < ng; q++)
+        if (group_mode(B, q, k) == mode) {
+          const int nb = int((kernel_items(B, k, q) + kPBlock - 1) / kPBlock);
+          for (int j = 0; j < nb; j++) tab.push_back(make_int4(mode, q, j, 0));
+        }
+    if (tab.empty()) continue;
+    B->pn[k] = int(tab.size());
+    MPAS_HIP(hipMalloc(&B->d_ptab[k], sizeof(int4) * tab.size()));
+    MPAS_HIP(hipMemcpy(B->d_ptab[k], tab.data(), sizeof(int4) * tab.size(),
+                       hipMemcpyHostToDevice));
+  }
   B->tables_ok = true;
   return 0;
 }
@@ -465,6 +601,48 @@ int launch_batch_step(mpas_batch *B) {
   return 0;
 }
 
+// The persistent launch of nsteps steps (see kb_persistent).
+template <int NT, int SCH>
+int launch_persistent_t(mpas_batch *B, int nsteps) {
+  const auto kern = kb_persistent<NT, 6, 10, SCH>;
+  int per_cu = 0;
+  MPAS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kPBlock, 0));
+  if (per_cu < 1) MPAS_FAIL(MPAS_EHIP, "the persistent ensemble kernel does not fit a CU");
+  int grid = per_cu * B->cus;
+  if (B->xcd) grid = std::max(kXcds, grid / kXcds * kXcds);
+  PersistArgs P{};
+  P.grp = B->d_grp;
+  for (int k = 0; k < kBatchKernels; k++) {
+    P.tab[k] = B->d_ptab[k];
+    P.n[k] = B->pn[k];
+  }
+  P.xcd = B->xcd;
+  int n = nsteps;
+  void *args[] = {&P, &n};
+  MPAS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(grid),
+                                      dim3(kPBlock), args, 0, B->stream));
+  return 0;
+}
+
+// Whether the persistent launch is built for this ensemble's shapes.
+bool persistent_supported(const mpas_batch *B) {
+  const int sch = B->desc.empty() ? MPAS_TRACER_NONE : B->desc[0].S[0].scheme;
+  return B->narrow && B->cus > 0 &&
+         ((B->nT == 0) || ((B->nT == 1 || B->nT == 2 || B->nT == 4) && sch == MPAS_TRACER_FCT));
+}
+
+int launch_persistent(mpas_batch *B, int nsteps) {
+  if (!persistent_supported(B))
+    MPAS_FAIL(MPAS_EINVAL, "the persistent launch serves meshes of at most 6 edges per cell "
+                           "with 0, 1, 2 or 4 FCT tracers (use MPAS_BATCH_GRAPH)");
+  switch (B->nT) {
+    case 0: return launch_persistent_t<0, MPAS_TRACER_NONE>(B, nsteps);
+    case 1: return launch_persistent_t<1, MPAS_TRACER_FCT>(B, nsteps);
+    case 2: return launch_persistent_t<2, MPAS_TRACER_FCT>(B, nsteps);
+    default: return launch_persistent_t<4, MPAS_TRACER_FCT>(B, nsteps);
+  }
+}
+
 int capture_graph(mpas_batch *B) {
   drop_graph(B);
   MPAS_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -524,6 +702,16 @@ void release_batch(mpas_batch *B) {
 int run_steps(mpas_batch *B, int nsteps, int mode) {
   if (!B->tables_ok)
     if (int rc = build_tables(B)) return rc;
+  if (mode == MPAS_BATCH_TUNED) mode = B->tuned_mode;
+  if (mode == MPAS_BATCH_PERSISTENT) {
+    // launches of at most 20 steps (each some tens of milliseconds)
+    for (int done = 0; done < nsteps;) {
+      const int n = std::min(20, nsteps - done);
+      if (int rc = launch_persistent(B, n)) return rc;
+      done += n;
+    }
+    return 0;
+  }
   if (mode == MPAS_BATCH_LAUNCH) {
     for (int step = 0; step < nsteps; step++)
       if (int rc = launch_batch_step(B)) return rc;
@@ -535,12 +723,12 @@ int run_steps(mpas_batch *B, int nsteps, int mode) {
   return 0;
 }
 
-double time_steps(mpas_batch *B, int nsteps, int *rc) {
-  *rc = run_steps(B, 1, MPAS_BATCH_GRAPH);  // capture + warm
+double time_steps(mpas_batch *B, int nsteps, int *rc, int mode = MPAS_BATCH_GRAPH) {
+  *rc = run_steps(B, 1, mode);  // capture + warm
   if (!*rc && hipStreamSynchronize(B->stream) != hipSuccess) *rc = MPAS_EHIP;
   if (*rc) return 0;
   const auto t0 = std::chrono::steady_clock::now();
-  *rc = run_steps(B, nsteps, MPAS_BATCH_GRAPH);
+  *rc = run_steps(B, nsteps, mode);
   if (!*rc && hipStreamSynchronize(B->stream) != hipSuccess) *rc = MPAS_EHIP;
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -590,6 +778,12 @@ int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers, const 
   }
   const mpas_ctx *c0 = meshes[0];
   B->device = c0->device;
+  {
+    int coop = 0;
+    MPAS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, B->device));
+    if (coop)
+      MPAS_HIP(hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, B->device));
+  }
   B->nT = c0->dm.nT;
   B->narrow = c0->dm.narrow;
   for (mpas_ctx *c : B->meshes) {
@@ -743,7 +937,7 @@ int mpas_batch_download(mpas_batch *B, int member, double *h, double *u, double 
 
 int mpas_batch_step(mpas_batch *B, int nsteps, int mode) {
   if (int rc = check_batch(B)) return rc;
-  if (mode != MPAS_BATCH_GRAPH && mode != MPAS_BATCH_LAUNCH)
+  if (mode < MPAS_BATCH_GRAPH || mode > MPAS_BATCH_TUNED)
     MPAS_FAIL(MPAS_EINVAL, "mpas_batch_step: unknown mode");
   for (int m = 0; m < B->nmem; m++)
     if (!B->ready[m]) MPAS_FAIL(MPAS_ESTATE, "member " + std::to_string(m) + " has no state");
@@ -790,6 +984,11 @@ int mpas_batch_get_config(const mpas_batch *B, int32_t *block, int *members_per_
   if (xcd) *xcd = B->xcd;
   if (wave_entity) *wave_entity = B->wave_entity;
   return 0;
+}
+
+int mpas_batch_tuned_mode(const mpas_batch *B) {
+  if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
+  return B->tuned_mode;
 }
 
 int mpas_batch_get_layout_blocks(const mpas_batch *B, int32_t *blocks) {
@@ -903,6 +1102,20 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
         if (t < best) best = t, pick = bs;
       }
       B->mblock[k][m] = pick;
+    }
+  }
+  // then the launch: the graph of per-phase launches or one persistent launch
+  B->tuned_mode = MPAS_BATCH_GRAPH;
+  if (!rc && persistent_supported(B)) {
+    if (!(rc = build_tables(B))) {
+      const double t = time_steps(B, trial_steps, &rc, MPAS_BATCH_PERSISTENT);
+      char line[160];
+      std::snprintf(line, sizeof line,
+                    ", {\"trial\": \"persistent launch\", \"ms_per_step\": %.4f, "
+                    "\"graph_ms_per_step\": %.4f}",
+                    1e3 * t / trial_steps, 1e3 * best / trial_steps);
+      log += line;
+      if (!rc && t < best) B->tuned_mode = MPAS_BATCH_PERSISTENT;
     }
   }
   if (!rc) rc = build_tables(B);

@@ -19,7 +19,7 @@ P_f64 = ctypes.POINTER(ctypes.c_double)
 P_i64 = ctypes.POINTER(ctypes.c_int64)
 
 TRACER_NONE, TRACER_CENTRED, TRACER_FCT = 0, 1, 2
-BATCH_GRAPH, BATCH_LAUNCH = 0, 1
+BATCH_GRAPH, BATCH_LAUNCH, BATCH_PERSISTENT, BATCH_TUNED = 0, 1, 2, 3
 STATUS = {0: "MPAS_OK", -1: "MPAS_EINVAL", -2: "MPAS_EHIP", -3: "MPAS_ESHAPE",
           -4: "MPAS_ESTATE", -5: "MPAS_ENOMEM", -6: "MPAS_ENONFINITE"}
 
@@ -149,6 +149,7 @@ SIGNATURES = {
     "mpas_batch_config": (c_int, [c_void_p, P_i32, c_int, c_int, c_int, c_int]),
     "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 4),
     "mpas_batch_get_layout_blocks": (c_int, [c_void_p, P_i32]),
+    "mpas_batch_tuned_mode": (c_int, [c_void_p]),
     "mpas_batch_set_layout_blocks": (c_int, [c_void_p, P_i32]),
     "mpas_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
 }

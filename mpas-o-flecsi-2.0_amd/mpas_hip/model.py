@@ -485,6 +485,11 @@ class Batch:
         return {"block": blk.tolist(), "members_per_lane": lpl.value, "order": o.value,
                 "xcd": x.value, "wave_entity": we.value}
 
+    def tuned_mode(self):
+        """The launch mode the autotuner found fastest (BATCH_GRAPH or
+        BATCH_PERSISTENT)."""
+        return L.check_count(L.hip().mpas_batch_tuned_mode(self._b), "mpas_batch_tuned_mode")
+
     def layout_blocks(self):
         """Workgroup sizes per kernel (rows) and group layout (columns), 0 for
         the kernel's own size."""

@@ -19,14 +19,15 @@ DTS = (600.0, 300.0, 450.0, 300.0, 600.0, 600.0, 300.0)
 def _ensemble(levels=LEVELS, nT=2, dts=DTS):
     from mpas_hip import TRACER_FCT, Batch, Context, sw_workload_state
     lv = sorted(set(levels))
-    ctxs = [Context(mesh(L), nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    ctxs = [Context(mesh(L), nTracers=nT, tracer_scheme=TRACER_FCT if nT else 0, visc2=1e5,
+                    kappa2=1e3 if nT else 0.0)
             for L in lv]
     mesh_of = [lv.index(L) for L in levels]
     states = []
     for i, L in enumerate(levels):
         u, h, tr = sw_workload_state(mesh(L), nT, seed=10 + i, u0=10.0 + 3 * i,
                                      bump_amp=5.0 + 2 * i)
-        states.append((h, u, tr.ravel(), np.full(mesh(L).nCells, 4500.0)))
+        states.append((h, u, tr.ravel() if nT else None, np.full(mesh(L).nCells, 4500.0)))
     b = Batch(ctxs, mesh_of, dts)
     return ctxs, b, states, mesh_of
 
@@ -70,6 +71,38 @@ def test_members_equal_single_runs_both_modes():
         for i in range(b.n):
             for a, r in zip(b.download(i), ref[i]):
                 assert bitwise_equal(a, r), (mode, i)
+    _close(ctxs, b)
+
+
+@pytest.mark.parametrize("nT", [0, 2, 4])
+def test_persistent_launch_keeps_results(nT):
+    """Every phase of several steps in one cooperative launch (grid barriers
+    between phases): every member equals its single run bit for bit, in
+    several layouts and placements; the tuned mode is one of the two."""
+    from mpas_hip import BATCH_GRAPH, BATCH_PERSISTENT, BATCH_TUNED
+    levels = (3, 4, 3, 4, 4, 3, 4, 4, 5, 3, 5)  # groups of 4, 5 and 2 members
+    dts = tuple({3: 600.0, 4: 300.0, 5: 150.0}[L] for L in levels)
+    ctxs, b, states, mesh_of = _ensemble(levels, nT, dts)
+    ref = _singles(ctxs, states, mesh_of, 23, dts)  # more than one launch of 20 steps
+    for lpl, order, xcd, we in ((1, 0, 1, 1), (1, 1, 0, 0), (2, 0, 1, 2)):
+        b.config([256] * 6, lpl, order, xcd, we)
+        _load(b, states)
+        print(f"persistent: nT {nT}, members/lane {lpl}, order {order}, xcd {xcd}, "
+              f"entity/wave {we}", flush=True)
+        b.step(23, BATCH_PERSISTENT)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (nT, lpl, order, xcd, we, i)
+    _load(b, states)
+    b.autotune(trial_steps=2)
+    assert b.tuned_mode() in (BATCH_GRAPH, BATCH_PERSISTENT)
+    _load(b, states)
+    b.step(23, BATCH_TUNED)
+    b.sync()
+    for i in range(b.n):
+        for a, r in zip(b.download(i), ref[i]):
+            assert bitwise_equal(a, r), ("tuned", i)
     _close(ctxs, b)
 
 
@@ -158,7 +191,7 @@ def test_autotune_keeps_results_and_state():
     before = [b.download(i) for i in range(b.n)]
     trials = b.autotune(trial_steps=2)
     assert len(trials) >= 5 and all(t["ms_per_step"] > 0 for t in trials)
-    assert all(len(t["layout_block"]) == 36 for t in trials)
+    assert all(len(t["layout_block"]) == 36 for t in trials if "block" in t)
     print("tuned:", b.get_config())
     cfg, lb = b.get_config(), b.layout_blocks()
     # the tuned configuration is reproducible: config + layout blocks
