@@ -111,6 +111,21 @@ int mpas_ctx_sync(mpas_ctx *ctx);
 /* Sizes: out[0..4] = nCells, nEdges, nVertices, nLevels, nTracers. */
 int mpas_ctx_dims(const mpas_ctx *ctx, int64_t out[5]);
 
+/* Mesh orders (no counterpart in the reference, whose order is the file's):
+   a context with the entities of src renumbered, cells in the given order,
+   reverse Cuthill-McKee or along a 3-D Morton curve of the cell centres,
+   edges by their (lower, higher) new cell pair, vertices by their sorted
+   new cell triple.  Every list keeps its entries' order, and the sums the
+   reference forms in ascending edge id stay in the original ids' order, so
+   a state advanced on the reordered context equals, entity for entity, the
+   one advanced on src bit for bit.  Fields of a reordered context are in
+   its numbering; mpas_ctx_order gives the original id of each entity
+   (arrays of nCells, nEdges, nVertices; any may be NULL). */
+enum { MPAS_ORDER_GIVEN = 0, MPAS_ORDER_RCM = 1, MPAS_ORDER_MORTON = 2 };
+int mpas_ctx_create_reordered(const mpas_ctx *src, int order, mpas_ctx **out);
+int mpas_ctx_order(const mpas_ctx *ctx, int32_t *cell_of, int32_t *edge_of,
+                   int32_t *vertex_of);
+
 /* The context's current stream (hipStream_t). */
 void *mpas_ctx_stream(const mpas_ctx *ctx);
 
@@ -142,6 +157,11 @@ int mpas_buf_copy(mpas_ctx *ctx, void *dst, const void *src, size_t bytes);
 int mpas_host_mesh_create(const mpas_mesh_desc *mesh, const mpas_params *params,
                           mpas_host_mesh **out);
 int mpas_host_mesh_destroy(mpas_host_mesh *hm);
+/* The host mesh renumbered as mpas_ctx_create_reordered does (a new mesh,
+   freed with mpas_host_mesh_destroy), and the original id of each entity. */
+int mpas_host_mesh_reorder(const mpas_host_mesh *hm, int order, mpas_host_mesh **out);
+int mpas_host_mesh_order(const mpas_host_mesh *hm, int32_t *cell_of, int32_t *edge_of,
+                         int32_t *vertex_of);
 const mpas_host_mesh *mpas_ctx_host_mesh(const mpas_ctx *ctx);
 
 /* Host copies of the flattened connectivity, for checks.  which = "c2v",
@@ -426,6 +446,13 @@ int mpas_batch_set_layout_blocks(mpas_batch *batch, const int32_t *blocks);
    the faster), and restores the members' state; writes the trials as JSON
    into report. */
 int mpas_batch_tuned_mode(const mpas_batch *batch);
+/* Mesh order of the groups (MPAS_ORDER_*, as mpas_ctx_create_reordered):
+   the batch runs on reordered copies of the mesh contexts and permutes the
+   members' state; uploads and downloads keep the callers' numbering, and
+   every member's result is unchanged bit for bit.  The autotuner tries
+   every order first. */
+int mpas_batch_set_order(mpas_batch *batch, int order);
+int mpas_batch_get_order(const mpas_batch *batch);
 int mpas_batch_autotune(mpas_batch *batch, int trial_steps, char *report, size_t cap);
 
 #ifdef __cplusplus

@@ -100,6 +100,12 @@ struct mpas_batch {
   int pn[mpasx::kBatchKernels] = {};
   int cus = 0;          // compute units of the device
   int tuned_mode = 0;   // the launch mode the autotuner found fastest
+  // mesh order of the groups (MPAS_ORDER_*): per order and mesh, the context
+  // the groups run on (order 0: the caller's; others owned, created on first
+  // use) and, on the device, the original id of each cell and edge
+  int mesh_order = 0;
+  std::vector<mpas_ctx *> octx[3];
+  std::vector<int32_t *> d_cell_of[3], d_edge_of[3];
   bool tables_ok = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -336,19 +342,39 @@ __global__ __launch_bounds__(kPBlock) MPAS_PERSIST_WAVES void kb_persistent(Pers
 
 // One member's field between its contiguous copy (n entities x w doubles)
 // and the group layout (n entities x members x w doubles).
+// Entity e of the group's (possibly reordered) mesh is entity of[e] of the
+// caller's numbering (of = nullptr: the same).
 __global__ __launch_bounds__(256) void kb_member_scatter(double *group, const double *src,
-                                                         int64_t n, int w, int members, int idx) {
+                                                         int64_t n, int w, int members, int idx,
+                                                         const int32_t *of) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n * w) return;
   const int64_t e = i / w, j = i - e * w;
-  group[(e * members + idx) * w + j] = src[i];
+  const int64_t o = of ? int64_t(of[e]) : e;
+  group[(e * members + idx) * w + j] = src[o * w + j];
 }
 __global__ __launch_bounds__(256) void kb_member_gather(double *dst, const double *group,
-                                                        int64_t n, int w, int members, int idx) {
+                                                        int64_t n, int w, int members, int idx,
+                                                        const int32_t *of) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n * w) return;
   const int64_t e = i / w, j = i - e * w;
-  dst[i] = group[(e * members + idx) * w + j];
+  const int64_t o = of ? int64_t(of[e]) : e;
+  dst[o * w + j] = group[(e * members + idx) * w + j];
+}
+// Rows of r doubles from one entity order to another: dst row i = src row
+// map[i].
+__global__ __launch_bounds__(256) void kb_permute_rows(double *dst, const double *src, int64_t n,
+                                                       int64_t r, const int32_t *map) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * r) return;
+  const int64_t e = i / r, j = i - e * r;
+  dst[i] = src[int64_t(map[e]) * r + j];
+}
+
+// The context group q's mesh runs on in the batch's mesh order.
+mpas_ctx *mesh_ctx(const mpas_batch *B, int mesh) {
+  return B->mesh_order ? B->octx[B->mesh_order][mesh] : B->meshes[mesh];
 }
 
 int lanes_per_entity(int nT) { return nT > 0 ? nT / (nT % 2 == 0 ? 2 : 1) : 1; }
@@ -393,7 +419,7 @@ int64_t entity_lanes(const mpas_batch *B, int q, int k) {
 
 // Work items of kernel k for group q.
 int64_t kernel_items(const mpas_batch *B, int k, int q) {
-  const DMesh &M = B->meshes[B->groups[q].ctx]->dm;
+  const DMesh &M = mesh_ctx(B, B->groups[q].ctx)->dm;
   switch (k) {
     case 0: return M.nE * entity_lanes(B, q, k);
     case 1: return B->nT ? M.nE * entity_lanes(B, q, k) : 0;
@@ -436,7 +462,7 @@ void drop_graph(mpas_batch *B) {
 void describe_group(mpas_batch *B, int q) {
   const GroupInfo &G = B->groups[q];
   const GroupBufs &W = G.b;
-  mpas_ctx *ctx = B->meshes[G.ctx];
+  mpas_ctx *ctx = mesh_ctx(B, G.ctx);
   BatchGroup &d = B->desc[q];
   d.M = ctx->dm;
   d.M.nL = int(G.members.size());
@@ -693,6 +719,14 @@ int check_batch(const mpas_batch *B) {
 void release_batch(mpas_batch *B) {
   drop_graph(B);
   free_tables(B);
+  for (int o = 1; o < 3; o++) {
+    for (mpas_ctx *c : B->octx[o])
+      if (c) (void)mpas_ctx_destroy(c);
+    for (int32_t *p : B->d_cell_of[o])
+      if (p) (void)hipFree(p);
+    for (int32_t *p : B->d_edge_of[o])
+      if (p) (void)hipFree(p);
+  }
   if (B->d_grp) (void)hipFree(B->d_grp);
   if (B->staging) (void)hipFree(B->staging);
   if (B->arena) (void)hipFree(B->arena);
@@ -736,26 +770,132 @@ double time_steps(mpas_batch *B, int nsteps, int *rc, int mode = MPAS_BATCH_GRAP
 // One member's field between its host array (n entities x w doubles) and
 // the group layout, through the staging buffer: one contiguous copy and a
 // scatter / gather kernel.
+// `cells`: the field is per cell (else per edge), for the mesh order.
 int member_copy(mpas_batch *B, int m, double *group_field, const double *host, double *host_out,
-                size_t n, size_t w) {
+                size_t n, size_t w, bool cells = true) {
   const GroupInfo &G = B->groups[B->grp_of[m]];
   const int members = int(G.members.size()), i = B->idx_in_grp[m];
+  const int o = B->mesh_order;
+  const int32_t *of = o ? (cells ? B->d_cell_of[o][G.ctx] : B->d_edge_of[o][G.ctx]) : nullptr;
   const int64_t cnt = int64_t(n * w);
   const int grid = int((cnt + 255) / 256);
   if (host) {
     MPAS_HIP(hipMemcpyAsync(B->staging, host, sizeof(double) * cnt, hipMemcpyHostToDevice,
                             B->stream));
     hipLaunchKernelGGL(kb_member_scatter, dim3(grid), dim3(256), 0, B->stream, group_field,
-                       B->staging, int64_t(n), int(w), members, i);
+                       B->staging, int64_t(n), int(w), members, i, of);
     return after_launch("kb_member_scatter");
   }
   hipLaunchKernelGGL(kb_member_gather, dim3(grid), dim3(256), 0, B->stream, B->staging,
-                     group_field, int64_t(n), int(w), members, i);
+                     group_field, int64_t(n), int(w), members, i, of);
   if (int rc = after_launch("kb_member_gather")) return rc;
   MPAS_HIP(hipMemcpyAsync(host_out, B->staging, sizeof(double) * cnt, hipMemcpyDeviceToHost,
                           B->stream));
   MPAS_HIP(hipStreamSynchronize(B->stream));
   return 0;
+}
+
+// The reordered contexts of order o and their maps to the original ids.
+int ensure_order(mpas_batch *B, int o) {
+  if (o == 0 || !B->octx[o].empty()) return 0;
+  const size_t nm = B->meshes.size();
+  B->octx[o].assign(nm, nullptr);
+  B->d_cell_of[o].assign(nm, nullptr);
+  B->d_edge_of[o].assign(nm, nullptr);
+  for (size_t q = 0; q < nm; q++) {
+    if (int rc = mpas_ctx_create_reordered(B->meshes[q], o, &B->octx[o][q])) return rc;
+    const mpas_ctx *c = B->octx[o][q];
+    const HostMesh &h = c->host;
+    MPAS_HIP(hipMalloc(&B->d_cell_of[o][q], sizeof(int32_t) * h.nC));
+    MPAS_HIP(hipMalloc(&B->d_edge_of[o][q], sizeof(int32_t) * h.nE));
+    MPAS_HIP(hipMemcpy(B->d_cell_of[o][q], h.cellOf.data(), sizeof(int32_t) * h.nC,
+                       hipMemcpyHostToDevice));
+    MPAS_HIP(hipMemcpy(B->d_edge_of[o][q], h.edgeOf.data(), sizeof(int32_t) * h.nE,
+                       hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+// Original ids of a mesh's cells or edges in order o (identity for 0).
+std::vector<int32_t> original_ids(const mpas_batch *B, int o, int mesh, bool cells) {
+  const mpas_ctx *c = o ? B->octx[o][mesh] : B->meshes[mesh];
+  const HostMesh &h = c->host;
+  const int n = cells ? h.nC : h.nE;
+  const std::vector<int32_t> &of = cells ? h.cellOf : h.edgeOf;
+  std::vector<int32_t> r(n);
+  for (int i = 0; i < n; i++) r[i] = of.empty() ? i : of[i];
+  return r;
+}
+
+// The groups' diagnostics of their current state.
+int group_diagnostics(mpas_batch *B) {
+  for (size_t q = 0; q < B->groups.size(); q++) {
+    GroupInfo &G = B->groups[q];
+    const mpas_ctx *ctx = mesh_ctx(B, G.ctx);
+    DMesh Mg = ctx->dm;
+    Mg.nL = int(G.members.size());
+    if (int rc = diagnostics_dm(Mg, B->stream, ctx->params.visc2 != 0.0, G.b.u, G.b.h,
+                                G.b.ke[0], G.b.div[0], G.b.vort[0], G.b.pv_vertex,
+                                G.b.h_edge[0], G.b.pv_edge[0]))
+      return rc;
+  }
+  return 0;
+}
+
+// The members' state (u, h, tracers, h_s), held in order `from` in the arena
+// image `src`, into the arena in order `to`, then the diagnostics there.
+int permute_state(mpas_batch *B, const char *src, int from, int to) {
+  if (int rc = ensure_order(B, to)) return rc;
+  const char *arena = static_cast<const char *>(B->arena);
+  std::vector<int32_t *> maps;
+  int rc = 0;
+  for (size_t q = 0; q < B->groups.size() && !rc; q++) {
+    GroupInfo &G = B->groups[q];
+    const int members = int(G.members.size());
+    for (int cells = 0; cells < 2 && !rc; cells++) {
+      const std::vector<int32_t> of_from = original_ids(B, from, G.ctx, cells);
+      const std::vector<int32_t> of_to = original_ids(B, to, G.ctx, cells);
+      const int n = int(of_to.size());
+      std::vector<int32_t> pos(n), map(n);
+      for (int i = 0; i < n; i++) pos[of_from[i]] = i;
+      for (int i = 0; i < n; i++) map[i] = pos[of_to[i]];
+      int32_t *d = nullptr;
+      if (hipMalloc(&d, sizeof(int32_t) * n) != hipSuccess) {
+        set_error("mpas_batch: no memory for a mesh order map");
+        rc = MPAS_ENOMEM;
+        break;
+      }
+      maps.push_back(d);
+      if (hipMemcpy(d, map.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("mpas_batch: copy of a mesh order map failed");
+        rc = MPAS_EHIP;
+        break;
+      }
+      struct F { double *f; int64_t r; };
+      std::vector<F> fields;
+      if (cells) {
+        fields = {{G.b.h, members}, {G.b.h_s, 1}};
+        if (B->nT) fields.push_back({G.b.tr, int64_t(members) * B->nT});
+      } else {
+        fields = {{G.b.u, members}};
+      }
+      for (const F &f : fields) {
+        const double *s_ = reinterpret_cast<const double *>(
+            src + (reinterpret_cast<const char *>(f.f) - arena));
+        const int64_t cnt = int64_t(n) * f.r;
+        hipLaunchKernelGGL(kb_permute_rows, dim3(int((cnt + 255) / 256)), dim3(256), 0,
+                           B->stream, f.f, s_, int64_t(n), f.r, d);
+        if ((rc = after_launch("kb_permute_rows"))) break;
+      }
+    }
+  }
+  if (!rc) {
+    B->mesh_order = to;
+    if (!(rc = build_tables(B))) rc = group_diagnostics(B);
+  }
+  (void)hipStreamSynchronize(B->stream);
+  for (int32_t *d : maps) (void)hipFree(d);
+  return rc;
 }
 
 }  // namespace
@@ -886,7 +1026,7 @@ int mpas_batch_upload(mpas_batch *B, int member, const double *h, const double *
   if (!h || !u || !h_s || (B->nT && !tracers))
     MPAS_FAIL(MPAS_EINVAL, "mpas_batch_upload: NULL array");
   GroupInfo &G = B->groups[B->grp_of[member]];
-  const mpas_ctx *ctx = B->meshes[G.ctx];
+  const mpas_ctx *ctx = mesh_ctx(B, G.ctx);
   const DMesh &M = ctx->dm;
   {
     int rc;
@@ -898,14 +1038,18 @@ int mpas_batch_upload(mpas_batch *B, int member, const double *h, const double *
   }
   if (G.h_s.empty()) {
     G.h_s.assign(h_s, h_s + M.nC);
-    MPAS_HIP(hipMemcpyAsync(G.b.h_s, h_s, sizeof(double) * M.nC, hipMemcpyHostToDevice,
+    const int32_t *of = B->mesh_order ? B->d_cell_of[B->mesh_order][G.ctx] : nullptr;
+    MPAS_HIP(hipMemcpyAsync(B->staging, h_s, sizeof(double) * M.nC, hipMemcpyHostToDevice,
                             B->stream));
+    hipLaunchKernelGGL(kb_member_scatter, dim3((M.nC + 255) / 256), dim3(256), 0, B->stream,
+                       G.b.h_s, B->staging, int64_t(M.nC), 1, 1, 0, of);
+    if (int rc = after_launch("kb_member_scatter")) return rc;
   } else if (std::memcmp(G.h_s.data(), h_s, sizeof(double) * M.nC) != 0) {
     MPAS_FAIL(MPAS_EINVAL, "members of one mesh and time step share h_s (bottom topography)");
   }
   int rc;
   if ((rc = member_copy(B, member, G.b.h, h, nullptr, M.nC, 1)) ||
-      (rc = member_copy(B, member, G.b.u, u, nullptr, M.nE, 1)) ||
+      (rc = member_copy(B, member, G.b.u, u, nullptr, M.nE, 1, false)) ||
       (B->nT && (rc = member_copy(B, member, G.b.tr, tracers, nullptr, M.nC, B->nT))))
     return rc;
   // diagnostics of the group's state (every member: cheap, and the members
@@ -928,7 +1072,7 @@ int mpas_batch_download(mpas_batch *B, int member, double *h, double *u, double 
   const DMesh &M = B->meshes[G.ctx]->dm;
   int rc;
   if ((h && (rc = member_copy(B, member, G.b.h, nullptr, h, M.nC, 1))) ||
-      (u && (rc = member_copy(B, member, G.b.u, nullptr, u, M.nE, 1))) ||
+      (u && (rc = member_copy(B, member, G.b.u, nullptr, u, M.nE, 1, false))) ||
       (tracers && B->nT && (rc = member_copy(B, member, G.b.tr, nullptr, tracers, M.nC, B->nT))))
     return rc;
   MPAS_HIP(hipStreamSynchronize(B->stream));
@@ -986,6 +1130,32 @@ int mpas_batch_get_config(const mpas_batch *B, int32_t *block, int *members_per_
   return 0;
 }
 
+int mpas_batch_set_order(mpas_batch *B, int order) {
+  if (int rc = check_batch(B)) return rc;
+  if (order < MPAS_ORDER_GIVEN || order > MPAS_ORDER_MORTON)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_batch_set_order: unknown order");
+  if (order == B->mesh_order) return 0;
+  MPAS_HIP(hipStreamSynchronize(B->stream));
+  void *save = nullptr;
+  if (hipMalloc(&save, B->arena_bytes) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "mpas_batch_set_order: no memory for the state copy");
+  int rc = 0;
+  if (hipMemcpyAsync(save, B->arena, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream) !=
+      hipSuccess) {
+    set_error("mpas_batch_set_order: state copy failed");
+    rc = MPAS_EHIP;
+  }
+  if (!rc) rc = permute_state(B, static_cast<const char *>(save), B->mesh_order, order);
+  (void)hipStreamSynchronize(B->stream);
+  (void)hipFree(save);
+  return rc;
+}
+
+int mpas_batch_get_order(const mpas_batch *B) {
+  if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
+  return B->mesh_order;
+}
+
 int mpas_batch_tuned_mode(const mpas_batch *B) {
   if (!B) MPAS_FAIL(MPAS_EINVAL, "batch is NULL");
   return B->tuned_mode;
@@ -1034,14 +1204,14 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     if (rc) return 1e30;
     if ((rc = build_tables(B))) return 1e30;
     const double t = time_steps(B, trial_steps, &rc);
-    char line[256];
+    char line[320];
     std::snprintf(line, sizeof line,
                   "%s{\"trial\": \"%s\", \"block\": [%d, %d, %d, %d, %d, %d], "
                   "\"members_per_lane\": %d, \"wave_entity\": %d, \"order\": %d, \"xcd\": %d, "
-                  "\"ms_per_step\": %.4f, \"layout_block\": [",
+                  "\"mesh_order\": %d, \"ms_per_step\": %.4f, \"layout_block\": [",
                   log.size() > 1 ? ", " : "", what, B->block[0], B->block[1], B->block[2],
                   B->block[3], B->block[4], B->block[5], B->lpl, B->wave_entity, B->order,
-                  B->xcd, 1e3 * t / trial_steps);
+                  B->xcd, B->mesh_order, 1e3 * t / trial_steps);
     log += line;
     for (int k = 0; k < kBatchKernels; k++)
       for (int m = 0; m < kBatchModes; m++)
@@ -1050,6 +1220,16 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     return rc ? 1e30 : t;
   };
   double best = trial("start");
+  // the mesh order first, each candidate from the saved state
+  const int start_order = B->mesh_order;
+  for (int o = MPAS_ORDER_GIVEN; o <= MPAS_ORDER_MORTON && !rc; o++) {
+    if (o == B->mesh_order) continue;
+    const int keep = B->mesh_order;
+    if ((rc = permute_state(B, static_cast<const char *>(save), start_order, o))) break;
+    const double t = trial("mesh order");
+    if (t < best) best = t;
+    else rc = permute_state(B, static_cast<const char *>(save), start_order, keep);
+  }
   size_t gmax = 1;
   for (const GroupInfo &G : B->groups) gmax = std::max(gmax, G.members.size());
   // entity per wave and members per lane, placement and order, then each
@@ -1119,7 +1299,11 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     }
   }
   if (!rc) rc = build_tables(B);
-  (void)hipMemcpyAsync(B->arena, save, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream);
+  if (B->mesh_order == start_order) {
+    (void)hipMemcpyAsync(B->arena, save, B->arena_bytes, hipMemcpyDeviceToDevice, B->stream);
+  } else if (!rc) {
+    rc = permute_state(B, static_cast<const char *>(save), start_order, B->mesh_order);
+  }
   (void)hipStreamSynchronize(B->stream);
   (void)hipFree(save);
   if (rc) return rc;

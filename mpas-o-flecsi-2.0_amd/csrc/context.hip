@@ -159,7 +159,12 @@ int build_device_mesh(mpas_ctx *ctx) {
         if (j >= m.c2c.size(c) || m.c2c.at(c, j) != opp.idx[m.c2e.off[c] + j]) aligned = false;
         slot[j] = j;
       }
-      std::sort(slot, slot + ne, [&](int a, int b) { return m.c2e.at(c, a) < m.c2e.at(c, b); });
+      // ascending edge id: the original ids of a reordered mesh
+      auto key = [&](int j) {
+        const int e = m.c2e.at(c, j);
+        return m.edgeOf.empty() ? e : m.edgeOf[e];
+      };
+      std::sort(slot, slot + ne, [&](int a, int b) { return key(a) < key(b); });
       for (int r = 0; r < ne; r++) {
         perm[c] |= uint32_t(slot[r]) << (4 * r);
         rank[c] &= ~(0xfu << (4 * slot[r]));
@@ -325,6 +330,29 @@ int mpas_host_mesh_create(const mpas_mesh_desc *mesh, const mpas_params *params,
   return 0;
 }
 
+int mpas_host_mesh_reorder(const mpas_host_mesh *hm, int order, mpas_host_mesh **out) {
+  if (!hm || !out) MPAS_FAIL(MPAS_EINVAL, "mpas_host_mesh_reorder: NULL argument");
+  *out = nullptr;
+  auto o = std::make_unique<mpas_host_mesh>();
+  const std::string err = reorder_host_mesh(*hm, order, *o);
+  if (!err.empty()) MPAS_FAIL(MPAS_EINVAL, "mpas_host_mesh_reorder: " + err);
+  *out = o.release();
+  return 0;
+}
+
+int mpas_host_mesh_order(const mpas_host_mesh *hm, int32_t *cell_of, int32_t *edge_of,
+                         int32_t *vertex_of) {
+  if (!hm) MPAS_FAIL(MPAS_EINVAL, "mpas_host_mesh_order: NULL mesh");
+  auto put = [](const std::vector<int32_t> &of, int n, int32_t *dst) {
+    if (!dst) return;
+    for (int i = 0; i < n; i++) dst[i] = of.empty() ? i : of[i];
+  };
+  put(hm->cellOf, hm->nC, cell_of);
+  put(hm->edgeOf, hm->nE, edge_of);
+  put(hm->vertexOf, hm->nV, vertex_of);
+  return 0;
+}
+
 int mpas_host_mesh_destroy(mpas_host_mesh *hm) {
   delete hm;
   return 0;
@@ -334,11 +362,36 @@ const mpas_host_mesh *mpas_ctx_host_mesh(const mpas_ctx *ctx) {
   return ctx ? &ctx->host : nullptr;
 }
 
+static int finish_ctx(std::unique_ptr<mpas_ctx> ctx, const mpas_params *params,
+                      mpas_ctx **out);
+
 int mpas_ctx_create(const mpas_mesh_desc *mesh, const mpas_params *params, mpas_ctx **out) {
   if (!out) MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_create: out is NULL");
   *out = nullptr;
   auto ctx = std::make_unique<mpas_ctx>();
   if (int rc = prepare_host_mesh(mesh, params, ctx->host)) return rc;
+  return finish_ctx(std::move(ctx), params, out);
+}
+
+int mpas_ctx_create_reordered(const mpas_ctx *src, int order, mpas_ctx **out) {
+  if (!out) MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_create_reordered: out is NULL");
+  *out = nullptr;
+  if (int rc = check_ctx(src)) return rc;
+  if (order < MPAS_ORDER_GIVEN || order > MPAS_ORDER_MORTON)
+    MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_create_reordered: unknown order");
+  auto ctx = std::make_unique<mpas_ctx>();
+  const std::string err = reorder_host_mesh(src->host, order, ctx->host);
+  if (!err.empty()) MPAS_FAIL(MPAS_EINVAL, "mpas_ctx_create_reordered: " + err);
+  return finish_ctx(std::move(ctx), &src->params, out);
+}
+
+int mpas_ctx_order(const mpas_ctx *ctx, int32_t *cell_of, int32_t *edge_of, int32_t *vertex_of) {
+  if (!ctx) MPAS_FAIL(MPAS_EINVAL, "context is NULL");
+  return mpas_host_mesh_order(&ctx->host, cell_of, edge_of, vertex_of);
+}
+
+static int finish_ctx(std::unique_ptr<mpas_ctx> ctx, const mpas_params *params,
+                      mpas_ctx **out) {
   ctx->params = *params;
   {
     // The kernels form byte offsets in 32 bits (ld / st): every array they

@@ -4,6 +4,7 @@
 #include "mesh_setup.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -511,6 +512,186 @@ std::string setup_tracer_operators(HostMesh &m, double a) {
       return "the FCT stencil of edge " + std::to_string(i / 16) +
              " is not finite: sphere_radius must be the mesh's radius";
   m.fct = true;
+  return "";
+}
+
+namespace {
+
+// Cells in reverse Cuthill-McKee order: breadth-first from a peripheral
+// cell (the last one reached by a sweep from cell 0), neighbours by
+// ascending degree then id, the whole sequence reversed.
+std::vector<int32_t> rcm_order(const HostMesh &m) {
+  const int C = m.nC;
+  auto bfs = [&](int start, std::vector<int32_t> &seq) {
+    std::vector<char> seen(C, 0);
+    seq.clear();
+    for (int root = start, done = 0; done < C; root = 0) {
+      while (root < C && seen[root]) root++;
+      if (root >= C) break;
+      seen[root] = 1;
+      size_t head = seq.size();
+      seq.push_back(root);
+      while (head < seq.size()) {
+        const int c = seq[head++];
+        std::vector<int> nb;
+        for (int j = 0; j < m.c2c.size(c); j++)
+          if (!seen[m.c2c.at(c, j)]) nb.push_back(m.c2c.at(c, j));
+        std::sort(nb.begin(), nb.end(), [&](int a, int b) {
+          return m.c2c.size(a) != m.c2c.size(b) ? m.c2c.size(a) < m.c2c.size(b) : a < b;
+        });
+        for (int x : nb)
+          if (!seen[x]) seen[x] = 1, seq.push_back(x);
+      }
+      done = int(seq.size());
+    }
+  };
+  std::vector<int32_t> seq;
+  bfs(0, seq);
+  bfs(seq.back(), seq);
+  std::reverse(seq.begin(), seq.end());
+  return seq;
+}
+
+// Cells along a 3-D Morton (Z-order) curve of their unit positions.
+std::vector<int32_t> morton_order(const HostMesh &m) {
+  const int C = m.nC;
+  auto spread = [](uint64_t x) {  // 21 bits -> every third bit of 63
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(C);
+  for (int c = 0; c < C; c++) {
+    const double r = std::sqrt(m.xCell[c] * m.xCell[c] + m.yCell[c] * m.yCell[c] +
+                               m.zCell[c] * m.zCell[c]);
+    auto q = [&](double v) { return uint64_t(std::min(2097151.0, (v / r + 1.0) * 1048575.5)); };
+    key[c] = {spread(q(m.xCell[c])) | spread(q(m.yCell[c])) << 1 | spread(q(m.zCell[c])) << 2, c};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> seq(C);
+  for (int i = 0; i < C; i++) seq[i] = key[i].second;
+  return seq;
+}
+
+// Rows of `in` in the order `of` (new -> old), entries renamed by `map`
+// (old -> new), each row's entry order kept.
+Csr remap_rows(const Csr &in, const std::vector<int32_t> &of, const std::vector<int32_t> &map) {
+  Csr out;
+  out.off.assign(1, 0);
+  out.idx.reserve(in.idx.size());
+  for (int32_t o : of) {
+    for (int j = 0; j < in.size(o); j++) out.idx.push_back(map[in.at(o, j)]);
+    out.off.push_back(int32_t(out.idx.size()));
+  }
+  return out;
+}
+
+template <class T>
+std::vector<T> move_rows(const std::vector<T> &a, const std::vector<int32_t> &of, size_t w) {
+  if (a.empty()) return a;
+  std::vector<T> out(a.size());
+  for (size_t i = 0; i < of.size(); i++)
+    std::copy(a.begin() + size_t(of[i]) * w, a.begin() + size_t(of[i] + 1) * w,
+              out.begin() + i * w);
+  return out;
+}
+
+}  // namespace
+
+std::string reorder_host_mesh(const HostMesh &m, int order, HostMesh &o) {
+  const int C = m.nC, E = m.nE, V = m.nV;
+  std::vector<int32_t> cof;
+  if (order == kOrderGiven) {
+    cof.resize(C);
+    for (int c = 0; c < C; c++) cof[c] = c;
+  } else if (order == kOrderRcm) {
+    cof = rcm_order(m);
+  } else if (order == kOrderMorton) {
+    if (m.xCell.size() != size_t(C)) return "the Morton order needs the cell positions";
+    cof = morton_order(m);
+  } else {
+    return "unknown cell order";
+  }
+  if (int(cof.size()) != C) return "cell order does not cover the mesh";
+  std::vector<int32_t> cnew(C);
+  for (int i = 0; i < C; i++) cnew[cof[i]] = i;
+  // edges by their (lower, higher) new cell pair, vertices by their sorted
+  // new cell triple (ties by old id)
+  std::vector<int32_t> eof(E), vof(V);
+  {
+    std::vector<std::pair<std::pair<int, int>, int>> k(E);
+    for (int e = 0; e < E; e++) {
+      const int a = cnew[m.e2c.at(e, 0)], b = cnew[m.e2c.at(e, 1)];
+      k[e] = {{std::min(a, b), std::max(a, b)}, e};
+    }
+    std::sort(k.begin(), k.end());
+    for (int i = 0; i < E; i++) eof[i] = k[i].second;
+    std::vector<std::pair<std::array<int, 3>, int>> kv(V);
+    for (int v = 0; v < V; v++) {
+      std::array<int, 3> t{};
+      for (int i = 0; i < 3 && i < m.v2c.size(v); i++) t[i] = cnew[m.v2c.at(v, i)];
+      std::sort(t.begin(), t.end());
+      kv[v] = {t, v};
+    }
+    std::sort(kv.begin(), kv.end());
+    for (int i = 0; i < V; i++) vof[i] = kv[i].second;
+  }
+  std::vector<int32_t> enew(E), vnew(V);
+  for (int i = 0; i < E; i++) enew[eof[i]] = i;
+  for (int i = 0; i < V; i++) vnew[vof[i]] = i;
+
+  o = HostMesh();
+  o.nC = C, o.nE = E, o.nV = V, o.nL = m.nL;
+  o.maxEdges = m.maxEdges, o.maxEdges2 = m.maxEdges2, o.vdeg = m.vdeg;
+  o.c2v = remap_rows(m.c2v, cof, vnew);
+  o.c2e = remap_rows(m.c2e, cof, enew);
+  o.c2c = remap_rows(m.c2c, cof, cnew);
+  o.e2v = remap_rows(m.e2v, eof, vnew);
+  o.e2e = remap_rows(m.e2e, eof, enew);
+  o.e2c = remap_rows(m.e2c, eof, cnew);
+  o.v2c = remap_rows(m.v2c, vof, cnew);
+  o.v2e = remap_rows(m.v2e, vof, enew);
+  o.areaCell = move_rows(m.areaCell, cof, 1);
+  o.xCell = move_rows(m.xCell, cof, 1);
+  o.yCell = move_rows(m.yCell, cof, 1);
+  o.zCell = move_rows(m.zCell, cof, 1);
+  o.meshDensity = move_rows(m.meshDensity, cof, 1);
+  o.maxLevelCell = move_rows(m.maxLevelCell, cof, 1);
+  o.boundaryCell = move_rows(m.boundaryCell, cof, size_t(m.nL));
+  o.edgeSignOnCell = move_rows(m.edgeSignOnCell, cof, 8);
+  o.dvEdge = move_rows(m.dvEdge, eof, 1);
+  o.dcEdge = move_rows(m.dcEdge, eof, 1);
+  o.weightsOnEdge = move_rows(m.weightsOnEdge, eof, 16);
+  o.maxLevelEdgeTop = move_rows(m.maxLevelEdgeTop, eof, 1);
+  o.nAdvCellsForEdge = move_rows(m.nAdvCellsForEdge, eof, 1);
+  o.advCellsForEdge = move_rows(m.advCellsForEdge, eof, 16);
+  for (size_t e = 0; e < o.nAdvCellsForEdge.size(); e++)
+    for (int i = 0; i < o.nAdvCellsForEdge[e]; i++)
+      o.advCellsForEdge[e * 16 + i] = cnew[o.advCellsForEdge[e * 16 + i]];
+  o.highOrderAdvMask = move_rows(m.highOrderAdvMask, eof, size_t(m.nL));
+  o.meshScalingDel2 = move_rows(m.meshScalingDel2, eof, 1);
+  o.derivTwo = move_rows(m.derivTwo, eof, 32);
+  o.advCoef = move_rows(m.advCoef, eof, 16);
+  o.advCoef_3rd = move_rows(m.advCoef_3rd, eof, 16);
+  o.areaTriangle = move_rows(m.areaTriangle, vof, 1);
+  o.kite = move_rows(m.kite, vof, 3);
+  o.fVertex = move_rows(m.fVertex, vof, 1);
+  o.xVertex = move_rows(m.xVertex, vof, 1);
+  o.yVertex = move_rows(m.yVertex, vof, 1);
+  o.zVertex = move_rows(m.zVertex, vof, 1);
+  o.fct = m.fct;
+  auto compose = [](const std::vector<int32_t> &prev, const std::vector<int32_t> &of) {
+    std::vector<int32_t> r(of.size());
+    for (size_t i = 0; i < of.size(); i++) r[i] = prev.empty() ? of[i] : prev[of[i]];
+    return r;
+  };
+  o.cellOf = compose(m.cellOf, cof);
+  o.edgeOf = compose(m.edgeOf, eof);
+  o.vertexOf = compose(m.vertexOf, vof);
   return "";
 }
 

@@ -37,7 +37,23 @@ struct HostMesh {
       advCellsForEdge /*[E][16]*/, highOrderAdvMask /*[E][nL]*/;
   std::vector<double> edgeSignOnCell /*[C][8]*/, meshScalingDel2 /*[E]*/,
       derivTwo /*[E][16][2]*/, advCoef /*[E][16]*/, advCoef_3rd /*[E][16]*/;
+  // After a reordering: the original id of each entity (empty: the ids are
+  // the original ones).  edgeKey orders a cell's edges for the sums the
+  // reference forms in ascending edge id, so that a reordered mesh keeps
+  // every sum's order and its bits.
+  std::vector<int32_t> cellOf, edgeOf, vertexOf;
 };
+
+// Cell orders of reorder_host_mesh.
+enum { kOrderGiven = 0, kOrderRcm = 1, kOrderMorton = 2, kOrders = 3 };
+
+// The mesh m renumbered: cells in the given order (as m, reverse
+// Cuthill-McKee, or along a 3-D Morton curve of the cell centres), edges by
+// their (lower, higher) new cell pair, vertices by their sorted new cell
+// triple.  Every list keeps its entries' order (ids renumbered), every
+// per-entity array moves with its entity, and the original ids are kept
+// (cellOf / edgeOf / vertexOf, composed with m's own).
+std::string reorder_host_mesh(const HostMesh &m, int order, HostMesh &out);
 
 // Returns an empty string on success, else the reason the mesh is rejected.
 std::string flatten_mesh(const mpas_mesh_desc &d, int nLevels, HostMesh &m);

@@ -20,6 +20,7 @@ P_i64 = ctypes.POINTER(ctypes.c_int64)
 
 TRACER_NONE, TRACER_CENTRED, TRACER_FCT = 0, 1, 2
 BATCH_GRAPH, BATCH_LAUNCH, BATCH_PERSISTENT, BATCH_TUNED = 0, 1, 2, 3
+ORDER_GIVEN, ORDER_RCM, ORDER_MORTON = 0, 1, 2
 STATUS = {0: "MPAS_OK", -1: "MPAS_EINVAL", -2: "MPAS_EHIP", -3: "MPAS_ESHAPE",
           -4: "MPAS_ESTATE", -5: "MPAS_ENOMEM", -6: "MPAS_ENONFINITE"}
 
@@ -131,6 +132,10 @@ SIGNATURES = {
     "mpas_sw_state_check": (c_int, [c_void_p, ctypes.POINTER(c_int64)]),
     "mpas_sw_field_download": (c_int, [c_void_p, c_char_p, P_f64, c_int64]),
     "mpas_rk4_step": (c_int, [c_void_p, c_double, c_int]),
+    "mpas_ctx_create_reordered": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p)]),
+    "mpas_ctx_order": (c_int, [c_void_p, P_i32, P_i32, P_i32]),
+    "mpas_host_mesh_reorder": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p)]),
+    "mpas_host_mesh_order": (c_int, [c_void_p, P_i32, P_i32, P_i32]),
     "mpas_ocean_rk4_step": (c_int, [c_void_p, c_double, c_int]),
     "mpas_rk4_step_noalloc": (c_int, [c_void_p, c_double, c_int, _D, _D, _D, _D]),
     "mpas_adj_init": (c_int, [c_void_p, c_int]),
@@ -150,6 +155,8 @@ SIGNATURES = {
     "mpas_batch_get_config": (c_int, [c_void_p, P_i32] + [ctypes.POINTER(c_int)] * 4),
     "mpas_batch_get_layout_blocks": (c_int, [c_void_p, P_i32]),
     "mpas_batch_tuned_mode": (c_int, [c_void_p]),
+    "mpas_batch_set_order": (c_int, [c_void_p, c_int]),
+    "mpas_batch_get_order": (c_int, [c_void_p]),
     "mpas_batch_set_layout_blocks": (c_int, [c_void_p, P_i32]),
     "mpas_batch_autotune": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
 }

@@ -119,6 +119,26 @@ class HostMesh:
     def setup_field(self, name):
         return _setup_field(self._h, name, self.nCells, self.nEdges, self.nLevels)
 
+    def reordered(self, order):
+        """mpas_host_mesh_reorder: this mesh renumbered (ORDER_*)."""
+        h = ctypes.c_void_p()
+        L.check(L.hip().mpas_host_mesh_reorder(self._h, int(order), ctypes.byref(h)),
+                "mpas_host_mesh_reorder")
+        o = HostMesh.__new__(HostMesh)
+        o._h, o.nCells, o.nEdges, o.nLevels = h.value, self.nCells, self.nEdges, self.nLevels
+        o.nVertices = getattr(self, "nVertices", None)
+        return o
+
+    def order(self, nVertices):
+        """The original id of each cell, edge and vertex."""
+        cs = np.zeros(self.nCells, dtype=np.int32)
+        es = np.zeros(self.nEdges, dtype=np.int32)
+        vs = np.zeros(nVertices, dtype=np.int32)
+        L.check(L.hip().mpas_host_mesh_order(self._h, cs.ctypes.data_as(L.P_i32),
+                                             es.ctypes.data_as(L.P_i32),
+                                             vs.ctypes.data_as(L.P_i32)), "mpas_host_mesh_order")
+        return cs, es, vs
+
 
 class Context:
     def __init__(self, mesh, nLevels=1, nTracers=0, tracer_scheme=L.TRACER_NONE, freeze_u=False,
@@ -141,6 +161,29 @@ class Context:
         dims = np.zeros(5, dtype=np.int64)
         L.check(L.hip().mpas_ctx_dims(self._h, dims.ctypes.data_as(L.P_i64)), "mpas_ctx_dims")
         self.nCells, self.nEdges, self.nVertices, self.nLevels, self.nTracers = map(int, dims)
+
+    def reordered(self, order):
+        """mpas_ctx_create_reordered: a context on this mesh renumbered
+        (ORDER_GIVEN, ORDER_RCM or ORDER_MORTON); its fields are in its own
+        numbering (order() maps each entity to its original id)."""
+        h = ctypes.c_void_p()
+        L.check(L.hip().mpas_ctx_create_reordered(self._h, int(order), ctypes.byref(h)),
+                "mpas_ctx_create_reordered")
+        c = Context.__new__(Context)
+        c._h, c.mesh, c.params = h.value, self.mesh, self.params
+        c.nCells, c.nEdges, c.nVertices = self.nCells, self.nEdges, self.nVertices
+        c.nLevels, c.nTracers = self.nLevels, self.nTracers
+        return c
+
+    def order(self):
+        """The original id of each cell, edge and vertex."""
+        cs = np.zeros(self.nCells, dtype=np.int32)
+        es = np.zeros(self.nEdges, dtype=np.int32)
+        vs = np.zeros(self.nVertices, dtype=np.int32)
+        L.check(L.hip().mpas_ctx_order(self._h, cs.ctypes.data_as(L.P_i32),
+                                       es.ctypes.data_as(L.P_i32), vs.ctypes.data_as(L.P_i32)),
+                "mpas_ctx_order")
+        return cs, es, vs
 
     def close(self):
         if self._h:
@@ -484,6 +527,13 @@ class Batch:
                 "mpas_batch_get_config")
         return {"block": blk.tolist(), "members_per_lane": lpl.value, "order": o.value,
                 "xcd": x.value, "wave_entity": we.value}
+
+    def set_order(self, order):
+        """mpas_batch_set_order: run the groups on reordered meshes (ORDER_*)."""
+        L.check(L.hip().mpas_batch_set_order(self._b, int(order)), "mpas_batch_set_order")
+
+    def get_order(self):
+        return L.check_count(L.hip().mpas_batch_get_order(self._b), "mpas_batch_get_order")
 
     def tuned_mode(self):
         """The launch mode the autotuner found fastest (BATCH_GRAPH or

@@ -106,6 +106,36 @@ def test_persistent_launch_keeps_results(nT):
     _close(ctxs, b)
 
 
+def test_mesh_orders_keep_results():
+    """The groups on reordered meshes (reverse Cuthill-McKee, Morton, back to
+    the given order): uploads and downloads keep the callers' numbering and
+    every member equals its single run bit for bit, also across a change of
+    order between steps; the autotuner's choice keeps the results."""
+    from mpas_hip import BATCH_GRAPH, BATCH_PERSISTENT
+    ctxs, b, states, mesh_of = _ensemble()
+    ref = _singles(ctxs, states, mesh_of, 4)
+    for order in (1, 2, 0):
+        b.set_order(order)
+        assert b.get_order() == order
+        _load(b, states)
+        b.step(2, BATCH_GRAPH)
+        b.set_order((order + 1) % 3)  # the state follows the new numbering
+        b.step(2, BATCH_PERSISTENT)
+        b.sync()
+        for i in range(b.n):
+            for a, r in zip(b.download(i), ref[i]):
+                assert bitwise_equal(a, r), (order, i)
+    _load(b, states)
+    trials = b.autotune(trial_steps=2)
+    assert sum(t["trial"] == "mesh order" for t in trials) == 2
+    b.step(4)
+    b.sync()
+    for i in range(b.n):
+        for a, r in zip(b.download(i), ref[i]):
+            assert bitwise_equal(a, r), ("tuned", i)
+    _close(ctxs, b)
+
+
 def test_configurations_keep_results():
     ctxs, b, states, mesh_of = _ensemble()
     ref = _singles(ctxs, states, mesh_of, 4)

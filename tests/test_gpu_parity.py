@@ -396,3 +396,31 @@ def test_compute_error_case1(mesh3):
     o.set("h", h)
     assert abs(err - o.compute_error()) <= 1e-12 * max(1.0, abs(err))
     ctx.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_reordered_context_bitwise(order):
+    """A context on the mesh renumbered (reverse Cuthill-McKee or Morton
+    cells; edges and vertices after them) advances the same state, seen
+    through the permutation, to the same bits: every list keeps its order
+    and the ascending-edge sums keep the original ids' order."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(4)
+    u, h, tr = sw_workload_state(m, 4, seed=12)
+    tr = tr.reshape(m.nCells, 4)
+    h_s = np.full(m.nCells, 4500.0)
+    ctx = Context(m, nTracers=4, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
+    r = ctx.reordered(order)
+    cof, eof, _ = r.order()
+    ctx.sw_upload(h, u, tr.ravel(), h_s)
+    ctx.rk4_step(480.0, 4)
+    ref = ctx.sw_download()
+    r.sw_upload(h[cof], u[eof], tr[cof].ravel(), h_s[cof])
+    r.rk4_step(480.0, 4)
+    gh, gu, gt = r.sw_download()
+    assert bitwise_equal(gh, ref[0][cof])
+    assert bitwise_equal(gu, ref[1][eof])
+    assert bitwise_equal(gt.reshape(-1, 4), ref[2].reshape(-1, 4)[cof])
+    r.close()
+    ctx.close()
+
