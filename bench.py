@@ -46,23 +46,53 @@ def model_bytes_per_step(ctx, mesh, n_tr):
     return 4 * stage + state
 
 
-def cpu_baseline(mesh, u, h, tr, h_s, dt, steps, n_tr):
-    from oracle.oracle import OracleMesh, OracleSW
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, u, h, tr, h_s, dt, n_tr, steps_1=2, steps_all=4, runs=5):
+    """The oracle (the C++ restatement of the reference's task loops, g++ -O2)
+    on the host cores, on a bounded sample of the workload: `runs` runs of
+    `steps_1` steps on one thread (the reference's serial single-rank loops)
+    and of `steps_all` steps on every core the box gives this process (its
+    gather loops threaded, its scatter loops serial: the same bits), each
+    from the same state after one untimed setup; medians of the runs."""
+    from oracle.oracle import OracleMesh, OracleSW, set_threads
     t0 = time.perf_counter()
     om = OracleMesh(mesh, nL=1)
-    o = OracleSW(om, nL=1, nT=n_tr, tracer_adv=2, visc2=1e5, kappa2=1e3)
-    o.set("h", h)
-    o.set("u", u)
-    o.set("tracers", tr)
-    o.set("h_s", h_s)
-    o.init_diagnostics()
     setup_s = time.perf_counter() - t0
-    sec = o.run(steps, dt)
-    return {"value": mesh.nCells * steps / sec, "unit": "cell-steps/s", "cores": 1,
+    cores = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)))
+
+    def rate(threads, steps):
+        set_threads(threads)
+        o = OracleSW(om, nL=1, nT=n_tr, tracer_adv=2, visc2=1e5, kappa2=1e3)
+        o.set("h", h)
+        o.set("u", u)
+        o.set("tracers", tr)
+        o.set("h_s", h_s)
+        o.init_diagnostics()
+        return mesh.nCells * steps / o.run(steps, dt)
+
+    one = [rate(1, steps_1) for _ in range(runs)]
+    many = [rate(cores, steps_all) for _ in range(runs)]
+    set_threads(1)
+    return {"value": float(np.median(many)), "unit": "cell-steps/s", "cores": cores,
             "kind": "port",
-            "sample": f"{steps} RK4 steps of the same L8 workload on the oracle (C++ restatement "
-                      f"of the reference task loops, g++ -O2, 1 thread); step loop only, "
-                      f"{sec:.2f} s timed, {setup_s:.1f} s setup untimed"}
+            "sample": f"median of {runs} runs of {steps_all} RK4 steps of the same level-8 "
+                      f"workload on the oracle (C++ restatement of the reference task loops, "
+                      f"g++ -O2), gather loops on {cores} OpenMP threads, scatter loops serial; "
+                      f"step loop only, mesh setup {setup_s:.1f} s untimed",
+            "one_core": {"value": float(np.median(one)), "cores": 1,
+                         "sample": f"median of {runs} runs of {steps_1} steps, one thread"},
+            "runs_all_cores": many, "runs_one_core": one,
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3):
@@ -202,7 +232,8 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
     trace_steps > 0: a kernel-trace run instead, with the default launch
     configuration (no autotuning) and trace_steps direct-launch steps (one
     launch per phase and layout, no graph), nothing else."""
-    from mpas_hip import BATCH_GRAPH, BATCH_LAUNCH, TRACER_FCT, Batch, Context, Mesh, sw_workload_state
+    from mpas_hip import (BATCH_GRAPH, BATCH_LAUNCH, BATCH_PERSISTENT, BATCH_TUNED, TRACER_FCT,
+                          Batch, Context, Mesh, sw_workload_state)
     t0 = time.perf_counter()
     meshes = [Mesh(L) for L, _, _ in groups]
     ctxs = [Context(m, nTracers=n_tr, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
@@ -240,7 +271,10 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
         return {"trace_steps": trace_steps, "ms_per_step_batched_launch": ms, "config": cfg}
     trials = b.autotune(trial_steps=tune_steps)
     cfg = b.get_config()
-    log(f"  ensemble: autotuned {cfg} over {len(trials)} trials")
+    cfg["mesh_order"] = ["given (Hilbert)", "reverse Cuthill-McKee", "Morton"][b.get_order()]
+    order_trials = {["given (Hilbert)", "reverse Cuthill-McKee", "Morton"][t["mesh_order"]]:
+                    t["ms_per_step"] for t in trials[:3] if "mesh_order" in t}
+    log(f"  ensemble: autotuned {cfg} over {len(trials)} trials; mesh orders {order_trials}")
 
     def run(nsteps, mode):
         load()
@@ -254,10 +288,18 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
     t_graph = run(steps, BATCH_GRAPH)
     log(f"  ensemble: {steps} batched steps (graph) {1e3 * t_graph / steps:.3f} ms/step")
     t_launch = run(steps, BATCH_LAUNCH)
+    run(2, BATCH_PERSISTENT)
+    t_pers = run(steps, BATCH_PERSISTENT)
+    log(f"  ensemble: {steps} batched steps (persistent launch) "
+        f"{1e3 * t_pers / steps:.3f} ms/step")
+    tuned = b.tuned_mode()
+    t_tuned = run(steps, BATCH_TUNED)
+    mode_name = {BATCH_GRAPH: "graph", BATCH_PERSISTENT: "persistent launch"}[tuned]
+    log(f"  ensemble: tuned mode {mode_name}: {1e3 * t_tuned / steps:.3f} ms/step")
     # baseline: every member alone, seq_steps steps each; bitwise check on
     # one member per mesh against the batched run of the same length
     probe = {mesh_of.index(g) for g in range(len(groups))}
-    run(seq_steps, BATCH_GRAPH)
+    run(seq_steps, BATCH_TUNED)
     got = {i: b.download(i) for i in probe}
     t_seq, same = 0.0, True
     for i, st in enumerate(states):
@@ -283,6 +325,7 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
     for c in ctxs:
         c.close()
     ms_graph = 1e3 * t_graph / steps
+    ms_tuned = 1e3 * t_tuned / steps
     ms_seq = 1e3 * t_seq / seq_steps
     return {
         "workload": "ensemble of 256 members: 96 x level 5 (10,242 cells, dt 480 s), 128 x "
@@ -291,14 +334,17 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
         "members": len(mesh_of), "member_cells": cells, "steps": steps,
         "ms_per_step_batched_graph": ms_graph,
         "ms_per_step_batched_launch": 1e3 * t_launch / steps,
+        "ms_per_step_persistent_launch": 1e3 * t_pers / steps,
+        "tuned_mode": mode_name, "ms_per_step_tuned": ms_tuned,
         "ms_per_step_sequential": ms_seq, "sequential_steps_timed": seq_steps,
-        "speedup_vs_sequential": ms_seq / ms_graph,
-        "member_cell_steps_per_s": cells * steps / t_graph,
+        "speedup_vs_sequential": ms_seq / ms_tuned,
+        "member_cell_steps_per_s": cells * steps / t_tuned,
         "hbm_frac_model": {"bytes_per_step": model,
-                           "achieved_GBs": model / (ms_graph * 1e-3) / 1e9,
-                           "frac": model / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                           "achieved_GBs": model / (ms_tuned * 1e-3) / 1e9,
+                           "frac": model / (ms_tuned * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "bitwise_equal_to_single_runs": bool(same),
         "autotuned_config": cfg, "autotune_trials": len(trials),
+        "mesh_order_trials_ms_per_step": order_trials,
         "setup_s": setup_s,
     }
 
@@ -487,8 +533,11 @@ def main():
     ap.add_argument("--dt", type=float, default=60.0)
     ap.add_argument("--prof-steps", type=int, default=50,
                     help="steps of the per-kernel event-timed pass")
-    ap.add_argument("--cpu-steps", type=int, default=10,
-                    help="oracle steps of the CPU baseline (about 1.3 s each on one core)")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="oracle steps per one-thread run of the CPU baseline (about 1.5 s "
+                         "each); the all-cores runs take twice as many")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="further timed blocks of --steps steps for the median")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (level 6, 200 steps, 1 FCT tracer)")
@@ -555,6 +604,15 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = mesh.nCells * args.steps / elapsed
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step")
+    blocks = []
+    for _ in range(args.repeats):
+        t0 = time.perf_counter()
+        ctx.rk4_step(args.dt, args.steps)
+        ctx.sync()
+        blocks.append(1e3 * (time.perf_counter() - t0) / args.steps)
+    if blocks:
+        log(f"{args.repeats} more blocks of {args.steps} steps: median "
+            f"{float(np.median(blocks)):.4f} ms/step")
 
     # per-kernel HIP-event pass on the context's stream
     ctx.profile(True)
@@ -603,6 +661,10 @@ def main():
                      "traffic": traffic, "bytes_per_launch": dom["bytes_per_launch"],
                      "avg_launch_us": avg_s * 1e6, "profiled_ms_per_step": prof_ms_step},
         "mass_drift": mass_drift,
+        "repeats": {"blocks": len(blocks), "steps_each": args.steps,
+                    "ms_per_step_median": float(np.median(blocks)) if blocks else None,
+                    "ms_per_step_min": min(blocks) if blocks else None,
+                    "ms_per_step_max": max(blocks) if blocks else None},
     }
     if not args.no_adjoint:
         log("adjoint (level 6, 200 steps) ...")
@@ -613,7 +675,7 @@ def main():
     if not args.no_ensemble:
         log("ensemble (256 members, 500 steps) ...")
         out["ensemble"] = ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps)
-        log(f"  {out['ensemble']['ms_per_step_batched_graph']:.3f} ms/step batched, "
+        log(f"  {out['ensemble']['ms_per_step_tuned']:.3f} ms/step batched, "
             f"{out['ensemble']['ms_per_step_sequential']:.3f} sequential, speedup "
             f"{out['ensemble']['speedup_vs_sequential']:.1f}")
     if not args.no_adjoint_ensemble:
@@ -625,8 +687,11 @@ def main():
             f"{ae['sampled_gradients_bitwise_equal_to_single_runs']}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.cpu_steps,
-                                           args.tracers)
+        out["cpu_baseline"] = cpu_baseline(mesh, u, h, tr, h_s, args.dt, args.tracers,
+                                           steps_1=args.cpu_steps, steps_all=2 * args.cpu_steps)
+        cb = out["cpu_baseline"]
+        log(f"  cpu: {cb['value']:.3e} cell-steps/s on {cb['cores']} threads, "
+            f"{cb['one_core']['value']:.3e} on one ({cb['cpu_model']}, nproc {cb['nproc']})")
     print(json.dumps(out), flush=True)
 
 

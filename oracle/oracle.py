@@ -47,6 +47,8 @@ def lib():
         L.orc_mesh_finalize.argtypes = [vp]
         L.orc_mesh_setup_fct.argtypes = [vp, i32, d]
         L.orc_migs.argtypes = [i32, P_f64, P_f64]
+        L.orc_set_threads.argtypes = [i32]
+        L.orc_set_threads(1)  # serial unless set_threads asks for more
         L.orc_mesh_cnx.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(P_i32),
                                    ctypes.POINTER(P_i32), ctypes.POINTER(i32)]
         L.orc_mesh_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(i64),
@@ -76,6 +78,12 @@ def lib():
             getattr(L, name).restype = None
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's gather loops (the scatter loops stay serial,
+    so every value keeps its bits); returns the count in use."""
+    return lib().orc_set_threads(int(n))
 
 
 def _f(a):

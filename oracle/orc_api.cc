@@ -9,6 +9,10 @@
 
 #include "orc_internal.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 using namespace orc;
 
 namespace {
@@ -327,6 +331,17 @@ int orc_mesh_set_dbl(void *h, const char *name, const double *p, int64_t n) {
 
 int orc_mesh_finalize(void *h) {
   return guard([&] { finalize_mesh(*static_cast<Mesh *>(h)); });
+}
+
+// Threads of the gather loops (ORC_PAR); returns the count in use.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }
 
 int orc_migs(int n, const double *a, double *x) {

@@ -15,6 +15,13 @@
 #include <string>
 #include <vector>
 
+// Loops whose every iteration writes only its own entity's values (gathers)
+// may run on several OpenMP threads: each value is then formed by the same
+// operations in the same order as on one thread, so the bits do not change.
+// The scatter loops of the reference (several entities adding into one, in
+// ascending source order) stay serial.  orc_set_threads picks the count.
+#define ORC_PAR _Pragma("omp parallel for schedule(static)")
+
 namespace orc {
 
 // FleCSI ragged connectivity: one list per entity.

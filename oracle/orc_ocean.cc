@@ -13,6 +13,7 @@ namespace orc {
 // apps/ocean/tasks/solve.cc:13-26
 void ocn_copy_tracers(const Mesh &m, int nL, int nT, const double *tr,
                       double *trNew, double *trPrev) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++)
       for (int t = 0; t < nT; t++) {
@@ -23,6 +24,7 @@ void ocn_copy_tracers(const Mesh &m, int nL, int nT, const double *tr,
 
 // apps/ocean/tasks/solve.cc:28-38
 void ocn_reinitialize_tracer_tendencies(const Mesh &m, int nL, int nT, double *tend) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++)
       for (int t = 0; t < nT; t++) tend[TR(c, k, t)] = 0.0;
@@ -33,6 +35,7 @@ void ocn_calc_normal_thickness_flux(const Mesh &m, int nL,
                                     const double *layerThicknessEdge,
                                     const double *normalVelocity,
                                     double *normalThicknessFlux) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int k = 0; k < nL; k++)
       normalThicknessFlux[e * nL + k] = normalVelocity[e * nL + k] * layerThicknessEdge[e * nL + k];
@@ -44,6 +47,7 @@ void ocn_calc_provisional_layer_thicknesses(const Mesh &m, int nL,
                                             const double *normalThicknessFlux,
                                             double *hProv, double *hProvInv,
                                             double *hNewInv, double dt) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {
     double dtInvAreaCell = dt / m.areaCell[c];
     int kmax = m.maxLevelCell[c];
@@ -67,6 +71,7 @@ void ocn_calc_provisional_layer_thicknesses(const Mesh &m, int nL,
 void ocn_low_order(const Mesh &m, int nL, int nT,
                    const double *normalThicknessFlux, const double *tracers,
                    double *lowOrderFlux) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {
     int c1 = m.e2c.at(e, 0), c2 = m.e2c.at(e, 1);
     for (int k = 0; k < nL; k++) {
@@ -83,8 +88,11 @@ void ocn_low_order(const Mesh &m, int nL, int nT,
 void ocn_high_order(const Mesh &m, int nL, int nT, double coef3rdOrder,
                     const double *normalThicknessFlux, const double *tracers,
                     double *highOrderFlux) {
+#pragma omp parallel
+  {
   std::vector<double> wgtTmp(nL), sgnTmp(nL);
   std::vector<std::vector<double>> flxTmp(nL, std::vector<double>(nT));
+#pragma omp for schedule(static)
   for (int e = 0; e < m.nE; e++) {
     for (int k = 0; k < nL; k++) {
       wgtTmp[k] = normalThicknessFlux[e * nL + k] * m.highOrderAdvMask[e * nL + k];
@@ -104,6 +112,7 @@ void ocn_high_order(const Mesh &m, int nL, int nT, double coef3rdOrder,
     for (int k = 0; k < nL; k++)
       for (int t = 0; t < nT; t++) highOrderFlux[(e * nL + k) * nT + t] = flxTmp[k][t];
   }
+  }
 }
 
 // apps/ocean/tasks/solve.cc:169-285
@@ -117,6 +126,7 @@ void ocn_correct_and_rescale(const Mesh &m, int nL, int nT,
                              double *tracerMin, double dt) {
   const double eps = 1.e-10;
 #define EF(e, k, t) (((e) * nL + (k)) * nT + (t))
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {  // 195-214
     for (int k = 0; k < m.maxLevelCell[c]; k++)
       for (int t = 0; t < nT; t++) {
@@ -135,6 +145,7 @@ void ocn_correct_and_rescale(const Mesh &m, int nL, int nT,
         }
     }
   }
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {  // 216-229
     int c1 = m.e2c.at(e, 0), c2 = m.e2c.at(e, 1);
     for (int k = 0; k < m.maxLevelEdgeTop[e]; k++) {
@@ -146,6 +157,7 @@ void ocn_correct_and_rescale(const Mesh &m, int nL, int nT,
       }
     }
   }
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {  // 232-267
     double invAreaCell1 = 1.0 / m.areaCell[c];
     int je = 0;
@@ -175,6 +187,7 @@ void ocn_correct_and_rescale(const Mesh &m, int nL, int nT,
         flxOut[TR(c, k, t)] = std::min(1.0, std::max(0.0, scaleFactor));
       }
   }
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {  // 270-282
     int c1 = m.e2c.at(e, 0), c2 = m.e2c.at(e, 1);
     for (int k = 0; k < m.maxLevelEdgeTop[e]; k++)
@@ -193,6 +206,7 @@ void ocn_accumulate_fct(const Mesh &m, int nL, int nT, const double *hProvInv,
                         const double *layerThickness,
                         const double *highOrderFlux, const double *tracers,
                         double *tracersProv, double *tend, double dt) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {
     double invAreaCell1 = 1.0 / m.areaCell[c];
     int je = 0;
@@ -216,6 +230,7 @@ void ocn_accumulate_fct(const Mesh &m, int nL, int nT, const double *hProvInv,
 void ocn_hmix_del2(const Mesh &m, int nL, int nT,
                    const double *layerThicknessEdge, const double *tracers,
                    double *tend, double coef) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {
     double invArea = 1.0 / m.areaCell[c];
     int je = 0;
@@ -239,6 +254,7 @@ void ocn_step_tracers_forward(const Mesh &m, int nL, int nT, const double *tend,
                               const double *tracers, double *tracersNew,
                               double *tracersPrev, double rk_weight,
                               double rk_subweight) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++)
       for (int k = 0; k < nT; k++) {
@@ -249,6 +265,7 @@ void ocn_step_tracers_forward(const Mesh &m, int nL, int nT, const double *tend,
 
 // apps/ocean/tasks/solve.cc:401-414
 void ocn_update_tracers(const Mesh &m, int nL, int nT, const double *trNew, double *tr) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++)
       for (int t = 0; t < nT; t++) tr[TR(c, k, t)] = trNew[TR(c, k, t)];

@@ -13,11 +13,13 @@ void sw_compute_solve_diagnostics(const Mesh &m, int nL, const double *h,
                                   double *h_vertex, double *circulation,
                                   double *vorticity, double *ke,
                                   double *pv_edge, double *pv_vertex) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {  // 31-39
     int cell1 = m.e2c.at(e, 0), cell2 = m.e2c.at(e, 1);
     for (int k = 0; k < nL; k++)
       h_edge[e * nL + k] = 0.5 * (h[cell1 * nL + k] + h[cell2 * nL + k]);
   }
+  ORC_PAR
   for (int v = 0; v < m.nV; v++)  // 42-46
     for (int k = 0; k < nL; k++) circulation[v * nL + k] = 0.0;
   for (int e = 0; e < m.nE; e++) {  // 47-55
@@ -27,10 +29,12 @@ void sw_compute_solve_diagnostics(const Mesh &m, int nL, const double *h,
       circulation[v2 * nL + k] = circulation[v2 * nL + k] + m.dcEdge[e] * u[e * nL + k];
     }
   }
+  ORC_PAR
   for (int v = 0; v < m.nV; v++)  // 56-60
     for (int k = 0; k < nL; k++)
       vorticity[v * nL + k] = circulation[v * nL + k] / m.areaTriangle[v];
 
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {  // 63-77
     for (int k = 0; k < nL; k++) ke[c * nL + k] = 0.0;
     for (int j = 0; j < m.c2e.size(c); j++) {
@@ -41,6 +45,7 @@ void sw_compute_solve_diagnostics(const Mesh &m, int nL, const double *h,
     for (int k = 0; k < nL; k++) ke[c * nL + k] /= m.areaCell[c];
   }
 
+  ORC_PAR
   for (int v = 0; v < m.nV; v++) {  // 81-92 (ascending cells paired with
     for (int k = 0; k < nL; k++) {  //   file-order kite areas, line 86)
       h_vertex[v * nL + k] = 0.0;
@@ -51,6 +56,7 @@ void sw_compute_solve_diagnostics(const Mesh &m, int nL, const double *h,
     }
   }
 
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)  // 95-99
     for (int k = 0; k < nL; k++) pv_edge[e * nL + k] = 0.0;
   for (int v = 0; v < m.nV; v++)  // 100-106
@@ -67,6 +73,7 @@ void sw_init_provis(const Mesh &m, int nL, int nT, const double *u_old,
                     const double *pv_edge_old, double *u_p, double *h_p,
                     double *tr_p, double *h_edge_p, double *ke_p,
                     double *pv_edge_p) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++) {
       h_p[c * nL + j] = h_old[c * nL + j];
@@ -74,6 +81,7 @@ void sw_init_provis(const Mesh &m, int nL, int nT, const double *u_old,
       for (int k = 0; k < nT; k++)
         tr_p[(c * nL + j) * nT + k] = tr_old[(c * nL + j) * nT + k];
     }
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int j = 0; j < nL; j++) {
       u_p[e * nL + j] = u_old[e * nL + j];
@@ -86,8 +94,10 @@ void sw_init_provis(const Mesh &m, int nL, int nT, const double *u_old,
 void sw_setup_timestep(const Mesh &m, int nL, int nT, const double *u_old,
                        const double *h_old, const double *tr_old, double *u_new,
                        double *h_new, double *tr_new) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int j = 0; j < nL; j++) u_new[e * nL + j] = u_old[e * nL + j];
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++) {
       h_new[c * nL + j] = h_old[c * nL + j];
@@ -101,6 +111,7 @@ void sw_compute_tend(const Mesh &m, int nL, double gravity,
                      const double *h_edge, const double *h, const double *h_s,
                      const double *u, const double *ke, const double *pv_edge,
                      double *tend_h, double *tend_u) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++) tend_h[c * nL + k] = 0;
   for (int e = 0; e < m.nE; e++) {  // 191-201
@@ -111,11 +122,14 @@ void sw_compute_tend(const Mesh &m, int nL, double gravity,
       tend_h[cell2 * nL + k] += flux;
     }
   }
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++) tend_h[c * nL + k] /= m.areaCell[c];
 
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int k = 0; k < nL; k++) tend_u[e * nL + k] = 0.0;
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {  // 215-233
     int cell1 = m.e2c.at(e, 0), cell2 = m.e2c.at(e, 1);
     for (int k = 0; k < nL; k++) {
@@ -139,6 +153,7 @@ void sw_compute_tend(const Mesh &m, int nL, double gravity,
 void sw_compute_scalar_tend(const Mesh &m, int nL, int nT,
                             const double *h_edge, const double *u,
                             const double *tracers, double *tracer_tend) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int k = 0; k < nL; k++)
       for (int i = 0; i < nT; i++) tracer_tend[(c * nL + k) * nT + i] = 0.0;
@@ -161,12 +176,15 @@ void sw_compute_substep(const Mesh &m, int nL, int nT, int freeze_u,
                         double *h_p, const double *h_old, const double *h_tend,
                         double *tr_p, const double *tr_old,
                         const double *tr_tend, double rkweight) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int j = 0; j < nL; j++)
       u_p[e * nL + j] = u_old[e * nL + j] + rkweight * u_tend[e * nL + j];
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++)
       h_p[c * nL + j] = h_old[c * nL + j] + rkweight * h_tend[c * nL + j];
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++)
       for (int k = 0; k < nT; k++)
@@ -184,8 +202,10 @@ void sw_accumulate_update(const Mesh &m, int nL, int nT, double *u_new,
                           const double *u_tend, double *h_new,
                           const double *h_tend, double *tr_new,
                           const double *tr_tend, double rkweight) {
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int j = 0; j < nL; j++) u_new[e * nL + j] += rkweight * u_tend[e * nL + j];
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++) {
       h_new[c * nL + j] += rkweight * h_tend[c * nL + j];
@@ -198,6 +218,7 @@ void sw_accumulate_update(const Mesh &m, int nL, int nT, double *u_new,
 void sw_finalize_timestep(const Mesh &m, int nL, int nT, int freeze_u,
                           const double *u_old, double *u_new,
                           const double *h_new, double *tr_new) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++)
       for (int k = 0; k < nT; k++) tr_new[(c * nL + j) * nT + k] /= h_new[c * nL + j];
@@ -212,6 +233,7 @@ void sw_timeshift(const Mesh &m, int nL, int nT, double *u_old, double *h_old,
                   double *pv_edge_old, const double *u_new, const double *h_new,
                   const double *tr_new, const double *h_edge_new,
                   const double *ke_new, const double *pv_edge_new) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++)
     for (int j = 0; j < nL; j++) {
       h_old[c * nL + j] = h_new[c * nL + j];
@@ -219,6 +241,7 @@ void sw_timeshift(const Mesh &m, int nL, int nT, double *u_old, double *h_old,
       for (int k = 0; k < nT; k++)
         tr_old[(c * nL + j) * nT + k] = tr_new[(c * nL + j) * nT + k];
     }
+  ORC_PAR
   for (int e = 0; e < m.nE; e++)
     for (int j = 0; j < nL; j++) {
       u_old[e * nL + j] = u_new[e * nL + j];
@@ -269,6 +292,7 @@ double sw_compute_error(const Mesh &m, int nL, double a, const double *h,
 void sw_del2_viscosity(const Mesh &m, int nL, double visc2, const double *u,
                        const double *vorticity, double *divergence,
                        double *tend_u) {
+  ORC_PAR
   for (int c = 0; c < m.nC; c++) {
     for (int k = 0; k < nL; k++) divergence[c * nL + k] = 0.0;
     for (int j = 0; j < m.c2e.size(c); j++) {
@@ -279,6 +303,7 @@ void sw_del2_viscosity(const Mesh &m, int nL, double visc2, const double *u,
     }
     for (int k = 0; k < nL; k++) divergence[c * nL + k] /= m.areaCell[c];
   }
+  ORC_PAR
   for (int e = 0; e < m.nE; e++) {
     int c1 = m.e2c.at(e, 0), c2 = m.e2c.at(e, 1);
     int v1 = m.e2v.at(e, 0), v2 = m.e2v.at(e, 1);
