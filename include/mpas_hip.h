@@ -388,6 +388,22 @@ int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0,
                             const double *h0, const double *tr0, double *u1,
                             double *h1, double *tr1);
 
+/* Tile and schedule of the TLM and adjoint (no reference counterpart):
+   workgroup sizes of the adjoint and tangent-linear kernels (multiples of
+   64 up to 512), whether each segment's replay runs on a second stream
+   beside the sweep of the segment after it (a second tape, when the device
+   has the memory), and whether each segment's reverse sweep is replayed
+   from a hipGraph.  Results do not depend on them, bit for bit.
+   mpas_adj_autotune times gradients (forward with checkpoints + sweep) of
+   nsteps steps from the resident state for each candidate (schedule, then
+   the adjoint kernels' size) and TLM runs for the tangent-linear kernels'
+   size, keeps the fastest, restores the resident state and writes the
+   trials as JSON into report; the tape is left empty. */
+int mpas_adj_config(mpas_ctx *ctx, int ad_block, int tl_block, int overlap, int graph);
+int mpas_adj_get_config(const mpas_ctx *ctx, int *ad_block, int *tl_block, int *overlap,
+                        int *graph);
+int mpas_adj_autotune(mpas_ctx *ctx, double dt, int nsteps, char *report, size_t cap);
+
 /* ---- ensembles -------------------------------------------------------------
    No counterpart in the reference (one mesh per run there): many members,
    on the meshes of several contexts, advanced together by the fused step.

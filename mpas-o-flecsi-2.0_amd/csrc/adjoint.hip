@@ -27,7 +27,10 @@
 // antidiffusive fluxes into a tape of 4K + 1 stage records (the replay is
 // bit-identical to the first pass), then sweeps the segment in reverse.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "ctx.h"
@@ -98,7 +101,18 @@ struct AdjState {
   int spare_state = 0;  // 0 not tried, 1 allocated, -1 not enough memory
   hipStream_t replay_stream = nullptr;
   hipEvent_t ev_swept = nullptr, ev_replayed = nullptr;
+  // tile and schedule: workgroup sizes of the adjoint (k_ad_*) and
+  // tangent-linear (k_tl_*) kernels; replays beside the sweep (spare tape)
+  // or in turn; each segment's reverse sweep replayed from a hipGraph
+  // (captured per tape, parity and length) or launched directly
+  int ad_block = 256, tl_block = 256, overlap = 1, graph = 0;
+  std::map<std::tuple<const void *, const void *, int, int>, hipGraphExec_t> sweeps;
 };
+
+void drop_sweeps(AdjState &A) {
+  for (auto &kv : A.sweeps) (void)hipGraphExecDestroy(kv.second);
+  A.sweeps.clear();
+}
 
 void adj_free(mpas_ctx *ctx) {
   if (!ctx->adj) return;
@@ -107,6 +121,7 @@ void adj_free(mpas_ctx *ctx) {
     (void)hipStreamSynchronize(A->replay_stream);
     (void)hipStreamDestroy(A->replay_stream);
   }
+  drop_sweeps(*A);
   if (A->ev_swept) (void)hipEventDestroy(A->ev_swept);
   if (A->ev_replayed) (void)hipEventDestroy(A->ev_replayed);
   for (void *p : A->bufs) (void)hipFree(p);
@@ -116,7 +131,9 @@ void adj_free(mpas_ctx *ctx) {
 
 namespace {
 
-constexpr int BS = 256;
+// Workgroup sizes are chosen at run time (64 to 512; mpas_adj_config,
+// mpas_adj_autotune): the kernels are built for the largest.
+constexpr int kAdjMaxBlock = 512;
 
 // Per-stage constants of the linearised kernels.
 struct AdjParams {
@@ -168,7 +185,7 @@ __device__ __forceinline__ void vertex_state(const DMesh &M, int v, uint32_t k, 
 
 // ---- tangent-linear model -----------------------------------------------------
 
-__global__ __launch_bounds__(BS) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ du,
                                                   const double *__restrict__ dh, double *dvort,
                                                   double *dpvv) {
@@ -205,7 +222,7 @@ __global__ __launch_bounds__(BS) void k_tl_vertex(DMesh M, AdjParams P, Rec R,
 // h_edge and pv_edge perturbations, and the perturbation of the edge's net
 // tracer flux into its cells: (1 - lambda) dLO + lambda dHO - dH (upwind,
 // 3rd-order, del2 mixing flux).
-__global__ __launch_bounds__(BS) void k_tl_edge_a(DMesh M, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_tl_edge_a(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ du,
                                                   const double *__restrict__ dh,
                                                   const double *__restrict__ dT,
@@ -255,7 +272,7 @@ __global__ __launch_bounds__(BS) void k_tl_edge_a(DMesh M, AdjParams P, Rec R,
 // Cell part: ke / divergence perturbations, tend_h and the tracer tendency,
 // then the RK4 bookkeeping of h and the tracers (solve.cc:297-305,
 // 332-340, 350-357 differentiated).
-__global__ __launch_bounds__(BS) void k_tl_cell(DMesh M, AdjParams P, Rec R, Rec Ry, Rec R0,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_tl_cell(DMesh M, AdjParams P, Rec R, Rec Ry, Rec R0,
                                                 const double *__restrict__ du,
                                                 const double *__restrict__ dh0,
                                                 const double *__restrict__ dT0,
@@ -321,7 +338,7 @@ __global__ __launch_bounds__(BS) void k_tl_cell(DMesh M, AdjParams P, Rec R, Rec
 
 // tend_u perturbation (PV flux, ke and height gradients, del2 viscosity) and
 // the RK4 bookkeeping of u.
-__global__ __launch_bounds__(BS) void k_tl_edge_b(DMesh M, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_tl_edge_b(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ du,
                                                   const double *__restrict__ du0,
                                                   const double *__restrict__ dh,
@@ -381,7 +398,7 @@ __global__ __launch_bounds__(BS) void k_tl_edge_b(DMesh M, AdjParams P, Rec R,
 // the final division T = Q / h reversed (aw_h = h adjoint after it, aw_tr =
 // Q adjoint), the adjoint of the step's initial state seeded with the
 // direct paths, and the stage-3 tendency adjoints.
-__global__ __launch_bounds__(BS) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Rec R0,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Rec R0,
                                                 const double *__restrict__ ub_in,
                                                 const double *__restrict__ hb_in,
                                                 const double *__restrict__ trb_in,
@@ -444,7 +461,7 @@ static_assert(kChunk >= 16, "the edge part reads a cell's advection slots in one
 // slots of every ragged list (edgesOnEdge, its transpose, the advection
 // stencil); one lane exchange sums the partials and lane 0 finishes.
 constexpr int kEdgeLanes = 2;
-__global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
                                                   const double *__restrict__ tb_u,
                                                   const double *__restrict__ tb_h,
                                                   const double *__restrict__ tb_tr, double *ubA,
@@ -565,7 +582,7 @@ __global__ __launch_bounds__(BS) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams 
 // Vertex part: pv_vertex = (f + vorticity) / h_vertex and the viscosity's
 // vorticity term reversed; vT and hvT are the vorticity and h_vertex
 // adjoints divided by areaTriangle.
-__global__ __launch_bounds__(BS) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ pveb,
                                                   const double *__restrict__ tb_u, double *vT,
                                                   double *hvT) {
@@ -611,7 +628,7 @@ __device__ __forceinline__ double cell_lanes_sum(double x) {
   x += __shfl_xor(x, 2);
   return x;
 }
-__global__ __launch_bounds__(BS) void k_ad_cell(
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     DMesh M, AdjMesh A, AdjParams P, Rec R, Rec R0, const double *__restrict__ tb_u,
     const double *__restrict__ heb, const double *__restrict__ hvT,
     const double *__restrict__ r1, const double *__restrict__ r2,
@@ -682,7 +699,7 @@ __global__ __launch_bounds__(BS) void k_ad_cell(
 
 // Edge part 2: the u adjoint of the stage input completed with the paths
 // through ke, the divergence and the vorticity, then the bookkeeping of u.
-__global__ __launch_bounds__(BS) void k_ad_edge_b(DMesh M, AdjParams P, Rec R,
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_b(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ ubA,
                                                   const double *__restrict__ keT,
                                                   const double *__restrict__ divT,
@@ -999,15 +1016,17 @@ AdjParams adj_params(const mpas_ctx *ctx, int s, double dt) {
   return P;
 }
 
-#define ADJ_LAUNCH(kernel, n, ...)                                                         \
+#define ADJ_LAUNCH_BS(bs, kernel, n, ...)                                                  \
   do {                                                                                     \
     if (int rc_ = prof_begin(ctx, #kernel, 0.0)) return rc_;                               \
-    hipLaunchKernelGGL(kernel, dim3(xcd_grid(int64_t(n) * M.nL, BS)), dim3(BS), 0,         \
+    hipLaunchKernelGGL(kernel, dim3(xcd_grid(int64_t(n) * M.nL, bs)), dim3(bs), 0,         \
                        ctx->stream,                                                        \
                        __VA_ARGS__);                                                       \
     if (int rc_ = after_launch(#kernel)) return rc_;                                       \
     if (int rc_ = prof_end(ctx)) return rc_;                                               \
   } while (0)
+#define ADJ_LAUNCH(kernel, n, ...) ADJ_LAUNCH_BS(A.ad_block, kernel, n, __VA_ARGS__)
+#define TLM_LAUNCH(kernel, n, ...) ADJ_LAUNCH_BS(A.tl_block, kernel, n, __VA_ARGS__)
 
 // TLM of one step along records 0..4: dX[0] -> dX[4].
 int tlm_step(mpas_ctx *ctx, AdjState &A, double dt) {
@@ -1015,12 +1034,12 @@ int tlm_step(mpas_ctx *ctx, AdjState &A, double dt) {
   for (int s = 0; s < 4; s++) {
     const AdjParams P = adj_params(ctx, s, dt);
     const Rec &R = A.rec[s], &Ry = A.rec[s + 1], &R0 = A.rec[0];
-    ADJ_LAUNCH(k_tl_vertex, M.nV, M, P, R, A.du[s], A.dh[s], A.dvort, A.dpvv);
-    ADJ_LAUNCH(k_tl_edge_a, M.nE, M, P, R, A.du[s], A.dh[s], A.dT[s], A.dpvv, A.dhe, A.dpve,
+    TLM_LAUNCH(k_tl_vertex, M.nV, M, P, R, A.du[s], A.dh[s], A.dvort, A.dpvv);
+    TLM_LAUNCH(k_tl_edge_a, M.nE, M, P, R, A.du[s], A.dh[s], A.dT[s], A.dpvv, A.dhe, A.dpve,
                A.dflux);
-    ADJ_LAUNCH(k_tl_cell, M.nC, M, P, R, Ry, R0, A.du[s], A.dh[0], A.dT[0], A.dhe, A.dflux,
+    TLM_LAUNCH(k_tl_cell, M.nC, M, P, R, Ry, R0, A.du[s], A.dh[0], A.dT[0], A.dhe, A.dflux,
                A.dnh, A.dQ, A.dke, A.ddiv, A.dh[s + 1], A.dT[s + 1]);
-    ADJ_LAUNCH(k_tl_edge_b, M.nE, M, P, R, A.du[s], A.du[0], A.dh[s], A.dhe, A.dpve, A.dke,
+    TLM_LAUNCH(k_tl_edge_b, M.nE, M, P, R, A.du[s], A.du[0], A.dh[s], A.dhe, A.dpve, A.dke,
                A.ddiv, A.dvort, A.dnu, A.du[s + 1]);
   }
   return 0;
@@ -1050,6 +1069,48 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
     ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
                A.tb_u);
   }
+  return 0;
+}
+
+// The reverse sweep of the len steps the tape holds, from the adjoint in
+// xb[src]: launched directly, or replayed from a hipGraph captured the first
+// time for this tape, parity, length and time step.
+int sweep_segment(mpas_ctx *ctx, AdjState &A, int len, int src) {
+  auto direct = [&]() -> int {
+    int x = src;
+    for (int i = len - 1; i >= 0; i--) {
+      if (int rc = adj_step(ctx, A, i, A.dt, x)) return rc;
+      x = 1 - x;
+    }
+    return 0;
+  };
+  if (!A.graph || ctx->prof.on) return direct();
+  // the tape's buffers: only records 0 and 4 ever trade places (the TLM's
+  // step), so these two name the whole set
+  const auto key = std::make_tuple(static_cast<const void *>(A.rec[0].u),
+                                   static_cast<const void *>(A.rec[4].u), src, len);
+  auto it = A.sweeps.find(key);
+  if (it != A.sweeps.end()) {
+    MPAS_HIP(hipGraphLaunch(it->second, ctx->stream));
+    return 0;
+  }
+  MPAS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = direct();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess)
+    MPAS_FAIL(MPAS_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess)
+    MPAS_FAIL(MPAS_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  A.sweeps[key] = x;
+  MPAS_HIP(hipGraphLaunch(x, ctx->stream));
   return 0;
 }
 
@@ -1146,6 +1207,7 @@ int mpas_adj_forward(mpas_ctx *ctx, double dt, int nsteps, const double *u0, con
       (tr1 && (rc = copy_d2d(ctx, tr1, W.tr, T))))
     return rc;
   A.nsteps = nsteps;
+  if (A.dt != dt) drop_sweeps(A);  // the sweeps hold the stage weights
   A.dt = dt;
   A.forward_done = true;
   A.taped = 0;
@@ -1168,8 +1230,8 @@ int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
     return rc;
   const int nseg = (A.nsteps + A.K - 1) / A.K;
   auto seg_len = [&](int m) { return std::min(A.K, A.nsteps - m * A.K); };
-  if (nseg > 1 && (rc = try_spare(ctx, A))) return rc;
-  const bool overlap = nseg > 1 && A.spare_state == 1;
+  if (nseg > 1 && A.overlap && (rc = try_spare(ctx, A))) return rc;
+  const bool overlap = nseg > 1 && A.overlap && A.spare_state == 1;
   hipStream_t rs = A.replay_stream;
   if (overlap) {  // the replays read checkpoints the context's stream wrote
     MPAS_HIP(hipEventRecord(A.ev_swept, ctx->stream));
@@ -1190,10 +1252,8 @@ int mpas_adj_backward(mpas_ctx *ctx, const double *u1_bar, const double *h1_bar,
     const bool ahead = overlap && m > 0;
     if (ahead && (rc = replay(ctx, A, spare(A), m - 1, seg_len(m - 1), rs))) return rc;
     if (ahead) MPAS_HIP(hipEventRecord(A.ev_replayed, rs));
-    for (int i = len - 1; i >= 0; i--) {
-      if ((rc = adj_step(ctx, A, i, A.dt, src))) return rc;
-      src = 1 - src;
-    }
+    if ((rc = sweep_segment(ctx, A, len, src))) return rc;
+    if (len % 2) src = 1 - src;
     A.taped = len;
     if (ahead) {
       MPAS_HIP(hipEventRecord(A.ev_swept, ctx->stream));
@@ -1268,6 +1328,150 @@ int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0, const d
   if ((u1 && (rc = copy_d2d(ctx, u1, W.u, E))) || (h1 && (rc = copy_d2d(ctx, h1, W.h, C))) ||
       (tr1 && (rc = copy_d2d(ctx, tr1, W.tr, T))))
     return rc;
+  return 0;
+}
+
+int mpas_adj_config(mpas_ctx *ctx, int ad_block, int tl_block, int overlap, int graph) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!ctx->adj) MPAS_FAIL(MPAS_ESTATE, "mpas_adj_init has not been called");
+  for (int b : {ad_block, tl_block})
+    if (b < 64 || b > kAdjMaxBlock || b % 64)
+      MPAS_FAIL(MPAS_EINVAL, "mpas_adj_config: workgroup sizes are multiples of 64 up to 512");
+  AdjState &A = *ctx->adj;
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  if (A.ad_block != ad_block) drop_sweeps(A);
+  A.ad_block = ad_block;
+  A.tl_block = tl_block;
+  A.overlap = overlap != 0;
+  A.graph = graph != 0;
+  return 0;
+}
+
+int mpas_adj_get_config(const mpas_ctx *ctx, int *ad_block, int *tl_block, int *overlap,
+                        int *graph) {
+  if (!ctx || !ctx->adj) MPAS_FAIL(MPAS_ESTATE, "mpas_adj_init has not been called");
+  const AdjState &A = *ctx->adj;
+  if (ad_block) *ad_block = A.ad_block;
+  if (tl_block) *tl_block = A.tl_block;
+  if (overlap) *overlap = A.overlap;
+  if (graph) *graph = A.graph;
+  return 0;
+}
+
+int mpas_adj_autotune(mpas_ctx *ctx, double dt, int nsteps, char *report, size_t cap) {
+  if (int rc = check_adj(ctx)) return rc;
+  if (nsteps < 1) MPAS_FAIL(MPAS_EINVAL, "mpas_adj_autotune: nsteps must be >= 1");
+  if (!(dt > 0)) MPAS_FAIL(MPAS_EINVAL, "mpas_adj_autotune: dt must be positive");
+  AdjState &A = *ctx->adj;
+  SwState &W = *ctx->sw;
+  const DMesh &M = ctx->dm;
+  const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, T = C * M.nT;
+  MPAS_HIP(hipStreamSynchronize(ctx->stream));
+  // the trials start from the resident state, which they advance: keep it
+  // (x0) and restore it after; y and xb are the sweep's input and output
+  std::vector<void *> tmp;
+  auto get = [&](size_t n, double **p) -> int {
+    *p = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(p), sizeof(double) * std::max<size_t>(n, 1)) !=
+        hipSuccess)
+      MPAS_FAIL(MPAS_ENOMEM, "mpas_adj_autotune: no memory for the trial state");
+    tmp.push_back(*p);
+    return 0;
+  };
+  double *x0[3], *y[3], *xb[3], *dx[3];
+  int rc = 0;
+  const size_t n3[3] = {E, C, T};
+  for (int k = 0; k < 3 && !rc; k++)
+    if (!(rc = get(n3[k], &x0[k])) && !(rc = get(n3[k], &y[k])) && !(rc = get(n3[k], &xb[k])))
+      rc = get(n3[k], &dx[k]);
+  const double *cur[3] = {W.u, W.h, W.tr};
+  for (int k = 0; k < 3 && !rc; k++)
+    if (n3[k] && hipMemcpyAsync(x0[k], cur[k], sizeof(double) * n3[k], hipMemcpyDeviceToDevice,
+                                ctx->stream) != hipSuccess)
+      rc = MPAS_EHIP;
+  std::string log = "[";
+  // seconds of one gradient (forward with checkpoints + sweep) or one TLM run
+  auto timed = [&](bool tlm) -> double {
+    if (rc) return 1e30;
+    double best = 1e30;
+    for (int rep = 0; rep < 2 && !rc; rep++) {  // the first run warms up (graph capture)
+      (void)hipStreamSynchronize(ctx->stream);
+      const auto t0 = std::chrono::steady_clock::now();
+      if (tlm)
+        rc = mpas_tlm_forward(ctx, dt, nsteps, x0[0], x0[1], x0[2], x0[0], x0[1], x0[2],
+                              nullptr, nullptr, nullptr, dx[0], dx[1], dx[2]);
+      else if (!(rc = mpas_adj_forward(ctx, dt, nsteps, x0[0], x0[1], x0[2], y[0], y[1], y[2])))
+        rc = mpas_adj_backward(ctx, y[0], y[1], y[2], xb[0], xb[1], xb[2]);
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = MPAS_EHIP;
+      best = std::min(best, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                                .count());
+    }
+    return rc ? 1e30 : best;
+  };
+  auto note = [&](const char *what, double t) {
+    char line[200];
+    std::snprintf(line, sizeof line,
+                  "%s{\"trial\": \"%s\", \"ad_block\": %d, \"tl_block\": %d, "
+                  "\"overlap\": %d, \"graph\": %d, \"ms\": %.4f}",
+                  log.size() > 1 ? ", " : "", what, A.ad_block, A.tl_block, A.overlap, A.graph,
+                  1e3 * t);
+    log += line;
+  };
+  const int sizes[] = {64, 128, 256, 512};
+  double best = timed(false);
+  note("start", best);
+  // the schedule: replays beside the sweep or in turn, sweeps from graphs or not
+  for (int ov = 0; ov < 2; ov++)
+    for (int gr = 0; gr < 2; gr++) {
+      if (ov == A.overlap && gr == A.graph) continue;
+      const int po = A.overlap, pg = A.graph;
+      A.overlap = ov;
+      A.graph = gr;
+      const double t = timed(false);
+      note("schedule", t);
+      if (t < best) best = t;
+      else A.overlap = po, A.graph = pg;
+    }
+  // the tile of the adjoint kernels
+  for (int bs : sizes) {
+    if (bs == A.ad_block) continue;
+    const int keep = A.ad_block;
+    drop_sweeps(A);
+    A.ad_block = bs;
+    const double t = timed(false);
+    note("adjoint block", t);
+    if (t < best) best = t;
+    else drop_sweeps(A), A.ad_block = keep;
+  }
+  // the tile of the tangent-linear kernels, on TLM runs
+  double tbest = timed(true);
+  note("tlm start", tbest);
+  for (int bs : sizes) {
+    if (bs == A.tl_block) continue;
+    const int keep = A.tl_block;
+    A.tl_block = bs;
+    const double t = timed(true);
+    note("tlm block", t);
+    if (t < tbest) tbest = t;
+    else A.tl_block = keep;
+  }
+  // the resident state as it was; the tape holds the trials' runs
+  for (int k = 0; k < 3 && !rc; k++)
+    if (n3[k] && hipMemcpyAsync(const_cast<double *>(cur[k]), x0[k], sizeof(double) * n3[k],
+                                hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+      rc = MPAS_EHIP;
+  if (!rc) rc = sw_diagnostics(ctx, W.u, W.h, W.cur);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : tmp) (void)hipFree(p);
+  A.forward_done = false;
+  A.taped = 0;
+  A.tape_seg = -1;
+  if (rc) return rc;
+  log += "]";
+  if (report && cap) {
+    if (log.size() + 1 > cap) MPAS_FAIL(MPAS_ESHAPE, "mpas_adj_autotune: report buffer too small");
+    std::memcpy(report, log.c_str(), log.size() + 1);
+  }
   return 0;
 }
 

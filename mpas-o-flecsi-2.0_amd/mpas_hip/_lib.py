@@ -143,6 +143,9 @@ SIGNATURES = {
     "mpas_adj_backward": (c_int, [c_void_p] + [_D] * 6),
     "mpas_tlm_forward": (c_int, [c_void_p, c_double, c_int] + [_D] * 12),
     "mpas_adj_frozen_forward": (c_int, [c_void_p, c_int] + [_D] * 6),
+    "mpas_adj_config": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "mpas_adj_get_config": (c_int, [c_void_p] + [ctypes.POINTER(c_int)] * 4),
+    "mpas_adj_autotune": (c_int, [c_void_p, c_double, c_int, c_char_p, c_size_t]),
     "mpas_batch_create": (c_int, [ctypes.POINTER(c_void_p), c_int, c_int, P_i32, P_f64,
                                    ctypes.POINTER(c_void_p)]),
     "mpas_batch_destroy": (c_int, [c_void_p]),

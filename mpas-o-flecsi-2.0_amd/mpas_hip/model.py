@@ -441,6 +441,24 @@ class Context:
                                          _ptr(x1[1]), self._trp(x1[2]), _ptr(dx1[0]),
                                          _ptr(dx1[1]), self._trp(dx1[2])), "mpas_tlm_forward")
 
+    def adj_config(self, ad_block=256, tl_block=256, overlap=True, graph=False):
+        L.check(L.hip().mpas_adj_config(self._h, int(ad_block), int(tl_block), int(bool(overlap)),
+                                        int(bool(graph))), "mpas_adj_config")
+
+    def adj_get_config(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        L.check(L.hip().mpas_adj_get_config(self._h, *[ctypes.byref(x) for x in v]),
+                "mpas_adj_get_config")
+        return dict(zip(("ad_block", "tl_block", "overlap", "graph"), (x.value for x in v)))
+
+    def adj_autotune(self, dt, nsteps):
+        """mpas_adj_autotune from the resident state; returns the trials."""
+        import json
+        buf = ctypes.create_string_buffer(1 << 14)
+        L.check(L.hip().mpas_adj_autotune(self._h, float(dt), int(nsteps), buf, len(buf)),
+                "mpas_adj_autotune")
+        return json.loads(buf.value.decode())
+
     def adj_frozen_forward(self, nsteps, x0, x1):
         L.check(L.hip().mpas_adj_frozen_forward(self._h, int(nsteps), _ptr(x0[0]), _ptr(x0[1]),
                                                 self._trp(x0[2]), _ptr(x1[0]), _ptr(x1[1]),

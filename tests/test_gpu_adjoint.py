@@ -175,6 +175,42 @@ def test_torch_autograd_observation_loss():
     assert res["grad_finite"] and res["rel_adjoint_vs_tlm"] < DOT_TOL
 
 
+def test_tile_and_schedule_keep_the_gradient():
+    """Workgroup sizes of the adjoint and TLM kernels, the replay beside the
+    sweep (second tape and stream) or in turn, and graph-replayed segment
+    sweeps: the gradient and the TLM are the same bits in every
+    configuration and after mpas_adj_autotune, over several segments."""
+    m, ctx, x0, _ = _setup(4, 2, K=3)
+    rng = np.random.default_rng(21)
+    y_h, y = _rand_state(ctx, rng)
+    dx_h, dx = _rand_state(ctx, rng)
+    dt, nsteps = 600.0, 8  # segments of 3, 3 and 2 steps
+
+    def run():
+        xb, dxN = ctx.state_arrays(), ctx.state_arrays()
+        ctx.adj_forward(dt, nsteps, x0)
+        ctx.adj_backward(y, xb)
+        ctx.adj_backward(y, ctx.state_arrays())  # a second sweep replays the graphs
+        ctx.tlm_forward(dt, nsteps, x0, dx, dx1=dxN)
+        ctx.sync()
+        return _get(ctx, xb) + _get(ctx, dxN)
+
+    ref = run()
+    for ad, tl, ov, gr in ((64, 512, True, True), (512, 64, False, True), (128, 128, True, False),
+                           (256, 256, False, False)):
+        ctx.adj_config(ad, tl, ov, gr)
+        assert ctx.adj_get_config() == dict(ad_block=ad, tl_block=tl, overlap=int(ov),
+                                            graph=int(gr))
+        for a, b in zip(run(), ref):
+            assert bitwise_equal(a, b), (ad, tl, ov, gr)
+    trials = ctx.adj_autotune(dt, nsteps)
+    assert len(trials) >= 8 and all(t["ms"] > 0 for t in trials)
+    print("adjoint tuned:", ctx.adj_get_config())
+    for a, b in zip(run(), ref):
+        assert bitwise_equal(a, b), "tuned"
+    ctx.close()
+
+
 def test_adjoint_error_paths():
     from mpas_hip import TRACER_CENTRED, Context
     from mpas_hip._lib import MpasError
