@@ -233,6 +233,41 @@ int build_device_mesh(mpas_ctx *ctx) {
         (rc = upload(ctx, slot_pairs(c1, padE, E), &M.advCoefP)) ||
         (rc = upload(ctx, slot_pairs(c3, padE, E), &M.advCoef3P)))
       return rc;
+    // advCoef_3rd in compact form: a slot that only one of the edge's cell
+    // fits holds has advCoef_3rd = +-advCoef exactly (init_functions.cc:
+    // 394-437 forms both from the same single derivTwo term); the code is
+    // taken from the bits themselves, so the kernel reproduces every value
+    {
+      std::vector<uint32_t> side(E, 0);
+      std::vector<double> x(size_t(kAdvExplicit) * E, 0.0);
+      bool fits = M.narrow;
+      for (int e = 0; e < E && fits; e++) {
+        int k = 0;
+        for (int i = 0; i < padE; i++) {
+          const double a = c1[size_t(i) * E + e], b = c3[size_t(i) * E + e];
+          uint32_t code;
+          if (std::memcmp(&a, &b, sizeof a) == 0) {
+            code = 1;
+          } else {
+            const double na_ = -a;
+            if (std::memcmp(&na_, &b, sizeof a) == 0) {
+              code = 2;
+            } else {
+              code = 0;
+              if (k == kAdvExplicit) {
+                fits = false;
+                break;
+              }
+              x[size_t(k++) * E + e] = b;
+            }
+          }
+          side[e] |= code << (2 * i);
+        }
+      }
+      if (fits && ((rc = upload(ctx, side, &M.advSide)) ||
+                   (rc = upload(ctx, slot_pairs(x, kAdvExplicit, E), &M.advC3xP))))
+        return rc;
+    }
     bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
                             [&](int x) { return x == m.nL; });
     if (!full) {

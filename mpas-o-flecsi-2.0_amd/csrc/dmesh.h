@@ -56,12 +56,23 @@ struct DMesh {
   const int *advP;                      // advCellsForEdge, [j/2][E][2]
   int advC12;  // 1: every edge's advection cells start with cells(e)[0], [1]
   const double *advCoefP, *advCoef3P;   // advCoef, advCoef_3rd, [j/2][E][2]
+  // advCoef_3rd in compact form (one-level kernels): per edge, 2 bits per
+  // slot say whether the slot's advCoef_3rd is advCoef (1), -advCoef (2) -
+  // true bit for bit of every slot only one of the edge's two cell fits
+  // holds - or the next of the edge's explicit values (0), which are
+  // [kAdvExplicit/2][E][2]; nullptr when some edge has more explicit slots
+  const uint32_t *advSide;
+  const double *advC3xP;
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
   double msd;         // meshScalingDel2: one value for every edge
   double gravity, coef3rd;
 };
+
+// explicit advCoef_3rd values per edge in the compact form: the two cells of
+// the edge and their two common neighbours (the slots both cell fits hold)
+constexpr int kAdvExplicit = 4;
 
 // std::max / std::min exactly as <algorithm> defines them (the result for
 // signed zeros and NaN follows the comparison, unlike fmax / fmin).

@@ -212,7 +212,11 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
     b.edges += touches * d * L * E;                                // u0 / nu / yu
   }
   // k_stage_ho: antidiffusive flux per edge and tracer
-  if (fct) b.ho = 8 * E + d * E + E + 20 * sum_adv + 2 * d * L * E + d * L * T * (C + E);
+  // advection stencil: id + advCoef (+ advCoef_3rd, or in compact form a
+  // side code and the explicit values per edge for one-level kernels)
+  const bool compact = MPAS_COMPACT_C3 && M.advSide && L == 1;
+  const double stencil = compact ? 12 * sum_adv + 4 * E + d * kAdvExplicit * E : 20 * sum_adv;
+  if (fct) b.ho = 8 * E + d * E + E + stencil + 2 * d * L * E + d * L * T * (C + E);
   // k_stage_cells
   b.cells = 14 * C + 4 * sum_eoc + 2 * d * E;                      // nEoC sgn rank area, c2e, dv dc
   b.cells += d * L * 3 * E;                                        // yu xu h_edge

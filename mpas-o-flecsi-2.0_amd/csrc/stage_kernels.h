@@ -106,6 +106,12 @@ __device__ __forceinline__ void grp_row(T (&out)[W], const T (&mine)[MS], int ba
 #define MPAS_PAIRED_STENCIL 1
 #endif
 
+// The one-level high-order flux kernel reads advCoef_3rd in compact form
+// (DMesh::advSide) where the context built it.
+#ifndef MPAS_COMPACT_C3
+#define MPAS_COMPACT_C3 1
+#endif
+
 // Slots of the momentum kernel's edgesOnEdge gathers per scheduling phase
 // with levels on lanes (0: all at once).
 #ifndef MPAS_QCH
@@ -340,6 +346,38 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     bcast_row<WA, double>(c3, ld(M.advCoef3, i * E + e), base);
 #pragma unroll
     for (int j = 0; j < WA; j++) c3[j] = c3[j] * M.coef3rd;
+  } else if (LOL == 0 && MPAS_PAIRED_STENCIL && MPAS_COMPACT_C3 && M.advP && M.advSide) {
+    // one access per slot pair; advCoef_3rd from its side code: +-advCoef,
+    // or the edge's next explicit value (picked by selects: a run-time
+    // index into a register array would go to scratch memory)
+    const uint32_t side = ldn<kNtStencil>(M.advSide, e);
+    double x[kAdvExplicit];
+#pragma unroll
+    for (int p = 0; p < kAdvExplicit / 2; p++) {
+      const mpas_d2v v = ld_vec<kNtStencil, mpas_d2v>(M.advC3xP, p * E + e);
+      x[2 * p] = v.x;
+      x[2 * p + 1] = v.y;
+    }
+#pragma unroll
+    for (int p = 0; p < WA / 2; p++) {
+      const mpas_i2v ij = ld_vec<kNtStencil, mpas_i2v>(M.advP, p * E + e);
+      const mpas_d2v a1 = ld_vec<kNtStencil, mpas_d2v>(M.advCoefP, p * E + e);
+      ic[2 * p] = uint32_t(ij.x);
+      ic[2 * p + 1] = uint32_t(ij.y);
+      c1[2 * p] = a1.x;
+      c1[2 * p + 1] = a1.y;
+    }
+    int next = 0;  // explicit values used so far
+#pragma unroll
+    for (int i = 0; i < WA; i++) {
+      const uint32_t code = (side >> (2 * i)) & 3u;
+      double xe = x[kAdvExplicit - 1];
+#pragma unroll
+      for (int q = kAdvExplicit - 2; q >= 0; q--) xe = next == q ? x[q] : xe;
+      const double a3 = code == 1u ? c1[i] : (code == 2u ? -c1[i] : xe);
+      next += code == 0u;
+      c3[i] = a3 * M.coef3rd;
+    }
   } else if (LOL == 0 && MPAS_PAIRED_STENCIL && M.advP) {
     // one access per slot pair
 #pragma unroll
