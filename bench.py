@@ -203,7 +203,7 @@ def adjoint_bench(level=6, steps=200, dt=240.0, n_tr=1, k=20, n_obs=1000, reps=3
         "workload": f"icosahedral SCVT level {level} ({mesh.nCells} cells), FP64 SW RK4 + FCT, "
                     f"{steps} steps dt={dt:g} s, {n_tr} tracer, checkpoint every {k} steps, "
                     f"loss over {n_obs} seeded cells (h and tracer) vs perturbed-run observations",
-        "tuned_config": adj_cfg, "autotune_trials": len(adj_trials),
+        "tuned_config": adj_cfg, "autotune_trials": adj_trials,
         "forward_ms": 1e3 * t_fwd, "forward_with_checkpoints_ms": 1e3 * t_adj_fwd,
         "backward_ms": 1e3 * t_bwd, "tlm_ms": 1e3 * t_tlm,
         # the adjoint sweep (segment replays from the checkpoints included)
