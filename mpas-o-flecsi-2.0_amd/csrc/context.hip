@@ -268,6 +268,54 @@ int build_device_mesh(mpas_ctx *ctx) {
                    (rc = upload(ctx, slot_pairs(x, kAdvExplicit, E), &M.advC3xP))))
         return rc;
     }
+    // LDS staging of the one-level cells kernel (FCT), on request
+    // (MPAS_HIP_CELLS_LDS=1): per workgroup the distinct edges of its cells,
+    // each cell's slots as positions in them.  Off by default: on the
+    // 655,362-cell mesh with 4 tracers the kernel takes 81 us with it and
+    // 71 us without, since L2 already serves the repeated edge reads and
+    // the copy adds a barrier-separated pass per workgroup.
+    const char *stg = std::getenv("MPAS_HIP_CELLS_LDS");
+    const int nT = ctx->params.nTracers;
+    if (m.nL == 1 && nT > 0 && M.narrow && ctx->params.tracer_scheme == MPAS_TRACER_FCT &&
+        stg && stg[0] == '1') {
+      const int npl = nT % 2 == 0 ? nT / 2 : nT;  // tracer lanes per cell
+      const int cpb = kCellsBlock / npl;
+      if (kCellsBlock % npl == 0) {
+        const int nb = (C + cpb - 1) / cpb;
+        std::vector<int32_t> off(nb + 1, 0), list;
+        std::vector<uint64_t> loc(C, 0);
+        int mx = 0;
+        bool ok = true;
+        for (int b = 0; b < nb && ok; b++) {
+          const int c0 = b * cpb, c1e = std::min(C, c0 + cpb);
+          std::vector<int32_t> es;
+          for (int c = c0; c < c1e; c++) {
+            ok = ok && m.c2e.size(c) <= 6;  // six 10-bit positions per cell
+            for (int j = 0; j < m.c2e.size(c); j++) es.push_back(m.c2e.at(c, j));
+          }
+          std::sort(es.begin(), es.end());
+          es.erase(std::unique(es.begin(), es.end()), es.end());
+          // positions take 10 bits; the LDS copy stays within 64 KiB
+          if (es.size() >= 1024 || es.size() * (5 + nT) * sizeof(double) > (64u << 10)) ok = false;
+          mx = std::max(mx, int(es.size()));
+          for (int c = c0; c < c1e; c++)
+            for (int j = 0; j < m.c2e.size(c); j++) {
+              const uint64_t pos = uint64_t(std::lower_bound(es.begin(), es.end(), m.c2e.at(c, j)) -
+                                            es.begin());
+              loc[c] |= pos << (10 * j);
+            }
+          list.insert(list.end(), es.begin(), es.end());
+          off[b + 1] = int32_t(list.size());
+        }
+        if (ok && ((rc = upload(ctx, loc, &M.c2eLoc)) || (rc = upload(ctx, off, &M.stgOff)) ||
+                   (rc = upload(ctx, list, &M.stgEdge))))
+          return rc;
+        if (ok) {
+          M.stgMax = mx;
+          M.stgCpb = cpb;
+        }
+      }
+    }
     bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
                             [&](int x) { return x == m.nL; });
     if (!full) {

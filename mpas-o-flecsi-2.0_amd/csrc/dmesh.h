@@ -63,12 +63,23 @@ struct DMesh {
   // [kAdvExplicit/2][E][2]; nullptr when some edge has more explicit slots
   const uint32_t *advSide;
   const double *advC3xP;
+  // LDS staging of the one-level cells kernel: the cells of logical block
+  // b (kCellsBlock lanes, stgCpb cells) touch the edges stgEdge[stgOff[b]
+  // .. stgOff[b + 1]) (ascending); the 10-bit field j of c2eLoc[c] is the
+  // position of edge slot j of cell c in its block's list.  nullptr: none.
+  const uint64_t *c2eLoc;
+  const int *stgOff, *stgEdge;
+  int stgMax, stgCpb;  // the largest block list; cells per block
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
   double msd;         // meshScalingDel2: one value for every edge
   double gravity, coef3rd;
 };
+
+// Workgroup size of the one-level cells kernel (its LDS staging groups the
+// cells by workgroup).
+constexpr int kCellsBlock = 256;
 
 // explicit advCoef_3rd values per edge in the compact form: the two cells of
 // the edge and their two common neighbours (the slots both cell fits hold)

@@ -127,6 +127,16 @@ __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_c
                                                                                StageIO S) {
   stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS), SCH, 1>(M, S, xcd_item());
 }
+// One-level FCT cells with the workgroup's edge values staged in LDS
+// (DMesh::c2eLoc); dynamic LDS of (5 + NT) * M.stgMax doubles.
+static_assert(BS_CELLS == kCellsBlock, "the LDS staging lists are built per kCellsBlock lanes");
+template <int NT, int W>
+__global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells_lds(DMesh M,
+                                                                                   StageIO S) {
+  extern __shared__ double lds[];
+  stage_cells<NT, W, 0, tpl_of<NT>(MPAS_TPL_CELLS), MPAS_TRACER_FCT, 1, true>(M, S, xcd_item(),
+                                                                                lds);
+}
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
@@ -299,9 +309,19 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
     else if (S.scheme == MPAS_TRACER_CENTRED)
       hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_CENTRED>), grid, dim3(BS_CELLS),
                          0, st, M, S);
-    else
-      hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_FCT>), grid, dim3(BS_CELLS), 0,
-                         st, M, S);
+    else {
+      bool staged = false;
+      if constexpr (LOL == 0 && NT > 0) {
+        if (M.c2eLoc) {
+          hipLaunchKernelGGL((k_stage_cells_lds<NT, W>), grid, dim3(BS_CELLS),
+                             size_t(M.stgMax) * (5 + NT) * sizeof(double), st, M, S);
+          staged = true;
+        }
+      }
+      if (!staged)
+        hipLaunchKernelGGL((k_stage_cells<NT, W, LOL, MPAS_TRACER_FCT>), grid, dim3(BS_CELLS), 0,
+                           st, M, S);
+    }
   }
   MPAS_STEP(after_launch("k_stage_cells"));
   MPAS_STEP(prof_end(ctx));

@@ -504,16 +504,56 @@ __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, doub
 // cell's first lane stores them), and per tracer either the centred
 // tendency with its update or the FCT bounds, upwind tendency and limiter
 // factors.  NPL lanes per cell, TPL tracers each (one lane without tracers).
-template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1, int FIX = 0>
-__device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item) {
+//
+// STG (one-level launches of kCellsBlock lanes, DMesh::c2eLoc set): the
+// workgroup first copies the edge values its cells read (new and
+// provisional u, h_edge, dvEdge, dcEdge, antidiffusive fluxes) of the
+// distinct edges of its cells into LDS, ascending edge id, then every cell
+// reads them there instead of gathering them from L2; `lds` holds
+// 5 + NT arrays of M.stgMax doubles.
+template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1, int FIX = 0, bool STG = false>
+__device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item,
+                                            double *lds = nullptr) {
   constexpr int TPL = NT > 0 ? (TPLX ? TPLX : tracers_per_lane<NT>()) : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
+  static_assert(!STG || (LOL == 0 && NT > 0 && SCH == MPAS_TRACER_FCT),
+                "LDS staging serves one-level FCT launches");
+  const uint32_t mx = STG ? uint32_t(M.stgMax) : 0u;
+  double *const s_yu = lds, *const s_xu = lds + mx, *const s_he = lds + 2 * mx;
+  double *const s_dv = lds + 3 * mx, *const s_dc = lds + 4 * mx, *const s_an = lds + 5 * mx;
+  if constexpr (STG) {
+    // every lane of the workgroup takes part, before any returns
+    const int lb = item / kCellsBlock;
+    const int nb = (M.nC + M.stgCpb - 1) / M.stgCpb;
+    if (lb < nb) {
+      const int b0 = ld(M.stgOff, uint32_t(lb)), nE = ld(M.stgOff, uint32_t(lb + 1)) - b0;
+      for (int i = int(threadIdx.x); i < nE; i += int(blockDim.x)) {
+        const uint32_t e = uint32_t(ld(M.stgEdge, uint32_t(b0 + i)));
+        if (!S.ocean) s_yu[i] = ld(S.yu, e);
+        s_xu[i] = ld(S.xu, e);
+        s_he[i] = ld(S.h_edge, e);
+        s_dv[i] = ldc(M.dv, e);
+        s_dc[i] = ldc(M.dc, e);
+        double a[NT];
+        ld_group<NT>(S.anti, e * NT, a);
+#pragma unroll
+        for (int t = 0; t < NT; t++) s_an[uint32_t(i) * NT + t] = a[t];
+      }
+    }
+    __syncthreads();
+  }
   int sub;
   uint32_t k0, k1;
   const int c = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
+  uint32_t pos[W];  // STG: each slot's position in the workgroup's edge list
+  if constexpr (STG) {
+    const uint64_t loc = ldc(M.c2eLoc, uint32_t(c));
+#pragma unroll
+    for (int j = 0; j < W; j++) pos[j] = uint32_t(loc >> (10 * j)) & 1023u;
+  }
   const int n = ldc(M.nEoC, c);
   const uint8_t neg = ldc(M.sgnNeg, c);
   const uint32_t rank = ldc(M.ascRank, c);
@@ -537,7 +577,14 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   // provides no value
   double dcS[W];
   uint32_t opS[W];
-  if constexpr (GRP) {
+  if constexpr (STG) {
+    if constexpr (GRP) base = int(threadIdx.x & 63u) - sub;  // for the cOpp rows
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      e[j] = 0;  // edge ids are not needed: the values come from LDS
+      dv[j] = s_dv[pos[j]];
+    }
+  } else if constexpr (GRP) {
     base = int(threadIdx.x & 63u) - sub;
     double dm[MS];
 #pragma unroll
@@ -562,7 +609,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   }
   int lvE[W];  // maxLevelEdgeTop of the cell's edges (partial columns only)
 #pragma unroll
-  for (int j = 0; j < W; j++) lvE[j] = M.mlet ? ldc(M.mlet, e[j]) : int(L);
+  for (int j = 0; j < W; j++) lvE[j] = (!STG && M.mlet) ? ldc(M.mlet, e[j]) : int(L);
 
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
@@ -574,9 +621,18 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
     if (lead && !S.ocean) {
       // ke (solve.cc:63-77) and divergence of the new state
       double yu[W], dc[W];
+      if constexpr (STG) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          yu[j] = s_yu[pos[j]];
+          dc[j] = s_dc[pos[j]];
+        }
+      } else {
 #pragma unroll
       for (int j = 0; j < W; j++) yu[j] = ld(S.yu, e[j] * L + k);
-      if (shared) {
+      }
+      if constexpr (STG) {
+      } else if (shared) {
 #pragma unroll
         for (int j = 0; j < W; j++) dc[j] = dcS[j];
       } else {
@@ -595,7 +651,13 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       dvg_v = dvg / ar;
     }
     double xu[W], he[W];
-    if constexpr (GRP) {
+    if constexpr (STG) {
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        xu[j] = s_xu[pos[j]];
+        he[j] = s_he[pos[j]];
+      }
+    } else if constexpr (GRP) {
       double xm[MS], hm[MS];
 #pragma unroll
       for (int m = 0; m < MS; m++) {
@@ -706,8 +768,15 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
         }
         const double hProvInv = 1.0 / hp;
         double an[W][TPL];
+        if constexpr (STG) {
+#pragma unroll
+          for (int j = 0; j < W; j++)
+#pragma unroll
+            for (int g = 0; g < TPL; g++) an[j][g] = s_an[pos[j] * NT + t0 + g];
+        } else {
 #pragma unroll
         for (int j = 0; j < W; j++) ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
+        }
         double rf[2 * TPL], tl[TPL];
 #pragma unroll
         for (int g = 0; g < TPL; g++) {

@@ -424,3 +424,26 @@ def test_reordered_context_bitwise(order):
     r.close()
     ctx.close()
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nT", [2, 4])
+def test_cells_lds_staging_bitwise(monkeypatch, nT):
+    """The one-level FCT cells kernel with its workgroup's edge values
+    staged in LDS (MPAS_HIP_CELLS_LDS=1, read when the context is made)
+    gives the bits of the default kernel, which the oracle tests pin; 2562
+    cells leave the last workgroup partly empty."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(4)
+    u, h, tr = sw_workload_state(m, nT, seed=21)
+    h_s = np.full(m.nCells, 4500.0)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MPAS_HIP_CELLS_LDS", flag)
+        ctx = Context(m, nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5)
+        ctx.sw_upload(h, u, tr, h_s)
+        ctx.rk4_step(480.0, 3)
+        out.append(ctx.sw_download())
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        assert bitwise_equal(a, b)
