@@ -100,6 +100,13 @@ __device__ __forceinline__ void grp_row(T (&out)[W], const T (&mine)[MS], int ba
   for (int j = 0; j < W; j++) out[j] = __shfl(mine[j / G], base + j % G, 64);
 }
 
+// One-level cells kernel: the cell's tracer lanes share the new-u and
+// dcEdge loads of the kinetic energy and divergence, which only the lead
+// lane forms.
+#ifndef MPAS_CELLS_SHARE_KE
+#define MPAS_CELLS_SHARE_KE 1
+#endif
+
 // The one-level momentum and high-order flux kernels read their stencils
 // from the slot-paired copies (one 8- or 16-byte access per slot pair).
 #ifndef MPAS_PAIRED_STENCIL
@@ -618,26 +625,40 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
     double ke_v = 0.0, dvg_v = 0.0, nh_v = 0.0;
     const bool w_ke = lead && !S.ocean;
     bool w_nh = false;
+    double yu[W], dc[W];  // new u and dcEdge of the edges, for ke and divergence
+    constexpr bool SHKE = GRP && !STG && MPAS_CELLS_SHARE_KE;
+    if constexpr (SHKE) {
+      // every lane of the cell loads its share of the rows the lead lane
+      // needs (the branch on S.ocean is uniform: no lane misses a permute)
+      if (!S.ocean) {
+        double ym[MS], cm[MS];
+#pragma unroll
+        for (int m = 0; m < MS; m++) {
+          ym[m] = ld(S.yu, me[m] * L + k);
+          cm[m] = ldc(M.dc, me[m]);
+        }
+        grp_row<NPL>(yu, ym, base);
+        grp_row<NPL>(dc, cm, base);
+      }
+    }
     if (lead && !S.ocean) {
       // ke (solve.cc:63-77) and divergence of the new state
-      double yu[W], dc[W];
       if constexpr (STG) {
 #pragma unroll
         for (int j = 0; j < W; j++) {
           yu[j] = s_yu[pos[j]];
           dc[j] = s_dc[pos[j]];
         }
-      } else {
+      } else if constexpr (!SHKE) {
 #pragma unroll
-      for (int j = 0; j < W; j++) yu[j] = ld(S.yu, e[j] * L + k);
-      }
-      if constexpr (STG) {
-      } else if (shared) {
+        for (int j = 0; j < W; j++) yu[j] = ld(S.yu, e[j] * L + k);
+        if (shared) {
 #pragma unroll
-        for (int j = 0; j < W; j++) dc[j] = dcS[j];
-      } else {
+          for (int j = 0; j < W; j++) dc[j] = dcS[j];
+        } else {
 #pragma unroll
-        for (int j = 0; j < W; j++) dc[j] = ldc(M.dc, e[j]);
+          for (int j = 0; j < W; j++) dc[j] = ldc(M.dc, e[j]);
+        }
       }
       double ke = 0.0, dvg = 0.0;
 #pragma unroll

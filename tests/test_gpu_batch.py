@@ -53,6 +53,46 @@ def _close(ctxs, b):
         c.close()
 
 
+def _oracle_member(level, nT, state, dt, nsteps):
+    """One member advanced by the oracle (the CPU restatement of the
+    reference's step) with the context settings of _ensemble."""
+    from mpas_hip import TRACER_FCT
+    from oracle.oracle import OracleMesh, OracleSW
+    h, u, tr, hs = state
+    o = OracleSW(OracleMesh(mesh(level), nL=1), nL=1, nT=nT,
+                 tracer_adv=TRACER_FCT if nT else 0, visc2=1e5, kappa2=1e3 if nT else 0.0)
+    o.set("h", h)
+    o.set("u", u)
+    if nT:
+        o.set("tracers", tr)
+    o.set("h_s", hs)
+    o.init_diagnostics()
+    o.run(nsteps, dt)
+    return o.get("h"), o.get("u"), o.get("tracers") if nT else np.zeros(0)
+
+
+@pytest.mark.parametrize("nT", [2, 4])
+def test_members_match_oracle(nT):
+    """Every batched member against the oracle's own run of that member,
+    bit for bit (not only through a single-context run): three groups of
+    different meshes and time steps, in two configurations (one member per
+    lane with one entity per wave where it fills the wave; two members per
+    lane, levels-on-lanes layouts only)."""
+    from mpas_hip import BATCH_GRAPH
+    ctxs, b, states, mesh_of = _ensemble(nT=nT)
+    nsteps = 3
+    ref = [_oracle_member(LEVELS[i], nT, states[i], DTS[i], nsteps) for i in range(len(LEVELS))]
+    for lpl, we in ((1, 1), (2, 0)):
+        b.config([256] * 6, lpl, 0, 1, we)
+        _load(b, states)
+        b.step(nsteps, BATCH_GRAPH)
+        b.sync()
+        for i in range(b.n):
+            for a, r, nm in zip(b.download(i), ref[i], ("h", "u", "tracers")):
+                assert bitwise_equal(a, r), (nT, lpl, we, i, nm)
+    _close(ctxs, b)
+
+
 def test_groups_follow_mesh_and_dt():
     ctxs, b, _, _ = _ensemble()
     # (L3, 600), (L4, 300), (L3, 450)
