@@ -138,6 +138,51 @@ int build_device_mesh(mpas_ctx *ctx) {
         (rc = upload(ctx, w, &M.woe)) || (rc = upload(ctx, slot_pairs(ids, padE, E), &M.eoeP)) ||
         (rc = upload(ctx, slot_pairs(w, padE, E), &M.woeP)))
       return rc;
+    // LDS staging of the one-level momentum kernel (MPAS_HIP_EDGES_LDS=0
+    // turns it off): per workgroup the distinct edgesOnEdge ids of its
+    // edges, each edge's slots as positions in that list.  The kernel's
+    // gathers of u, pv_edge and h_edge at ten slots touch about fifty cache
+    // lines per wave-wide load; the staging copy reads each list in
+    // ascending, mostly consecutive ids.
+    const char *stg = std::getenv("MPAS_HIP_EDGES_LDS");
+    if (m.nL == 1 && padE <= 12 && !(stg && stg[0] == '0')) {
+      const int nb = (E + kEdgesBlock - 1) / kEdgesBlock;
+      std::vector<int32_t> off(nb + 1, 0), list;
+      std::vector<uint64_t> loc(size_t(2) * E, 0);
+      int mx = 0;
+      bool ok = true;
+      std::vector<int32_t> es;
+      for (int b = 0; b < nb && ok; b++) {
+        const int e0 = b * kEdgesBlock, e1 = std::min(E, e0 + kEdgesBlock);
+        es.clear();
+        for (int e = e0; e < e1; e++)
+          for (int j = 0; j < m.e2e.size(e); j++) es.push_back(m.e2e.at(e, j));
+        std::sort(es.begin(), es.end());
+        es.erase(std::unique(es.begin(), es.end()), es.end());
+        // positions take 10 bits; the copy (3 doubles per entry) stays
+        // within 24 KiB
+        if (es.size() >= 1024) {
+          ok = false;
+          break;
+        }
+        mx = std::max(mx, int(es.size()));
+        for (int e = e0; e < e1; e++)
+          for (int j = 0; j < m.e2e.size(e); j++) {
+            const uint64_t pos =
+                uint64_t(std::lower_bound(es.begin(), es.end(), m.e2e.at(e, j)) - es.begin());
+            loc[size_t(2) * e + j / 6] |= pos << (10 * (j % 6));
+          }
+        list.insert(list.end(), es.begin(), es.end());
+        off[b + 1] = int32_t(list.size());
+      }
+      if (ok) {
+        if ((rc = upload(ctx, loc, &M.e2eLoc)) || (rc = upload(ctx, off, &M.eStgOff)) ||
+            (rc = upload(ctx, list, &M.eStgEdge)))
+          return rc;
+        M.eStgMax = mx;
+        M.eStgLen = int(list.size());
+      }
+    }
   }
   // cells
   {

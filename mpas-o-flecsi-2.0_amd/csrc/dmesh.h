@@ -70,6 +70,14 @@ struct DMesh {
   const uint64_t *c2eLoc;
   const int *stgOff, *stgEdge;
   int stgMax, stgCpb;  // the largest block list; cells per block
+  // LDS staging of the one-level momentum kernel: the edges of logical
+  // block b (kEdgesBlock lanes) gather their edgesOnEdge values from the
+  // distinct edges eStgEdge[eStgOff[b] .. eStgOff[b + 1]) (ascending);
+  // e2eLoc[2 e + j / 6] holds in bits 10 (j % 6) .. the position of slot j
+  // of edge e in its block's list.  nullptr: none.
+  const uint64_t *e2eLoc;
+  const int *eStgOff, *eStgEdge;
+  int eStgMax, eStgLen;  // the largest block list; all lists
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
@@ -80,6 +88,9 @@ struct DMesh {
 // Workgroup size of the one-level cells kernel (its LDS staging groups the
 // cells by workgroup).
 constexpr int kCellsBlock = 256;
+// Workgroup size of the one-level momentum kernel (its LDS staging groups
+// the edges by workgroup).
+constexpr int kEdgesBlock = 256;
 
 // explicit advCoef_3rd values per edge in the compact form: the two cells of
 // the edge and their two common neighbours (the slots both cell fits hold)
@@ -161,6 +172,7 @@ __device__ __forceinline__ T ldn(const T *__restrict__ p, uint32_t i) {
 // slot-paired row.
 typedef int mpas_i2v __attribute__((ext_vector_type(2)));
 typedef double mpas_d2v __attribute__((ext_vector_type(2)));
+typedef unsigned long long mpas_u2v __attribute__((ext_vector_type(2)));
 template <bool NT, class V, class S>
 __device__ __forceinline__ V ld_vec(const S *__restrict__ p, uint32_t i) {
   const MPAS_GLOBAL char *b = (const MPAS_GLOBAL char *)(p);

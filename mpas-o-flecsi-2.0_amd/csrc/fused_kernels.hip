@@ -127,6 +127,15 @@ __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_c
                                                                                StageIO S) {
   stage_cells<NT, W, LOL, tpl_of<NT>(MPAS_TPL_CELLS), SCH, 1>(M, S, xcd_item());
 }
+// One-level momentum with the edgesOnEdge values staged in LDS
+// (DMesh::e2eLoc); dynamic LDS of 3 * M.eStgMax doubles.
+static_assert(BS_EDGES == kEdgesBlock, "the LDS staging lists are built per kEdgesBlock lanes");
+template <int WE>
+__global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_edges_lds(DMesh M,
+                                                                                  StageIO S) {
+  extern __shared__ double lds[];
+  stage_edges<WE, 0, 1, 0, MPAS_QCH, true>(M, S, xcd_item(), lds);
+}
 // One-level FCT cells with the workgroup's edge values staged in LDS
 // (DMesh::c2eLoc); dynamic LDS of (5 + NT) * M.stgMax doubles.
 static_assert(BS_CELLS == kCellsBlock, "the LDS staging lists are built per kCellsBlock lanes");
@@ -217,6 +226,9 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
     b.edges = S.stage < 3 ? 2 * d * L * E : 0;
   } else {
     b.edges = 8 * E + d * E + 1 * E + 4 * sum_eoe + d * sum_eoe;   // e2c, dc, eoe, woe
+    if (M.e2eLoc)  // slot positions and the workgroups' lists instead of the ids
+      b.edges += 16 * E + 4.0 * M.eStgLen + 4 * ((E + kEdgesBlock - 1) / kEdgesBlock + 1) -
+                 4 * sum_eoe;
     b.edges += d * L * (3 * E + 2 * C) + d * C;                    // xu h_edge pv_edge, ke xh, h_s
     if (S.use_visc) b.edges += 8 * E + d * E + d * L * (C + V);    // e2v, dv, div, vort
     b.edges += touches * d * L * E;                                // u0 / nu / yu
@@ -278,8 +290,17 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
       hipLaunchKernelGGL((k_stage_edges<WE, LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_EDGES)),
                          dim3(BS_EDGES), 0, st, M, S2);
     } else {
-      hipLaunchKernelGGL((k_stage_edges<WE, LOL>), dim3(xcd_grid(M.nE * lv, BS_EDGES)),
-                         dim3(BS_EDGES), 0, st, M, S);
+      bool staged = false;
+      if constexpr (LOL == 0 && WE <= 12) {
+        if (M.e2eLoc) {
+          hipLaunchKernelGGL((k_stage_edges_lds<WE>), dim3(xcd_grid(M.nE, BS_EDGES)),
+                             dim3(BS_EDGES), size_t(3) * M.eStgMax * sizeof(double), st, M, S);
+          staged = true;
+        }
+      }
+      if (!staged)
+        hipLaunchKernelGGL((k_stage_edges<WE, LOL>), dim3(xcd_grid(M.nE * lv, BS_EDGES)),
+                           dim3(BS_EDGES), 0, st, M, S);
     }
     MPAS_STEP(after_launch("k_stage_edges"));
     MPAS_STEP(prof_end(ctx));

@@ -197,10 +197,35 @@ __device__ __forceinline__ int split_levels(const StageIO &S, uint32_t L, int it
 // compute_tend (+ viscosity), compute_substep, accumulate_update and, at the
 // last stage, finalize_timestep for u (solve.cc:215-233, 285-289, 326-330,
 // 359-366)
-template <int WE, int LOL = 0, int LP = 1, int FIX = 0, int QCX = MPAS_QCH>
-__device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item) {
+//
+// STG (one-level launches of kEdgesBlock lanes, DMesh::e2eLoc set): the
+// workgroup first copies u, pv_edge and h_edge of the distinct edgesOnEdge
+// edges of its edges into LDS (`lds`: 3 arrays of M.eStgMax doubles), then
+// every edge reads its slots there by position.
+template <int WE, int LOL = 0, int LP = 1, int FIX = 0, int QCX = MPAS_QCH, bool STG = false>
+__device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, int item,
+                                            double *lds = nullptr) {
+  static_assert(!STG || (LOL == 0 && LP == 1 && WE <= 12), "LDS staging serves one-level launches");
   int sub;
   uint32_t k0, k1;
+  const uint32_t mx = STG ? uint32_t(M.eStgMax) : 0u;
+  double *const s_u = lds, *const s_pv = lds + mx, *const s_he = lds + 2 * mx;
+  if constexpr (STG) {
+    if (!S.freeze_u) {  // uniform: the whole grid skips the copy or takes part
+      const int lb = item / kEdgesBlock;
+      const int nb = (M.nE + kEdgesBlock - 1) / kEdgesBlock;
+      if (lb < nb) {
+        const int b0 = ld(M.eStgOff, uint32_t(lb)), ne = ld(M.eStgOff, uint32_t(lb + 1)) - b0;
+        for (int i = int(threadIdx.x); i < ne; i += int(blockDim.x)) {
+          const uint32_t x = uint32_t(ld(M.eStgEdge, uint32_t(b0 + i)));
+          s_u[i] = ld(S.xu, x);
+          s_pv[i] = ld(S.pv_edge, x);
+          s_he[i] = ld(S.h_edge, x);
+        }
+      }
+      __syncthreads();
+    }
+  }
   const int e = split_levels<LOL, 1, FIX>(S, M.nL, item, sub, k0, k1);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE;
@@ -213,7 +238,19 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
   const int n = ldc(M.nEoE, e);
   uint32_t id[WE];
   double w[WE];
-  if (LOL == 1 && share_rows<WE>(S.nchunk)) {
+  if constexpr (STG) {
+    // slot positions in the workgroup's list, six per word
+    const mpas_u2v lw = ld_vec<kNtStencil, mpas_u2v>(M.e2eLoc, uint32_t(e));
+#pragma unroll
+    for (int j = 0; j < WE; j++)
+      id[j] = uint32_t((j < 6 ? lw.x : lw.y) >> (10 * (j % 6))) & 1023u;
+#pragma unroll
+    for (int p = 0; p < WE / 2; p++) {
+      const mpas_d2v wj = ld_vec<kNtStencil, mpas_d2v>(M.woeP, p * E + e);
+      w[2 * p] = wj.x;
+      w[2 * p + 1] = wj.y;
+    }
+  } else if (LOL == 1 && share_rows<WE>(S.nchunk)) {
     int base;
     const int j = row_slot(item, S.nchunk, WE, base);
     bcast_row<WE, uint32_t>(id, uint32_t(ld(M.eoe, j * E + e)), base);
@@ -282,10 +319,16 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
       double uu[QCH][LP], pv[QCH][LP], he[QCH][LP];
 #pragma unroll
       for (int jj = 0; jj < QCH && j0 + jj < WE; jj++) {
-        const uint32_t oj = id[j0 + jj] * L + k;
-        ld_lv<LP>(S.xu, oj, uu[jj]);
-        ld_lv<LP>(S.pv_edge, oj, pv[jj]);
-        ld_lv<LP>(S.h_edge, oj, he[jj]);
+        if constexpr (STG) {
+          uu[jj][0] = s_u[id[j0 + jj]];
+          pv[jj][0] = s_pv[id[j0 + jj]];
+          he[jj][0] = s_he[id[j0 + jj]];
+        } else {
+          const uint32_t oj = id[j0 + jj] * L + k;
+          ld_lv<LP>(S.xu, oj, uu[jj]);
+          ld_lv<LP>(S.pv_edge, oj, pv[jj]);
+          ld_lv<LP>(S.h_edge, oj, he[jj]);
+        }
       }
 #pragma unroll
       for (int p = 0; p < LP; p++) {
