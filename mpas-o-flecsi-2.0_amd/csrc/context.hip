@@ -313,6 +313,56 @@ int build_device_mesh(mpas_ctx *ctx) {
                    (rc = upload(ctx, slot_pairs(x, kAdvExplicit, E), &M.advC3xP))))
         return rc;
     }
+    // LDS staging of the one-level high-order flux kernel (MPAS_HIP_HO_LDS=0
+    // turns it off): per workgroup the distinct advection cells of its
+    // edges, each edge's slots as positions in that list (as for the
+    // momentum kernel; the tracers of a slot are one 16- or 32-byte record)
+    {
+      const char *hs = std::getenv("MPAS_HIP_HO_LDS");
+      const int nT = ctx->params.nTracers;
+      const bool full_cols = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
+                                         [&](int x) { return x == m.nL; });
+      const int npl = nT > 0 ? ho_lanes_per_edge(nT) : 1;
+      if (m.nL == 1 && nT > 0 && M.narrow && padE <= 12 && M.advSide && full_cols &&
+          kHoBlock % npl == 0 && !(hs && hs[0] == '0')) {
+        const int epb = kHoBlock / npl;
+        const int nb = (E + epb - 1) / epb;
+        std::vector<int32_t> off(nb + 1, 0), list, cs;
+        std::vector<uint64_t> loc(size_t(2) * E, 0);
+        int mx = 0;
+        bool ok = true;
+        for (int b = 0; b < nb && ok; b++) {
+          const int e0 = b * epb, e1 = std::min(E, e0 + epb);
+          cs.clear();
+          for (int e = e0; e < e1; e++)
+            for (int i = 0; i < padE; i++) cs.push_back(adv[size_t(i) * E + e]);
+          std::sort(cs.begin(), cs.end());
+          cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+          // 10-bit positions; the copy stays within 64 KiB
+          if (cs.size() >= 1024 || cs.size() * nT * sizeof(double) > (64u << 10)) {
+            ok = false;
+            break;
+          }
+          mx = std::max(mx, int(cs.size()));
+          for (int e = e0; e < e1; e++)
+            for (int i = 0; i < padE; i++) {
+              const uint64_t pos = uint64_t(
+                  std::lower_bound(cs.begin(), cs.end(), adv[size_t(i) * E + e]) - cs.begin());
+              loc[size_t(2) * e + i / 6] |= pos << (10 * (i % 6));
+            }
+          list.insert(list.end(), cs.begin(), cs.end());
+          off[b + 1] = int32_t(list.size());
+        }
+        if (ok) {
+          if ((rc = upload(ctx, loc, &M.a2cLoc)) || (rc = upload(ctx, off, &M.aStgOff)) ||
+              (rc = upload(ctx, list, &M.aStgCell)))
+            return rc;
+          M.aStgMax = mx;
+          M.aStgLen = int(list.size());
+          M.aStgEpb = epb;
+        }
+      }
+    }
     // LDS staging of the one-level cells kernel (FCT), on request
     // (MPAS_HIP_CELLS_LDS=1): per workgroup the distinct edges of its cells,
     // each cell's slots as positions in them.  Off by default: on the

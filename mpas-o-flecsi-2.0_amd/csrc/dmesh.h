@@ -78,6 +78,13 @@ struct DMesh {
   const uint64_t *e2eLoc;
   const int *eStgOff, *eStgEdge;
   int eStgMax, eStgLen;  // the largest block list; all lists
+  // LDS staging of the one-level high-order flux kernel, the same for the
+  // advection cells: the aStgEpb edges of logical block b (kHoBlock lanes)
+  // read the tracers of cells aStgCell[aStgOff[b] .. aStgOff[b + 1]);
+  // a2cLoc[2 e + i / 6] holds the 10-bit position of advection slot i.
+  const uint64_t *a2cLoc;
+  const int *aStgOff, *aStgCell;
+  int aStgMax, aStgLen, aStgEpb;
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
@@ -91,6 +98,16 @@ constexpr int kCellsBlock = 256;
 // Workgroup size of the one-level momentum kernel (its LDS staging groups
 // the edges by workgroup).
 constexpr int kEdgesBlock = 256;
+// The same for the high-order flux kernel, and its tracers per lane (four
+// where the count allows: each lane streams its edge's stencil once for
+// all of them; else pairs when the count is even).
+constexpr int kHoBlock = 256;
+#ifndef MPAS_TPL_HO
+#define MPAS_TPL_HO 4
+#endif
+inline int ho_lanes_per_edge(int nT) {
+  return nT % MPAS_TPL_HO == 0 ? nT / MPAS_TPL_HO : nT % 2 == 0 ? nT / 2 : nT;
+}
 
 // explicit advCoef_3rd values per edge in the compact form: the two cells of
 // the edge and their two common neighbours (the slots both cell fits hold)

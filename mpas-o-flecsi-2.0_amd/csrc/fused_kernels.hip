@@ -87,12 +87,8 @@ __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_e
                                                                                StageIO S) {
   stage_edges<WE, LOL, LP, LP>(M, S, xcd_item());
 }
-// Tracers per lane of the high-order flux and cell kernels (0: pairs when
-// the count is even).  The high-order flux takes four where the count
-// allows: each lane streams its edge's stencil once for all of them.
-#ifndef MPAS_TPL_HO
-#define MPAS_TPL_HO 4
-#endif
+// Tracers per lane of the cell kernels (0: pairs when the count is even;
+// the high-order flux kernel's count, MPAS_TPL_HO, is in dmesh.h).
 #ifndef MPAS_TPL_CELLS
 #define MPAS_TPL_CELLS 0
 #endif
@@ -135,6 +131,15 @@ __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_e
                                                                                   StageIO S) {
   extern __shared__ double lds[];
   stage_edges<WE, 0, 1, 0, MPAS_QCH, true>(M, S, xcd_item(), lds);
+}
+// One-level high-order flux with the advection cells' tracers staged in
+// LDS (DMesh::a2cLoc); dynamic LDS of NT * M.aStgMax doubles.
+static_assert(BS_HO == kHoBlock, "the LDS staging lists are built per kHoBlock lanes");
+template <int NT, int WA>
+__global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho_lds(DMesh M,
+                                                                         StageIO S) {
+  extern __shared__ double lds[];
+  stage_ho<NT, WA, 0, tpl_of<NT>(MPAS_TPL_HO), 1, 1, true>(M, S, xcd_item(), lds);
 }
 // One-level FCT cells with the workgroup's edge values staged in LDS
 // (DMesh::c2eLoc); dynamic LDS of (5 + NT) * M.stgMax doubles.
@@ -239,6 +244,8 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   const bool compact = MPAS_COMPACT_C3 && M.advSide && L == 1;
   const double stencil = compact ? 12 * sum_adv + 4 * E + d * kAdvExplicit * E : 20 * sum_adv;
   if (fct) b.ho = 8 * E + d * E + E + stencil + 2 * d * L * E + d * L * T * (C + E);
+  if (fct && M.a2cLoc)  // slot positions and the workgroups' lists instead of the ids
+    b.ho += 16 * E + 4.0 * M.aStgLen + 4 * ((E + M.aStgEpb - 1) / M.aStgEpb + 1) - 4 * sum_adv;
   // k_stage_cells
   b.cells = 14 * C + 4 * sum_eoc + 2 * d * E;                      // nEoC sgn rank area, c2e, dv dc
   b.cells += d * L * 3 * E;                                        // yu xu h_edge
@@ -313,10 +320,21 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
                            dim3(xcd_grid(M.nE * (lv / 2) * lanes_per_cell<NT, MPAS_TPL_HO>(),
                                          BS_HO)),
                            dim3(BS_HO), 0, st, M, S2);
-      else
-        hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>),
-                           dim3(xcd_grid(M.nE * lv * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
-                           dim3(BS_HO), 0, st, M, S);
+      else {
+        bool staged = false;
+        if constexpr (LOL == 0 && WE <= 12) {
+          if (M.a2cLoc) {
+            hipLaunchKernelGGL((k_stage_ho_lds<NT, WE>),
+                               dim3(xcd_grid(M.nE * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
+                               dim3(BS_HO), size_t(M.aStgMax) * NT * sizeof(double), st, M, S);
+            staged = true;
+          }
+        }
+        if (!staged)
+          hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL>),
+                             dim3(xcd_grid(M.nE * lv * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO)),
+                             dim3(BS_HO), 0, st, M, S);
+      }
       MPAS_STEP(after_launch("k_stage_ho"));
       MPAS_STEP(prof_end(ctx));
     }

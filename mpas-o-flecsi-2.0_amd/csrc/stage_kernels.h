@@ -372,9 +372,31 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // calc_normal_thickness_flux, the high-order flux and its correction by the
 // low-order flux (ocean/solve.cc:40-57, 120-167, 216-229); TPL tracers per
 // lane.
-template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1, int FIX = 0>
-__device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item) {
+//
+// STG (one-level launches of kHoBlock lanes, DMesh::a2cLoc set): the
+// workgroup first copies the tracers of the distinct advection cells of
+// its edges into LDS (`lds`: M.aStgMax records of NT doubles), then every
+// edge reads its slots there by position.
+template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1, int FIX = 0, bool STG = false>
+__device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item,
+                                         double *lds = nullptr) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
+  static_assert(!STG || (LOL == 0 && LP == 1 && WA <= 12), "LDS staging serves one-level launches");
+  if constexpr (STG) {
+    const int lb = item / kHoBlock;
+    const int nb = (M.nE + M.aStgEpb - 1) / M.aStgEpb;
+    if (lb < nb) {
+      const int b0 = ld(M.aStgOff, uint32_t(lb)), nc = ld(M.aStgOff, uint32_t(lb + 1)) - b0;
+      for (int i = int(threadIdx.x); i < nc; i += int(blockDim.x)) {
+        const uint32_t c = uint32_t(ld(M.aStgCell, uint32_t(b0 + i)));
+        double a[NT];
+        ld_group<NT>(S.xtr, c * NT, a);
+#pragma unroll
+        for (int t = 0; t < NT; t++) lds[uint32_t(i) * NT + t] = a[t];
+      }
+    }
+    __syncthreads();
+  }
   int sub;
   uint32_t k0, k1;
   const int e = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
@@ -396,10 +418,17 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     bcast_row<WA, double>(c3, ld(M.advCoef3, i * E + e), base);
 #pragma unroll
     for (int j = 0; j < WA; j++) c3[j] = c3[j] * M.coef3rd;
-  } else if (LOL == 0 && MPAS_PAIRED_STENCIL && MPAS_COMPACT_C3 && M.advP && M.advSide) {
+  } else if (STG || (LOL == 0 && MPAS_PAIRED_STENCIL && MPAS_COMPACT_C3 && M.advP && M.advSide)) {
     // one access per slot pair; advCoef_3rd from its side code: +-advCoef,
     // or the edge's next explicit value (picked by selects: a run-time
-    // index into a register array would go to scratch memory)
+    // index into a register array would go to scratch memory).  STG: the
+    // slots' positions in the workgroup's cell list instead of cell ids.
+    if constexpr (STG) {
+      const mpas_u2v lw = ld_vec<kNtStencil, mpas_u2v>(M.a2cLoc, uint32_t(e));
+#pragma unroll
+      for (int i = 0; i < WA; i++)
+        ic[i] = uint32_t((i < 6 ? lw.x : lw.y) >> (10 * (i % 6))) & 1023u;
+    }
     const uint32_t side = ldn<kNtStencil>(M.advSide, e);
     double x[kAdvExplicit];
 #pragma unroll
@@ -410,10 +439,12 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     }
 #pragma unroll
     for (int p = 0; p < WA / 2; p++) {
-      const mpas_i2v ij = ld_vec<kNtStencil, mpas_i2v>(M.advP, p * E + e);
+      if constexpr (!STG) {
+        const mpas_i2v ij = ld_vec<kNtStencil, mpas_i2v>(M.advP, p * E + e);
+        ic[2 * p] = uint32_t(ij.x);
+        ic[2 * p + 1] = uint32_t(ij.y);
+      }
       const mpas_d2v a1 = ld_vec<kNtStencil, mpas_d2v>(M.advCoefP, p * E + e);
-      ic[2 * p] = uint32_t(ij.x);
-      ic[2 * p + 1] = uint32_t(ij.y);
       c1[2 * p] = a1.x;
       c1[2 * p + 1] = a1.y;
     }
@@ -451,7 +482,7 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     }
   }
 #pragma unroll
-  for (int i = 0; i < WA; i++) lvc[i] = lvl_cell(M, ic[i]);
+  for (int i = 0; i < WA; i++) lvc[i] = STG ? int(L) : lvl_cell(M, ic[i]);  // STG: full columns
   for (uint32_t k = k0; k < k1; k += LP) {
     const uint32_t o = e * L + k;
     // every load of the level group first
@@ -463,8 +494,14 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
     for (int p = 0; p < LP; p++) {
       mask[p] = M.hoMask ? ldc(M.hoMask, o + p) : 1;
 #pragma unroll
-      for (int i = 0; i < WA; i++)
-        ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
+      for (int i = 0; i < WA; i++) {
+        if constexpr (STG) {
+#pragma unroll
+          for (int g = 0; g < TPL; g++) tv[i][p][g] = lds[ic[i] * NT + t0 + g];
+        } else {
+          ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
+        }
+      }
       if (M.advC12) {  // the stencil's first two cells are the edge's cells
 #pragma unroll
         for (int g = 0; g < TPL; g++) {

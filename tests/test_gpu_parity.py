@@ -449,19 +449,22 @@ def test_cells_lds_staging_bitwise(monkeypatch, nT):
         assert bitwise_equal(a, b)
 
 
-@pytest.mark.parametrize("nT", [0, 4])
-def test_edges_lds_staging_bitwise(monkeypatch, nT):
-    """The one-level momentum kernel with the edgesOnEdge values of its
-    workgroup staged in LDS (the default; MPAS_HIP_EDGES_LDS=0, read when
-    the context is made, turns it off) gives the bits of the global-gather
-    kernel; 7680 edges make 30 workgroups, with and without tracers."""
+@pytest.mark.parametrize("var,nT", [("MPAS_HIP_EDGES_LDS", 0), ("MPAS_HIP_EDGES_LDS", 4),
+                                    ("MPAS_HIP_HO_LDS", 2), ("MPAS_HIP_HO_LDS", 4),
+                                    ("MPAS_HIP_HO_LDS", 8)])
+def test_lds_staging_bitwise(monkeypatch, var, nT):
+    """The one-level momentum and high-order flux kernels with the values
+    their stencils gather staged in LDS per workgroup (the default; the
+    variable set to 0, read when the context is made, turns it off) give
+    the bits of the global-gather kernels: 7680 edges make 30 workgroups
+    (60 for the flux kernel at 8 tracers, two lanes per edge)."""
     from mpas_hip import TRACER_FCT, Context, sw_workload_state
     m = mesh(4)
     u, h, tr = sw_workload_state(m, nT, seed=22)
     h_s = np.full(m.nCells, 4500.0)
     out = []
     for flag in ("0", "1"):
-        monkeypatch.setenv("MPAS_HIP_EDGES_LDS", flag)
+        monkeypatch.setenv(var, flag)
         ctx = Context(m, nTracers=nT, tracer_scheme=TRACER_FCT if nT else 0, visc2=1e5)
         ctx.sw_upload(h, u, tr if nT else None, h_s)
         ctx.rk4_step(480.0, 3)
