@@ -493,20 +493,17 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
 #pragma unroll
     for (int p = 0; p < LP; p++) {
       mask[p] = M.hoMask ? ldc(M.hoMask, o + p) : 1;
+      // STG: the slots' tracers are read from LDS where the flux sum takes
+      // them (holding all of them took 152 VGPRs, 3 waves per SIMD)
+      if constexpr (!STG) {
 #pragma unroll
-      for (int i = 0; i < WA; i++) {
-        if constexpr (STG) {
-#pragma unroll
-          for (int g = 0; g < TPL; g++) tv[i][p][g] = lds[ic[i] * NT + t0 + g];
-        } else {
-          ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
-        }
+        for (int i = 0; i < WA; i++) ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
       }
       if (M.advC12) {  // the stencil's first two cells are the edge's cells
 #pragma unroll
         for (int g = 0; g < TPL; g++) {
-          t1[p][g] = tv[0][p][g];
-          t2[p][g] = tv[1][p][g];
+          t1[p][g] = STG ? lds[ic[0] * NT + t0 + g] : tv[0][p][g];
+          t2[p][g] = STG ? lds[ic[1] * NT + t0 + g] : tv[1][p][g];
         }
       } else {
         ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
@@ -526,9 +523,12 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
       for (int i = 0; i < WA; i++) {
         const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
         const bool live = i < na && kp < lvc[i];
+        double tvi[TPL];
+#pragma unroll
+        for (int g = 0; g < TPL; g++) tvi[g] = STG ? lds[ic[i] * NT + t0 + g] : tv[i][p][g];
 #pragma unroll
         for (int g = 0; g < TPL; g++) {
-          const double x = ho[p][g] + tv[i][p][g] * weightFactor;
+          const double x = ho[p][g] + tvi[g] * weightFactor;
           ho[p][g] = live ? x : ho[p][g];
         }
       }
