@@ -319,11 +319,7 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
       double uu[QCH][LP], pv[QCH][LP], he[QCH][LP];
 #pragma unroll
       for (int jj = 0; jj < QCH && j0 + jj < WE; jj++) {
-        if constexpr (STG) {
-          uu[jj][0] = s_u[id[j0 + jj]];
-          pv[jj][0] = s_pv[id[j0 + jj]];
-          he[jj][0] = s_he[id[j0 + jj]];
-        } else {
+        if constexpr (!STG) {  // STG: read from LDS where the sum takes them
           const uint32_t oj = id[j0 + jj] * L + k;
           ld_lv<LP>(S.xu, oj, uu[jj]);
           ld_lv<LP>(S.pv_edge, oj, pv[jj]);
@@ -335,8 +331,11 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 #pragma unroll
         for (int jj = 0; jj < QCH && j0 + jj < WE; jj++) {
           const int j = j0 + jj;
-          const double workpv = 0.5 * (pve[p] + pv[jj][p]);
-          const double qn = qs[p] + w[j] * uu[jj][p] * workpv * he[jj][p];
+          const double uj = STG ? s_u[id[j]] : uu[jj][p];
+          const double pj = STG ? s_pv[id[j]] : pv[jj][p];
+          const double hj = STG ? s_he[id[j]] : he[jj][p];
+          const double workpv = 0.5 * (pve[p] + pj);
+          const double qn = qs[p] + w[j] * uj * workpv * hj;
           qs[p] = (j < n) ? qn : qs[p];
         }
       }
