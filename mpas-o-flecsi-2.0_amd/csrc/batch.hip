@@ -166,6 +166,11 @@ __device__ __forceinline__ StageIO stage_of(const BatchGroup &g, int s) {
 #ifndef MPAS_BATCH_QCH
 #define MPAS_BATCH_QCH 4
 #endif
+// Advection slots per scheduling phase of the high-order flux kernels
+// (0: all at once).
+#ifndef MPAS_BATCH_QHO
+#define MPAS_BATCH_QHO 5
+#endif
 // The phase bodies of layout MODE for one work item of group g, stage s.
 template <int WE, int MODE>
 __device__ __forceinline__ void run_edges(const BatchGroup &g, int s, int item) {
@@ -177,7 +182,7 @@ template <int NT, int WA, int MODE>
 __device__ __forceinline__ void run_ho(const BatchGroup &g, int s, int item) {
   const DMesh M = g.M;
   const StageIO S = stage_of<MODE>(g, s);
-  stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE), fix_of(MODE)>(M, S, item);
+  stage_ho<NT, WA, lol_of(MODE), 0, lp_of(MODE), fix_of(MODE), false, MPAS_BATCH_QHO>(M, S, item);
 }
 template <int NT, int W, int MODE, int SCH>
 __device__ __forceinline__ void run_cells(const BatchGroup &g, int s, int item) {

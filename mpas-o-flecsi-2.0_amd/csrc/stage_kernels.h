@@ -376,7 +376,8 @@ __device__ __forceinline__ void stage_edges(const DMesh &M, const StageIO &S, in
 // workgroup first copies the tracers of the distinct advection cells of
 // its edges into LDS (`lds`: M.aStgMax records of NT doubles), then every
 // edge reads its slots there by position.
-template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1, int FIX = 0, bool STG = false>
+template <int NT, int WA, int LOL = 0, int TPLX = 0, int LP = 1, int FIX = 0, bool STG = false,
+          int QHX = 0>
 __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int item,
                                          double *lds = nullptr) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
@@ -482,55 +483,77 @@ __device__ __forceinline__ void stage_ho(const DMesh &M, const StageIO &S, int i
   }
 #pragma unroll
   for (int i = 0; i < WA; i++) lvc[i] = STG ? int(L) : lvl_cell(M, ic[i]);  // STG: full columns
+  // QA advection slots gathered per scheduling phase (QHX, with gathers
+  // from global memory): the flux sum takes the slots in order, chunk by
+  // chunk, so fewer gathered values are live at once
+  constexpr int QA = (!STG && QHX > 0 && QHX < WA) ? (QHX < 2 ? 2 : QHX) : WA;
   for (uint32_t k = k0; k < k1; k += LP) {
     const uint32_t o = e * L + k;
-    // every load of the level group first
-    double xu[LP], he[LP], tv[WA][LP][TPL], t1[LP][TPL], t2[LP][TPL];
+    double xu[LP], he[LP], t1[LP][TPL], t2[LP][TPL];
     int mask[LP];
     ld_lv<LP>(S.xu, o, xu);
     ld_lv<LP>(S.h_edge, o, he);
 #pragma unroll
-    for (int p = 0; p < LP; p++) {
-      mask[p] = M.hoMask ? ldc(M.hoMask, o + p) : 1;
+    for (int p = 0; p < LP; p++) mask[p] = M.hoMask ? ldc(M.hoMask, o + p) : 1;
+    double ho[LP][TPL];
+#pragma unroll
+    for (int p = 0; p < LP; p++)
+#pragma unroll
+      for (int g = 0; g < TPL; g++) ho[p][g] = 0.0;
+#pragma unroll
+    for (int i0 = 0; i0 < WA; i0 += QA) {
       // STG: the slots' tracers are read from LDS where the flux sum takes
       // them (holding all of them took 152 VGPRs, 3 waves per SIMD)
+      double tv[QA][LP][TPL];
       if constexpr (!STG) {
 #pragma unroll
-        for (int i = 0; i < WA; i++) ld_group<TPL>(S.xtr, (ic[i] * L + k + p) * NT + t0, tv[i][p]);
-      }
-      if (M.advC12) {  // the stencil's first two cells are the edge's cells
+        for (int ii = 0; ii < QA && i0 + ii < WA; ii++)
 #pragma unroll
-        for (int g = 0; g < TPL; g++) {
-          t1[p][g] = STG ? lds[ic[0] * NT + t0 + g] : tv[0][p][g];
-          t2[p][g] = STG ? lds[ic[1] * NT + t0 + g] : tv[1][p][g];
-        }
-      } else {
-        ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
-        ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k + p) * NT + t0, t2[p]);
+          for (int p = 0; p < LP; p++)
+            ld_group<TPL>(S.xtr, (ic[i0 + ii] * L + k + p) * NT + t0, tv[ii][p]);
       }
+      if (i0 == 0) {
+#pragma unroll
+        for (int p = 0; p < LP; p++) {
+          if (M.advC12) {  // the stencil's first two cells are the edge's cells
+#pragma unroll
+            for (int g = 0; g < TPL; g++) {
+              t1[p][g] = STG ? lds[ic[0] * NT + t0 + g] : tv[0][p][g];
+              t2[p][g] = STG ? lds[ic[1] * NT + t0 + g] : tv[1][p][g];
+            }
+          } else {
+            ld_group<TPL>(S.xtr, (uint32_t(cc.x) * L + k + p) * NT + t0, t1[p]);
+            ld_group<TPL>(S.xtr, (uint32_t(cc.y) * L + k + p) * NT + t0, t2[p]);
+          }
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < LP; p++) {
+        const int kp = int(k) + p;
+        const double F = xu[p] * he[p];
+        const double wgt = F * double(mask[p]);
+        const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
+#pragma unroll
+        for (int ii = 0; ii < QA && i0 + ii < WA; ii++) {
+          const int i = i0 + ii;
+          const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
+          const bool live = i < na && kp < lvc[i];
+          double tvi[TPL];
+#pragma unroll
+          for (int g = 0; g < TPL; g++) tvi[g] = STG ? lds[ic[i] * NT + t0 + g] : tv[ii][p][g];
+#pragma unroll
+          for (int g = 0; g < TPL; g++) {
+            const double x = ho[p][g] + tvi[g] * weightFactor;
+            ho[p][g] = live ? x : ho[p][g];
+          }
+        }
+      }
+      if (i0 + QA < WA) MPAS_PHASE();
     }
-    double ho[LP][TPL];
 #pragma unroll
     for (int p = 0; p < LP; p++) {
       const int kp = int(k) + p;
       const double F = xu[p] * he[p];
-      const double wgt = F * double(mask[p]);
-      const double sgn = (F < 0.0) ? -1.0 : double(F > 0.);
-#pragma unroll
-      for (int g = 0; g < TPL; g++) ho[p][g] = 0.0;
-#pragma unroll
-      for (int i = 0; i < WA; i++) {
-        const double weightFactor = wgt * (c1[i] + c3[i] * sgn);
-        const bool live = i < na && kp < lvc[i];
-        double tvi[TPL];
-#pragma unroll
-        for (int g = 0; g < TPL; g++) tvi[g] = STG ? lds[ic[i] * NT + t0 + g] : tv[i][p][g];
-#pragma unroll
-        for (int g = 0; g < TPL; g++) {
-          const double x = ho[p][g] + tvi[g] * weightFactor;
-          ho[p][g] = live ? x : ho[p][g];
-        }
-      }
       if (kp < lve) {
         const double tracerWeight = double((mask[p] + 1) & 1) * (dv * 0.5) * F;
         const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
