@@ -353,7 +353,7 @@ def ensemble_bench(steps=500, seq_steps=20, n_tr=2, tune_steps=10,
 
 
 def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
-                           groups=((7, 64, 120.0), (5, 64, 480.0)), sample=2):
+                           groups=((7, 64, 120.0), (5, 64, 480.0)), sample=2, baseline=True):
     """Batched adjoint ensemble: 64 members on the level-7 mesh and 64 on the
     level-5 mesh, 2 FCT tracers, 100 steps, checkpoints every 10 steps, each
     member with its own seeded bump and its own seeded set of 2,000
@@ -362,10 +362,12 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
     per lane; a mesh's members are split over as few equal contexts as keep
     every tape array below the 4 GiB the kernels address (the largest holds
     edges x levels x tracers x 16 stencil slots); forward with checkpoints and
-    adjoint sweep of every context run back to back.  Baseline: each member
-    alone (its own single-level adjoint), timed on `sample` members per mesh
-    and scaled to the member count.  The sampled members' gradients equal
-    their single runs bit for bit."""
+    adjoint sweep of every context run back to back.  Baseline: every member
+    alone (its own single-level adjoint, the context tuned likewise), all of
+    them timed; the first `sample` members per mesh also compare their
+    gradients with the batch's, bit for bit.  baseline=False skips the
+    baseline and the autotuner (for a kernel trace of the batched part:
+    workgroups of 512, replays beside the sweep, no graphs)."""
     from mpas_hip import TRACER_FCT, Context, Mesh, sw_workload_state
     kw = dict(nTracers=n_tr, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
     batches, singles, total = [], [], 0
@@ -398,7 +400,10 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
             ctx = Context(mesh, nLevels=nM, **kw)
             ctx.sw_upload(X0[1], X0[0], X0[2], np.full(C, 4500.0))
             ctx.adj_init(k)
-            ctx.adj_autotune(dt, 2 * k)  # tile and schedule for this mesh and member count
+            if baseline:
+                ctx.adj_autotune(dt, 2 * k)  # tile and schedule for this mesh and member count
+            else:  # kernel-trace runs: the profiler's kernel tracing fails on graph capture
+                ctx.adj_config(512, 512, True, False)
             x0, xp, x1 = ctx.state_arrays(), ctx.state_arrays(), ctx.state_arrays()
             for d, v in zip(x0, X0):
                 d.upload(v)
@@ -453,8 +458,9 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
     # baseline: every member alone (its own single-level context, tuned
     # likewise), and the first `sample` members' gradients against the batch
     t_seq, same, tuned_single = 0.0, True, {}
-    for (mesh, dt, all_members, n), b in zip(singles, [next(x for x in batches if x["L"] == g[0])
-                                                      for g in groups]):
+    for (mesh, dt, all_members, n), b in zip(singles if baseline else [],
+                                             [next(x for x in batches if x["L"] == g[0])
+                                              for g in groups]):
         C, E = mesh.nCells, mesh.nEdges
         c1 = Context(mesh, **kw)
         (u, h, tr), _, _ = all_members[0]
@@ -529,10 +535,10 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
         "tuned_config_per_context": tuned_ctx,
         "tuned_config_single_runs": tuned_single,
         "forward_with_checkpoints_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
-        "batched_ms": 1e3 * t_batch, "sequential_ms": 1e3 * t_seq,
-        "sequential_members_timed": total,
-        "speedup_vs_sequential": t_seq / t_batch, "loss": loss,
-        "sampled_gradients_bitwise_equal_to_single_runs": bool(same),
+        "batched_ms": 1e3 * t_batch, "sequential_ms": 1e3 * t_seq if baseline else None,
+        "sequential_members_timed": total if baseline else 0,
+        "speedup_vs_sequential": t_seq / t_batch if baseline else None, "loss": loss,
+        "sampled_gradients_bitwise_equal_to_single_runs": bool(same) if baseline else None,
     }
 
 
@@ -588,6 +594,9 @@ def main():
     ap.add_argument("--no-adjoint-ensemble", action="store_true",
                     help="skip the batched adjoint ensemble (64 x level 7 + 64 x level 5)")
     ap.add_argument("--adjoint-ensemble-only", action="store_true")
+    ap.add_argument("--adj-ens-no-baseline", action="store_true",
+                    help="with --adjoint-ensemble-only: skip the member-alone baseline "
+                         "(for a kernel trace of the batched part)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel k_copy over BYTES, 10 times "
                          "(the known-size reference for HBM counters)")
@@ -607,7 +616,8 @@ def main():
         print(json.dumps(adjoint_bench(steps=args.adj_steps)), flush=True)
         return
     if args.adjoint_ensemble_only:
-        print(json.dumps(adjoint_ensemble_bench()), flush=True)
+        print(json.dumps(adjoint_ensemble_bench(baseline=not args.adj_ens_no_baseline)),
+              flush=True)
         return
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(steps=args.ens_steps, seq_steps=args.ens_seq_steps,

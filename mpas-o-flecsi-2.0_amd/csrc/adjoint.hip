@@ -456,23 +456,43 @@ static_assert(kChunk >= 16, "the edge part reads a cell's advection slots in one
 // pv_edge adjoints, and per (edge, tracer) what the cells gather: r1 / r2
 // (into cell1 / cell2: upwind flux and mixing) and, per (stencil slot,
 // edge, tracer), hbf = the 3rd-order flux adjoint times F times the slot's
-// coefficient: the adjoint the slot's cell receives.
+// coefficient: the adjoint the slot's cell receives.  These three keep
+// levels innermost ([edge][tracer][level], hbf [slot][edge][tracer][level]):
+// the lanes of an edge's levels then store whole lines (with tracers
+// innermost every store filled half of each line it touched).
 // Two lanes per edge (adjacent, one wave) run the same code on alternate
 // slots of every ragged list (edgesOnEdge, its transpose, the advection
 // stencil); one lane exchange sums the partials and lane 0 finishes.
 constexpr int kEdgeLanes = 2;
+// Levels from which the edge and cell parts run one lane per entity and
+// level (the lanes of an entity's levels then cover whole lines).
+constexpr int kOneLaneLevels = 8;
+// The sums' exchange trees over kEdgeLanes (2) or kCellLanes (4) partials,
+// v[m] being the partial of lane `lane + m * PL` of the PL physical lanes:
+// with PL below the partial count one lane holds several partials (levels
+// on lanes: one lane per entity and level keeps every load a contiguous run
+// of levels) and adds them in the order of the exchange tree, so every
+// launch shape gives the same bits.
+template <int PL>
+__device__ __forceinline__ double edge_lanes_sum(const double (&v)[kEdgeLanes / PL]) {
+  if constexpr (PL == kEdgeLanes) return v[0] + __shfl_xor(v[0], 1);
+  else return v[0] + v[1];
+}
+template <int PL>
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
                                                   const double *__restrict__ tb_u,
                                                   const double *__restrict__ tb_h,
                                                   const double *__restrict__ tb_tr, double *ubA,
                                                   double *heb, double *pveb, double *r1,
                                                   double *r2, double *hbf) {
+  static_assert(PL == 1 || PL == kEdgeLanes, "one lane or kEdgeLanes lanes per edge and level");
+  constexpr int VPL = kEdgeLanes / PL;  // partials per lane
   uint32_t k0, k1;
   int lane;
-  const int e = level_item<kEdgeLanes>(M.nL, k0, k1, lane);
+  const int e = level_item<PL>(M.nL, k0, k1, lane);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE, NT = P.nT;
-  constexpr int Q = kChunk / kEdgeLanes;  // slots per lane and round
+  constexpr int Q = kChunk / kEdgeLanes;  // slots per partial and round
   const int2 cc = ld(M.e2c, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
   const double A1 = ld(M.area, c1), A2 = ld(M.area, c2);
@@ -485,74 +505,90 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
     if (!P.freeze_u) {
       const int n = ld(M.nEoE, e), nt = ld(A.nEt, e);
       const double tue = ld(tb_u, o);
-      for (int j0 = lane; j0 < n; j0 += kChunk) {
-        uint32_t oj[Q];
-        double w[Q];
+      double pbv[VPL], s1v[VPL], s2v[VPL];
 #pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const int jj = min(j0 + j * kEdgeLanes, n - 1);
-          oj[j] = uint32_t(ld(M.eoe, jj * E + e)) * L + k;
-          w[j] = ld(M.woe, jj * E + e);
-        }
+      for (int m = 0; m < VPL; m++) {
+        const int vl = lane + m * PL;
+        double pbx = 0.0, s1x = 0.0, s2x = 0.0;
+        for (int j0 = vl; j0 < n; j0 += kChunk) {
+          uint32_t oj[Q];
+          double w[Q];
 #pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const double x = 0.5 * w[j] * tue * ld(R.u, oj[j]) * ld(R.he, oj[j]);
-          pb += (j0 + j * kEdgeLanes < n) ? x : 0.0;
+          for (int j = 0; j < Q; j++) {
+            const int jj = min(j0 + j * kEdgeLanes, n - 1);
+            oj[j] = uint32_t(ld(M.eoe, jj * E + e)) * L + k;
+            w[j] = ld(M.woe, jj * E + e);
+          }
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const double x = 0.5 * w[j] * tue * ld(R.u, oj[j]) * ld(R.he, oj[j]);
+            pbx += (j0 + j * kEdgeLanes < n) ? x : 0.0;
+          }
         }
+        for (int j0 = vl; j0 < nt; j0 += kChunk) {
+          uint32_t o2[Q];
+          double w[Q];
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const int jj = min(j0 + j * kEdgeLanes, nt - 1);
+            o2[j] = uint32_t(ld(A.et, jj * E + e)) * L + k;
+            w[j] = ld(A.wt, jj * E + e);
+          }
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const double t = ld(tb_u, o2[j]) * w[j];
+            const bool live = j0 + j * kEdgeLanes < nt;
+            s1x += live ? t * 0.5 * (ld(R.pve, o2[j]) + pvee) : 0.0;
+            s2x += live ? t : 0.0;
+          }
+        }
+        pbv[m] = pbx;
+        s1v[m] = s1x;
+        s2v[m] = s2x;
       }
-      for (int j0 = lane; j0 < nt; j0 += kChunk) {
-        uint32_t o2[Q];
-        double w[Q];
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const int jj = min(j0 + j * kEdgeLanes, nt - 1);
-          o2[j] = uint32_t(ld(A.et, jj * E + e)) * L + k;
-          w[j] = ld(A.wt, jj * E + e);
-        }
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const double t = ld(tb_u, o2[j]) * w[j];
-          const bool live = j0 + j * kEdgeLanes < nt;
-          s1 += live ? t * 0.5 * (ld(R.pve, o2[j]) + pvee) : 0.0;
-          s2 += live ? t : 0.0;
-        }
-      }
-      pb += __shfl_xor(pb, 1);
-      s1 += __shfl_xor(s1, 1);
-      s2 += __shfl_xor(s2, 1);
+      pb = edge_lanes_sum<PL>(pbv);
+      s1 = edge_lanes_sum<PL>(s1v);
+      s2 = edge_lanes_sum<PL>(s2v);
     }
-    // tracer paths: the stencil sums split over the lanes, the rest on both
+    // tracer paths: the stencil sums split over the partials, the rest once
     double Fb = 0.0, hbt = 0.0;
     if (P.fct) {
       const double rho = M.msd * P.kappa * dv / dc;
       const double F = ue * hee, sgn = fsign(F);
       const double Fp = F > 0.0 ? F : 0.0, Fm = F < 0.0 ? F : 0.0;
-      // the lane's advection slots (at most 16 = kChunk slots: one chunk),
-      // loaded once for every tracer
-      uint32_t ib[Q];
-      double cf[Q];
+      // the partials' advection slots (at most 16 = kChunk slots: one
+      // chunk), loaded once for every tracer
+      uint32_t ib[VPL][Q];
+      double cf[VPL][Q];
 #pragma unroll
-      for (int j = 0; j < Q; j++) {
-        const int i = max(min(lane + j * kEdgeLanes, na - 1), 0);
-        ib[j] = uint32_t(ld(M.adv, i * E + e)) * L + k;
-        cf[j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
-      }
+      for (int m = 0; m < VPL; m++)
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+          const int i = max(min(lane + m * PL + j * kEdgeLanes, na - 1), 0);
+          ib[m][j] = uint32_t(ld(M.adv, i * E + e)) * L + k;
+          cf[m][j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
+        }
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
         const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
         const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
         const double lob = (1.0 - lam) * phib, hob = lam * phib;
         const double hf = hob * F;
-        double hs = 0.0;
+        double hsv[VPL];
 #pragma unroll
-        for (int j = 0; j < Q; j++) {
-          const int i = lane + j * kEdgeLanes;
-          if (i < na) {
-            hs += ld(R.tr, ib[j] * NT + t) * cf[j];
-            st(hbf, ((uint32_t(i) * E + e) * L + k) * NT + t, hf * cf[j]);
+        for (int m = 0; m < VPL; m++) {
+          double hsx = 0.0;
+#pragma unroll
+          for (int j = 0; j < Q; j++) {
+            const int i = lane + m * PL + j * kEdgeLanes;
+            if (i < na) {
+              hsx += ld(R.tr, ib[m][j] * NT + t) * cf[m][j];
+              st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf * cf[m][j]);
+            }
           }
+          hsv[m] = hsx;
         }
-        hs += __shfl_xor(hs, 1);
+        const double hs = edge_lanes_sum<PL>(hsv);
         if (lane) continue;
         const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
         Fb += lob * dv * ((F > 0.0 ? T1 : 0.0) + (F < 0.0 ? T2 : 0.0)) + hob * hs;
@@ -564,8 +600,8 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
           a1 -= Db;
           a2 += Db;
         }
-        st(r1, o * NT + t, a1);
-        st(r2, o * NT + t, a2);
+        st(r1, (e * NT + t) * L + k, a1);
+        st(r2, (e * NT + t) * L + k, a2);
       }
     }
     if (lane) continue;
@@ -628,6 +664,13 @@ __device__ __forceinline__ double cell_lanes_sum(double x) {
   x += __shfl_xor(x, 2);
   return x;
 }
+// The same tree over the partials v[m] of lanes lane + m * PL (edge_lanes_sum).
+template <int PL>
+__device__ __forceinline__ double cell_lanes_sum(const double (&v)[kCellLanes / PL]) {
+  if constexpr (PL == kCellLanes) return cell_lanes_sum(v[0]);
+  else return (v[0] + v[1]) + (v[2] + v[3]);
+}
+template <int PL>
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     DMesh M, AdjMesh A, AdjParams P, Rec R, Rec R0, const double *__restrict__ tb_u,
     const double *__restrict__ heb, const double *__restrict__ hvT,
@@ -635,40 +678,61 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     const double *__restrict__ hbf, double *keT, double *divT,
     double *xb_h, double *xb_tr, const double *__restrict__ aw_h,
     const double *__restrict__ aw_tr, double *tb_h, double *tb_tr) {
+  static_assert(PL == 1 || PL == kCellLanes, "one lane or kCellLanes lanes per cell and level");
+  constexpr int VPL = kCellLanes / PL;  // partials per lane
   uint32_t k0, k1;
   int lane;
-  const int c = level_item<kCellLanes>(M.nL, k0, k1, lane);
+  const int c = level_item<PL>(M.nL, k0, k1, lane);
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
   const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
   const uint8_t neg = ld(M.sgnNeg, c);
   const double ar = ld(M.area, c);
-  constexpr int Q = kChunk / kCellLanes;  // entries per lane and round
+  constexpr int Q = kChunk / kCellLanes;  // entries per partial and round
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
-    double g1 = 0.0, hsum = 0.0, Tb[8] = {};
-    for (int j = lane; j < n; j += kCellLanes) {
-      const uint32_t e = ld(M.c2e, j * C + c), oe = e * L + k;
-      if (!P.freeze_u) g1 += sign_on_cell(neg, j) * ld(tb_u, oe) / ld(M.dc, e);
-      hsum += 0.5 * ld(heb, oe);
-      const bool first = (neg >> j) & 1;
-      for (uint32_t t = 0; t < NT; t++) Tb[t] += first ? ld(r1, oe * NT + t) : ld(r2, oe * NT + t);
-    }
-    for (int i = lane; i < nv; i += kCellLanes)
-      hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
-    for (int i0 = lane; i0 < na; i0 += kChunk) {
-      uint32_t os[Q];
+    double g1v[VPL], hsumv[VPL], Tbv[VPL][8];
 #pragma unroll
-      for (int j = 0; j < Q; j++)
-        os[j] = (uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * L + k) * NT;
-      for (uint32_t t = 0; t < NT; t++) {
-#pragma unroll
-        for (int j = 0; j < Q; j++) Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, os[j] + t) : 0.0;
+    for (int m = 0; m < VPL; m++) {
+      const int vl = lane + m * PL;
+      double g1 = 0.0, hsum = 0.0, Tb[8] = {};
+      for (int j = vl; j < n; j += kCellLanes) {
+        const uint32_t e = ld(M.c2e, j * C + c), oe = e * L + k;
+        if (!P.freeze_u) g1 += sign_on_cell(neg, j) * ld(tb_u, oe) / ld(M.dc, e);
+        hsum += 0.5 * ld(heb, oe);
+        const bool first = (neg >> j) & 1;
+        for (uint32_t t = 0; t < NT; t++)
+          Tb[t] += first ? ld(r1, (e * NT + t) * L + k) : ld(r2, (e * NT + t) * L + k);
       }
+      for (int i = vl; i < nv; i += kCellLanes)
+        hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
+      for (int i0 = vl; i0 < na; i0 += kChunk) {
+        uint32_t os[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++)
+          os[j] = uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * NT;
+        for (uint32_t t = 0; t < NT; t++) {
+#pragma unroll
+          for (int j = 0; j < Q; j++)
+            Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, (os[j] + t) * L + k) : 0.0;
+        }
+      }
+      g1v[m] = g1;
+      hsumv[m] = hsum;
+#pragma unroll
+      for (int t = 0; t < 8; t++) Tbv[m][t] = Tb[t];
     }
-    g1 = cell_lanes_sum(g1);
-    hsum = cell_lanes_sum(hsum);
-    for (uint32_t t = 0; t < NT; t++) Tb[t] = cell_lanes_sum(Tb[t]);
+    const double g1 = cell_lanes_sum<PL>(g1v), hsum = cell_lanes_sum<PL>(hsumv);
+    double Tb[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      if (uint32_t(t) >= NT) break;  // uniform
+      double x[VPL];
+#pragma unroll
+      for (int m = 0; m < VPL; m++) x[m] = Tbv[m][t];
+      Tb[t] = cell_lanes_sum<PL>(x);
+    }
+
     if (lane) continue;
     const double hb = -M.gravity * g1 + hsum;
     st(keT, ol, -g1 / ar);
@@ -1060,12 +1124,22 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
   for (int s = 3; s >= 0; s--) {
     const AdjParams P = adj_params(ctx, s, dt);
     const Rec &R = A.rec[4 * i + s];
-    ADJ_LAUNCH(k_ad_edge_a, int64_t(M.nE) * kEdgeLanes, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr,
-               A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
+    // levels on lanes (kOneLaneLevels or more): one lane per entity and
+    // level, each holding all of the sums' partials
+    if (M.nL >= kOneLaneLevels)
+      ADJ_LAUNCH(k_ad_edge_a<1>, M.nE, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr, A.ubA, A.heb,
+                 A.pveb, A.r1, A.r2, A.hbf);
+    else
+      ADJ_LAUNCH(k_ad_edge_a<kEdgeLanes>, int64_t(M.nE) * kEdgeLanes, M, A.T, P, R, A.tb_u,
+                 A.tb_h, A.tb_tr, A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
     ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
-    ADJ_LAUNCH(k_ad_cell, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT,
-               A.r1, A.r2, A.hbf, A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr,
-               A.tb_h, A.tb_tr);
+    if (M.nL >= kOneLaneLevels)
+      ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT, A.r1, A.r2, A.hbf,
+                 A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
+    else
+      ADJ_LAUNCH(k_ad_cell<kCellLanes>, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, A.tb_u,
+                 A.heb, A.hvT, A.r1, A.r2, A.hbf, A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst],
+                 A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
     ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
                A.tb_u);
   }

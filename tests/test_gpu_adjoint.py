@@ -240,14 +240,18 @@ def _pack_members(m, members, nT):
     return U, H, T
 
 
-def test_members_as_levels_equal_single_runs():
+@pytest.mark.parametrize("nM", [6, 12])
+def test_members_as_levels_equal_single_runs(nM):
     """Levels of one context are independent, so a context whose levels are
     ensemble members (the batched adjoint) gives every member the final
     state and gradient of its own single-level run, bit for bit; the
-    levels run one per lane (adjacent lanes, adjacent members)."""
+    levels run one per lane (adjacent lanes, adjacent members).  With 12
+    members the adjoint's edge and cell parts take one lane per entity and
+    member, which adds the partial sums the single run spreads over 2 or 4
+    lanes in the same tree."""
     from mpas_hip import TRACER_FCT, Context, sw_workload_state
     m = mesh(3)
-    nM, nT, K, nsteps, dt = 6, 2, 3, 7, 600.0
+    nT, K, nsteps, dt = 2, 3, 7, 600.0
     kw = dict(nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5, kappa2=1e3)
     members = []
     for g in range(nM):
