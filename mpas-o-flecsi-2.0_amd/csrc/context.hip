@@ -411,6 +411,70 @@ int build_device_mesh(mpas_ctx *ctx) {
         }
       }
     }
+    // LDS staging of the one-level tracer update (MPAS_HIP_TRACER_LDS=0
+    // turns it off): per workgroup its own cells, then the other cells
+    // across their edges (ascending), whose limiter factors and tracers the
+    // update reads; each cell's slots as positions in that list.
+    const char *ts = std::getenv("MPAS_HIP_TRACER_LDS");
+    const bool full_cols = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
+                                       [&](int x) { return x == m.nL; });
+    const int tnpl = nT > 0 ? tracer_lanes_per_cell(nT) : 1;
+    if (m.nL == 1 && nT > 0 && nT % 2 == 0 && M.narrow && full_cols &&
+        ctx->params.tracer_scheme == MPAS_TRACER_FCT && kTracerBlock % tnpl == 0 &&
+        !(ts && ts[0] == '0')) {
+      const int cpb = kTracerBlock / tnpl;
+      const int nb = (C + cpb - 1) / cpb;
+      std::vector<int32_t> off(nb + 1, 0), list, halo;
+      std::vector<uint64_t> loc(C, 0);
+      int mx = 0;
+      bool ok = true;
+      for (int b = 0; b < nb && ok; b++) {
+        const int c0 = b * cpb, c1e = std::min(C, c0 + cpb);
+        halo.clear();
+        for (int c = c0; c < c1e; c++)
+          for (int j = 0; j < m.c2e.size(c); j++) {
+            const int e = m.c2e.at(c, j);
+            const int o = m.e2c.at(e, 0) == c ? m.e2c.at(e, 1) : m.e2c.at(e, 0);
+            if (o < c0 || o >= c1e) halo.push_back(o);
+          }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        const int own = c1e - c0, len = own + int(halo.size());
+        // 10-bit positions; the copy (3 nT doubles per cell) within 64 KiB
+        if (len >= 1024 || size_t(len) * 3 * nT * sizeof(double) > (64u << 10)) {
+          ok = false;
+          break;
+        }
+        mx = std::max(mx, len);
+        for (int c = c0; c < c1e; c++) {
+          uint64_t w = 0;
+          for (int j = 0; j < 6; j++) {
+            uint64_t p = uint64_t(c - c0);  // padding slots: the cell itself
+            if (j < m.c2e.size(c)) {
+              const int e = m.c2e.at(c, j);
+              const int o = m.e2c.at(e, 0) == c ? m.e2c.at(e, 1) : m.e2c.at(e, 0);
+              p = (o >= c0 && o < c1e)
+                      ? uint64_t(o - c0)
+                      : uint64_t(own + (std::lower_bound(halo.begin(), halo.end(), o) -
+                                        halo.begin()));
+            }
+            w |= p << (10 * j);
+          }
+          loc[c] = w;
+        }
+        for (int c = c0; c < c1e; c++) list.push_back(c);
+        list.insert(list.end(), halo.begin(), halo.end());
+        off[b + 1] = int32_t(list.size());
+      }
+      if (ok) {
+        if ((rc = upload(ctx, loc, &M.c2cLoc)) || (rc = upload(ctx, off, &M.tStgOff)) ||
+            (rc = upload(ctx, list, &M.tStgCell)))
+          return rc;
+        M.tStgMax = mx;
+        M.tStgLen = int(list.size());
+        M.tStgCpb = cpb;
+      }
+    }
     bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
                             [&](int x) { return x == m.nL; });
     if (!full) {

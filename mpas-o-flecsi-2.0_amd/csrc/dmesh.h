@@ -85,6 +85,16 @@ struct DMesh {
   const uint64_t *a2cLoc;
   const int *aStgOff, *aStgCell;
   int aStgMax, aStgLen, aStgEpb;
+  // LDS staging of the one-level tracer update: the tStgCpb cells of
+  // logical block b (kTracerBlock lanes, cells from b * tStgCpb) read the
+  // limiter factors and tracers of the cells tStgCell[tStgOff[b] ..
+  // tStgOff[b + 1]): the block's own cells first, in order, then the
+  // other cells across their edges, ascending; the 10-bit field j of
+  // c2cLoc[c] is the position of the cell across edge slot j of cell c
+  // (slots past the cell's count: its own position).  nullptr: none.
+  const uint64_t *c2cLoc;
+  const int *tStgOff, *tStgCell;
+  int tStgMax, tStgLen, tStgCpb;
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere
@@ -107,6 +117,16 @@ constexpr int kHoBlock = 256;
 #endif
 inline int ho_lanes_per_edge(int nT) {
   return nT % MPAS_TPL_HO == 0 ? nT / MPAS_TPL_HO : nT % 2 == 0 ? nT / 2 : nT;
+}
+
+// The same for the tracer update, whose lanes take MPAS_TPL_TRACER
+// tracers each (one: four lanes per cell at four tracers).
+constexpr int kTracerBlock = 256;
+#ifndef MPAS_TPL_TRACER
+#define MPAS_TPL_TRACER 1
+#endif
+inline int tracer_lanes_per_cell(int nT) {
+  return nT % MPAS_TPL_TRACER == 0 ? nT / MPAS_TPL_TRACER : nT % 2 == 0 ? nT / 2 : nT;
 }
 
 // explicit advCoef_3rd values per edge in the compact form: the two cells of

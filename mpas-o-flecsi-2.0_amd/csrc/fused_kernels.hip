@@ -67,6 +67,12 @@ constexpr int BS_EDGES = MPAS_BS_EDGES, BS_HO = MPAS_BS_HO, BS_CELLS = MPAS_BS_C
 #ifndef MPAS_WG_PER_CU_TRACER_LANES  // the same with levels on lanes
 #define MPAS_WG_PER_CU_TRACER_LANES MPAS_WG_PER_CU_TRACER
 #endif
+// The same for the LDS-staged tracer update, whose neighbour values come
+// from LDS: it ran fastest without a limit (66.0 us per launch, against
+// 89.8, 73.8 and 67.4 us with 4, 6 and 8 workgroups per CU).
+#ifndef MPAS_WG_PER_CU_TRACER_LDS
+#define MPAS_WG_PER_CU_TRACER_LDS 0
+#endif
 constexpr unsigned kLdsPerCu = 160u * 1024u;
 constexpr unsigned occupancy_lds(int wg) {
   return wg > 0 ? (kLdsPerCu / unsigned(wg + 1) + kLdsPerCu / unsigned(wg)) / 2 : 0u;
@@ -88,12 +94,10 @@ __global__ __launch_bounds__(BS_EDGES) MPAS_WAVES(MPAS_WPE_EDGES) void k_stage_e
   stage_edges<WE, LOL, LP, LP>(M, S, xcd_item());
 }
 // Tracers per lane of the cell kernels (0: pairs when the count is even;
-// the high-order flux kernel's count, MPAS_TPL_HO, is in dmesh.h).
+// the high-order flux kernel's count, MPAS_TPL_HO, and the one-level
+// tracer update's, MPAS_TPL_TRACER, are in dmesh.h).
 #ifndef MPAS_TPL_CELLS
 #define MPAS_TPL_CELLS 0
-#endif
-#ifndef MPAS_TPL_TRACER
-#define MPAS_TPL_TRACER 1
 #endif
 // With levels on lanes the tracer update takes pairs: the lanes of a cell
 // then cover its levels x tracer pairs, and one lane per pair halves the
@@ -155,6 +159,16 @@ template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
   stage_tracer<NT, W, LOL, tpl_of<NT>(tpl_tracer(LOL)), 1>(M, S, xcd_item());
+}
+// One-level tracer update with the limiter factors and tracers of its
+// cells and of their neighbours staged in LDS (DMesh::c2cLoc); dynamic LDS
+// of 3 * NT * M.tStgMax doubles, or the occupancy reservation if larger.
+static_assert(BS_TRACER == kTracerBlock, "the LDS staging lists are built per kTracerBlock lanes");
+template <int NT, int W>
+__global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer_lds(
+    DMesh M, StageIO S) {
+  extern __shared__ double lds[];
+  stage_tracer<NT, W, 0, tpl_of<NT>(tpl_tracer(0)), 1, true>(M, S, xcd_item(), lds);
 }
 
 template <int LOL, int LP = 1>
@@ -255,6 +269,9 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   // k_stage_tracer
   if (fct) {
     b.tracer = 10 * C + 8 * sum_eoc;                               // nEoC sgn area, c2e cOpp
+    if (M.c2cLoc && L == 1)  // slot positions and the workgroups' lists instead of cOpp
+      b.tracer += 8 * C + 4.0 * M.tStgLen + 4 * ((C + M.tStgCpb - 1) / M.tStgCpb + 1) -
+                  4 * sum_eoc;
     b.tracer += d * L * T * (C + 2 * C + E) + d * L * C;           // tlo rfac anti, yh
     b.tracer += d * L * T * C * touches + (S.stage > 0 && S.stage < 3 ? d * L * C : 0);
     if (S.use_hmix) b.tracer += 2 * d * E + d * L * E + d * L * T * C;  // dv dc h_edge xtr
@@ -367,12 +384,21 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   if constexpr (NT > 0) {
     if (fct) {
       MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
-      hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>),
-                         dim3(xcd_grid(M.nC * lv * lanes_per_cell<NT, tpl_tracer(LOL)>(),
-                                       BS_TRACER)),
-                         dim3(BS_TRACER),
-                         occupancy_lds(LOL ? MPAS_WG_PER_CU_TRACER_LANES : MPAS_WG_PER_CU_TRACER),
-                         st, M, S);
+      const dim3 grid(xcd_grid(M.nC * lv * lanes_per_cell<NT, tpl_tracer(LOL)>(), BS_TRACER));
+      const size_t occ =
+          occupancy_lds(LOL ? MPAS_WG_PER_CU_TRACER_LANES : MPAS_WG_PER_CU_TRACER);
+      bool staged = false;
+      if constexpr (LOL == 0 && W <= 6 && NT % 2 == 0) {
+        if (M.c2cLoc) {
+          const size_t bytes = size_t(3) * NT * M.tStgMax * sizeof(double);
+          const size_t occ_s = occupancy_lds(MPAS_WG_PER_CU_TRACER_LDS);
+          hipLaunchKernelGGL((k_stage_tracer_lds<NT, W>), grid, dim3(BS_TRACER),
+                             bytes > occ_s ? bytes : occ_s, st, M, S);
+          staged = true;
+        }
+      }
+      if (!staged)
+        hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>), grid, dim3(BS_TRACER), occ, st, M, S);
       MPAS_STEP(after_launch("k_stage_tracer"));
       MPAS_STEP(prof_end(ctx));
     }

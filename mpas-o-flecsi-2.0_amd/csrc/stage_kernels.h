@@ -957,9 +957,50 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 }
 
 // ---- cells x tracers: FCT rescale + accumulate, del2 mixing, update --------
-template <int NT, int W, int LOL = 0, int TPLX = 0, int FIX = 0>
-__device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item) {
+//
+// STG (one-level launches of kTracerBlock lanes, DMesh::c2cLoc set): the
+// workgroup first copies the limiter factors and tracers of its own cells
+// and of the cells across their edges into LDS (`lds`: M.tStgMax records
+// of 2 NT factors, then M.tStgMax records of NT tracers), then every cell
+// reads its own and its neighbours' values there by position, where the
+// sums take them.  A cell's neighbours are mostly the block's own cells or
+// shared with other cells of the block, so the copy reads each record once
+// where the gathers read it about four times.
+template <int NT, int W, int LOL = 0, int TPLX = 0, int FIX = 0, bool STG = false>
+__device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, int item,
+                                             double *lds = nullptr) {
   constexpr int TPL = TPLX ? TPLX : tracers_per_lane<NT>(), NPL = NT / TPL;
+  static_assert(!STG || (LOL == 0 && W <= 6 && NT % 2 == 0),
+                "LDS staging serves one-level launches, at most six edges per cell, even tracer counts");
+  const uint32_t mx = STG ? uint32_t(M.tStgMax) : 0u;
+  double *const s_rf = lds, *const s_tr = lds + 2 * NT * mx;
+  int cb = 0;  // STG: the block's first cell
+  if constexpr (STG) {
+    // every lane of the workgroup takes part, before any returns
+    const int lb = item / kTracerBlock;
+    cb = lb * M.tStgCpb;
+    const int nb = (M.nC + M.tStgCpb - 1) / M.tStgCpb;
+    if (lb < nb) {
+      const int b0 = ld(M.tStgOff, uint32_t(lb)), nc = ld(M.tStgOff, uint32_t(lb + 1)) - b0;
+      // 16-byte pieces: NT of a cell's factor record (2 NT doubles), then
+      // NT / 2 of its tracer record; consecutive lanes take consecutive
+      // pieces, so the block's own cells are copied in one contiguous run
+      constexpr int P = NT + NT / 2;
+      const double2 *rf2 = reinterpret_cast<const double2 *>(S.rfac);
+      const double2 *tr2 = reinterpret_cast<const double2 *>(S.xtr);
+      double2 *lrf = reinterpret_cast<double2 *>(s_rf);
+      double2 *ltr = reinterpret_cast<double2 *>(s_tr);
+      for (int q = int(threadIdx.x); q < nc * P; q += int(blockDim.x)) {
+        const int i = q / P, r = q - i * P;
+        const uint32_t x = uint32_t(ld(M.tStgCell, uint32_t(b0 + i)));
+        if (r < NT)
+          lrf[i * NT + r] = ld(rf2, x * uint32_t(NT) + uint32_t(r));
+        else
+          ltr[i * (NT / 2) + (r - NT)] = ld(tr2, x * uint32_t(NT / 2) + uint32_t(r - NT));
+      }
+    }
+    __syncthreads();
+  }
   int sub;
   uint32_t k0, k1;
   const int c = split_levels<LOL, NPL, FIX>(S, M.nL, item, sub, k0, k1);
@@ -970,6 +1011,14 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
   const uint8_t neg = ldc(M.sgnNeg, c);
   const double invAreaCell1 = 1.0 / ldc(M.area, c);
   uint32_t e[W], op[W];
+  // STG: positions of the cell (pos[W]) and of its neighbours in the list
+  uint32_t pos[W + 1];
+  if constexpr (STG) {
+    const uint64_t loc = ldc(M.c2cLoc, uint32_t(c));
+#pragma unroll
+    for (int j = 0; j < W; j++) pos[j] = uint32_t(loc >> (10 * j)) & 1023u;
+    pos[W] = uint32_t(c - cb);
+  }
   // with levels on lanes lane r of the cell's block loads slot r's rows
   const bool shared = LOL == 1 && share_rows<W>(S.nchunk * NPL);
   int base = 0, js = 0;
@@ -989,10 +1038,10 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     for (int m = 0; m < MS; m++) {
       const int j = min(sub + m * NPL, W - 1);
       me[m] = ldn<kNtCells>(M.c2e, j * C + c);
-      mo[m] = ldn<kNtCells>(M.cOpp, j * C + c);
+      if constexpr (!STG) mo[m] = ldn<kNtCells>(M.cOpp, j * C + c);
     }
     grp_row<NPL>(e, me, base);
-    grp_row<NPL>(op, mo, base);
+    if constexpr (!STG) grp_row<NPL>(op, mo, base);
   } else if (shared) {
     js = row_slot(item, S.nchunk * NPL, W, base);
     ej = ld(M.c2e, js * C + c);
@@ -1004,7 +1053,7 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
 #pragma unroll
     for (int j = 0; j < W; j++) {
       e[j] = ldn<LOL == 0 && kNtCells>(M.c2e, j * C + c);
-      op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
+      if constexpr (!STG) op[j] = ldn<LOL == 0 && kNtCells>(M.cOpp, j * C + c);
     }
   }
   MPAS_PHASE();
@@ -1012,24 +1061,31 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     const uint32_t ol = c * L + k, ot = ol * NT + t0;
     double te[TPL], rs[2 * TPL];
     ld_group<TPL>(S.tlo, ot, te);
+    if constexpr (STG) {
 #pragma unroll
-    for (int g = 0; g < TPL; g++) {
-      double pair[2];
-      ld_group<2>(S.rfac, (ot + g) * 2, pair);
-      rs[2 * g] = pair[0];
-      rs[2 * g + 1] = pair[1];
+      for (int g = 0; g < 2 * TPL; g++) rs[g] = s_rf[pos[W] * (2 * NT) + 2 * t0 + g];
+    } else {
+#pragma unroll
+      for (int g = 0; g < TPL; g++) {
+        double pair[2];
+        ld_group<2>(S.rfac, (ot + g) * 2, pair);
+        rs[2 * g] = pair[0];
+        rs[2 * g + 1] = pair[1];
+      }
     }
     double an[W][TPL], ro[W][2 * TPL];
 #pragma unroll
     for (int j = 0; j < W; j++) {
       ld_group<TPL>(S.anti, (e[j] * L + k) * NT + t0, an[j]);
-      const uint32_t oo = (op[j] * L + k) * NT + t0;
+      if constexpr (!STG) {
+        const uint32_t oo = (op[j] * L + k) * NT + t0;
 #pragma unroll
-      for (int g = 0; g < TPL; g++) {
-        double pair[2];
-        ld_group<2>(S.rfac, (oo + g) * 2, pair);
-        ro[j][2 * g] = pair[0];
-        ro[j][2 * g + 1] = pair[1];
+        for (int g = 0; g < TPL; g++) {
+          double pair[2];
+          ld_group<2>(S.rfac, (oo + g) * 2, pair);
+          ro[j][2 * g] = pair[0];
+          ro[j][2 * g + 1] = pair[1];
+        }
       }
     }
     MPAS_PHASE();
@@ -1041,7 +1097,9 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
 #pragma unroll
       for (int g = 0; g < TPL; g++) {
         const double rin = rs[2 * g], rout = rs[2 * g + 1];
-        const double oin = ro[j][2 * g], oout = ro[j][2 * g + 1];
+        // STG: the neighbour's factors read from LDS here
+        const double oin = STG ? s_rf[pos[j] * (2 * NT) + 2 * (t0 + g)] : ro[j][2 * g];
+        const double oout = STG ? s_rf[pos[j] * (2 * NT) + 2 * (t0 + g) + 1] : ro[j][2 * g + 1];
         const double in1 = first ? rin : oin, out1 = first ? rout : oout;
         const double in2 = first ? oin : rin, out2 = first ? oout : rout;
         const double x = an[j][g];
@@ -1053,7 +1111,12 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
     if (S.use_hmix) {
       // tracer_hmix_del2_tend (ocean/solve.cc:328-366)
       double tc[TPL], tn[W][TPL], he[W], rho[W];
-      ld_group<TPL>(S.xtr, ot, tc);
+      if constexpr (STG) {
+#pragma unroll
+        for (int g = 0; g < TPL; g++) tc[g] = s_tr[pos[W] * NT + t0 + g];
+      } else {
+        ld_group<TPL>(S.xtr, ot, tc);
+      }
       if constexpr (GRP) {
         double hm[MS], rm[MS];
 #pragma unroll
@@ -1061,14 +1124,16 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
           hm[m] = ld(S.h_edge, me[m] * L + k);
           rm[m] = M.msd * S.kappa * ldc(M.dv, me[m]) / ldc(M.dc, me[m]);
         }
+        if constexpr (!STG) {
 #pragma unroll
-        for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+          for (int j = 0; j < W; j++) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+        }
         grp_row<NPL>(he, hm, base);
         grp_row<NPL>(rho, rm, base);
       } else {
 #pragma unroll
         for (int j = 0; j < W; j++) {
-          ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
+          if constexpr (!STG) ld_group<TPL>(S.xtr, (op[j] * L + k) * NT + t0, tn[j]);
           he[j] = ld(S.h_edge, e[j] * L + k);
         }
       }
@@ -1088,7 +1153,8 @@ __device__ __forceinline__ void stage_tracer(const DMesh &M, const StageIO &S, i
         const bool first = (neg >> j) & 1;
 #pragma unroll
         for (int g = 0; g < TPL; g++) {
-          const double turb = first ? tn[j][g] - tc[g] : tc[g] - tn[j][g];
+          const double tnj = STG ? s_tr[pos[j] * NT + t0 + g] : tn[j][g];
+          const double turb = first ? tnj - tc[g] : tc[g] - tnj;
           const double flux = he[j] * turb * rho[j];
           const double y = te[g] - (first ? -1.0 : 1.0) * flux * invAreaCell1;
           // every level is below maxLevelEdgeTop here: the fused FCT runs

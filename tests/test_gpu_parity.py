@@ -451,13 +451,17 @@ def test_cells_lds_staging_bitwise(monkeypatch, nT):
 
 @pytest.mark.parametrize("var,nT", [("MPAS_HIP_EDGES_LDS", 0), ("MPAS_HIP_EDGES_LDS", 4),
                                     ("MPAS_HIP_HO_LDS", 2), ("MPAS_HIP_HO_LDS", 4),
-                                    ("MPAS_HIP_HO_LDS", 8)])
+                                    ("MPAS_HIP_HO_LDS", 8), ("MPAS_HIP_TRACER_LDS", 2),
+                                    ("MPAS_HIP_TRACER_LDS", 4), ("MPAS_HIP_TRACER_LDS", 8)])
 def test_lds_staging_bitwise(monkeypatch, var, nT):
-    """The one-level momentum and high-order flux kernels with the values
-    their stencils gather staged in LDS per workgroup (the default; the
-    variable set to 0, read when the context is made, turns it off) give
-    the bits of the global-gather kernels: 7680 edges make 30 workgroups
-    (60 for the flux kernel at 8 tracers, two lanes per edge)."""
+    """The one-level momentum, high-order flux and tracer-update kernels
+    with the values their stencils gather staged in LDS per workgroup (the
+    default; the variable set to 0, read when the context is made, turns it
+    off) give the bits of the global-gather kernels: 7680 edges make 30
+    workgroups (60 for the flux kernel at 8 tracers, two lanes per edge);
+    the tracer update's 2562 cells make 21, 41 and 81 workgroups of 128, 64
+    and 32 cells at 2, 4 and 8 tracers, the last one partly empty, with
+    del2 mixing on (its neighbours' tracers come from LDS too)."""
     from mpas_hip import TRACER_FCT, Context, sw_workload_state
     m = mesh(4)
     u, h, tr = sw_workload_state(m, nT, seed=22)
@@ -465,7 +469,8 @@ def test_lds_staging_bitwise(monkeypatch, var, nT):
     out = []
     for flag in ("0", "1"):
         monkeypatch.setenv(var, flag)
-        ctx = Context(m, nTracers=nT, tracer_scheme=TRACER_FCT if nT else 0, visc2=1e5)
+        ctx = Context(m, nTracers=nT, tracer_scheme=TRACER_FCT if nT else 0, visc2=1e5,
+                      kappa2=1e3 if var == "MPAS_HIP_TRACER_LDS" else 0.0)
         ctx.sw_upload(h, u, tr if nT else None, h_s)
         ctx.rk4_step(480.0, 3)
         out.append(ctx.sw_download())
