@@ -1225,10 +1225,11 @@ int mpas_batch_autotune(mpas_batch *B, int trial_steps, char *report, size_t cap
     return rc ? 1e30 : t;
   };
   double best = trial("start");
-  // the mesh order first, each candidate from the saved state
+  // the mesh order first, each candidate from the saved state (the start
+  // trial measured the starting order: every other order is tried once)
   const int start_order = B->mesh_order;
   for (int o = MPAS_ORDER_GIVEN; o <= MPAS_ORDER_MORTON && !rc; o++) {
-    if (o == B->mesh_order) continue;
+    if (o == start_order) continue;
     const int keep = B->mesh_order;
     if ((rc = permute_state(B, static_cast<const char *>(save), start_order, o))) break;
     const double t = trial("mesh order");

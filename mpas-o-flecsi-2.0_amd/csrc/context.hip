@@ -78,6 +78,65 @@ int upload(mpas_ctx *ctx, const std::vector<T> &v, const T **out) {
   return 0;
 }
 
+// Cell staging lists of the one-level cell kernels, for workgroups of cpb
+// cells: per block its own cells in order, then the other cells across
+// their edges, ascending (list, off); loc[c] holds in bits 10 j .. the
+// position of the cell across edge slot j of cell c (padding slots: its
+// own).  False when a list would not fit 10-bit positions or `bytes_per`
+// bytes of LDS per listed cell within 64 KiB.
+struct CellLists {
+  std::vector<int32_t> off, list;
+  std::vector<uint64_t> loc;
+  int mx = 0;
+};
+bool cell_lists(const HostMesh &m, int cpb, size_t bytes_per, CellLists &out) {
+  const int C = m.nC;
+  const int nb = (C + cpb - 1) / cpb;
+  out.off.assign(nb + 1, 0);
+  out.list.clear();
+  out.loc.assign(C, 0);
+  out.mx = 0;
+  std::vector<int32_t> halo;
+  auto other = [&](int c, int j) {
+    const int e = m.c2e.at(c, j);
+    return m.e2c.at(e, 0) == c ? m.e2c.at(e, 1) : m.e2c.at(e, 0);
+  };
+  for (int b = 0; b < nb; b++) {
+    const int c0 = b * cpb, c1e = std::min(C, c0 + cpb);
+    halo.clear();
+    for (int c = c0; c < c1e; c++) {
+      if (m.c2e.size(c) > 6) return false;  // six 10-bit positions per cell
+      for (int j = 0; j < m.c2e.size(c); j++) {
+        const int o = other(c, j);
+        if (o < c0 || o >= c1e) halo.push_back(o);
+      }
+    }
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    const int own = c1e - c0, len = own + int(halo.size());
+    if (len >= 1024 || size_t(len) * bytes_per > (64u << 10)) return false;
+    out.mx = std::max(out.mx, len);
+    for (int c = c0; c < c1e; c++) {
+      uint64_t w = 0;
+      for (int j = 0; j < 6; j++) {
+        uint64_t p = uint64_t(c - c0);
+        if (j < m.c2e.size(c)) {
+          const int o = other(c, j);
+          p = (o >= c0 && o < c1e)
+                  ? uint64_t(o - c0)
+                  : uint64_t(own + (std::lower_bound(halo.begin(), halo.end(), o) - halo.begin()));
+        }
+        w |= p << (10 * j);
+      }
+      out.loc[c] = w;
+    }
+    for (int c = c0; c < c1e; c++) out.list.push_back(c);
+    out.list.insert(out.list.end(), halo.begin(), halo.end());
+    out.off[b + 1] = int32_t(out.list.size());
+  }
+  return true;
+}
+
 int alloc_field(mpas_ctx *ctx, size_t n, double **out) {
   *out = nullptr;
   if (!n) return 0;
@@ -411,68 +470,40 @@ int build_device_mesh(mpas_ctx *ctx) {
         }
       }
     }
-    // LDS staging of the one-level tracer update (MPAS_HIP_TRACER_LDS=0
-    // turns it off): per workgroup its own cells, then the other cells
-    // across their edges (ascending), whose limiter factors and tracers the
-    // update reads; each cell's slots as positions in that list.
-    const char *ts = std::getenv("MPAS_HIP_TRACER_LDS");
+    // LDS staging of cell values in the one-level tracer update
+    // (MPAS_HIP_TRACER_LDS=0 turns it off: its cells' and their
+    // neighbours' limiter factors and tracers) and, on request,
+    // the cells kernel (MPAS_HIP_CELLS_CST=1: their tracers and thickness),
+    // each over its own workgroups' cells.  The cells kernel is slower
+    // with it on the 655,362-cell mesh with 4 tracers (84.0 against 72.9 us
+    // per launch): at 118 VGPRs it holds 4 waves per SIMD, and the copy's
+    // barrier costs more than the gathers it saves.
     const bool full_cols = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),
                                        [&](int x) { return x == m.nL; });
-    const int tnpl = nT > 0 ? tracer_lanes_per_cell(nT) : 1;
     if (m.nL == 1 && nT > 0 && nT % 2 == 0 && M.narrow && full_cols &&
-        ctx->params.tracer_scheme == MPAS_TRACER_FCT && kTracerBlock % tnpl == 0 &&
-        !(ts && ts[0] == '0')) {
-      const int cpb = kTracerBlock / tnpl;
-      const int nb = (C + cpb - 1) / cpb;
-      std::vector<int32_t> off(nb + 1, 0), list, halo;
-      std::vector<uint64_t> loc(C, 0);
-      int mx = 0;
-      bool ok = true;
-      for (int b = 0; b < nb && ok; b++) {
-        const int c0 = b * cpb, c1e = std::min(C, c0 + cpb);
-        halo.clear();
-        for (int c = c0; c < c1e; c++)
-          for (int j = 0; j < m.c2e.size(c); j++) {
-            const int e = m.c2e.at(c, j);
-            const int o = m.e2c.at(e, 0) == c ? m.e2c.at(e, 1) : m.e2c.at(e, 0);
-            if (o < c0 || o >= c1e) halo.push_back(o);
-          }
-        std::sort(halo.begin(), halo.end());
-        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-        const int own = c1e - c0, len = own + int(halo.size());
-        // 10-bit positions; the copy (3 nT doubles per cell) within 64 KiB
-        if (len >= 1024 || size_t(len) * 3 * nT * sizeof(double) > (64u << 10)) {
-          ok = false;
-          break;
-        }
-        mx = std::max(mx, len);
-        for (int c = c0; c < c1e; c++) {
-          uint64_t w = 0;
-          for (int j = 0; j < 6; j++) {
-            uint64_t p = uint64_t(c - c0);  // padding slots: the cell itself
-            if (j < m.c2e.size(c)) {
-              const int e = m.c2e.at(c, j);
-              const int o = m.e2c.at(e, 0) == c ? m.e2c.at(e, 1) : m.e2c.at(e, 0);
-              p = (o >= c0 && o < c1e)
-                      ? uint64_t(o - c0)
-                      : uint64_t(own + (std::lower_bound(halo.begin(), halo.end(), o) -
-                                        halo.begin()));
-            }
-            w |= p << (10 * j);
-          }
-          loc[c] = w;
-        }
-        for (int c = c0; c < c1e; c++) list.push_back(c);
-        list.insert(list.end(), halo.begin(), halo.end());
-        off[b + 1] = int32_t(list.size());
-      }
-      if (ok) {
-        if ((rc = upload(ctx, loc, &M.c2cLoc)) || (rc = upload(ctx, off, &M.tStgOff)) ||
-            (rc = upload(ctx, list, &M.tStgCell)))
+        ctx->params.tracer_scheme == MPAS_TRACER_FCT) {
+      const char *ts = std::getenv("MPAS_HIP_TRACER_LDS");
+      const int tnpl = tracer_lanes_per_cell(nT);
+      CellLists L;
+      if (kTracerBlock % tnpl == 0 && !(ts && ts[0] == '0') &&
+          cell_lists(m, kTracerBlock / tnpl, 3 * nT * sizeof(double), L)) {
+        if ((rc = upload(ctx, L.loc, &M.c2cLoc)) || (rc = upload(ctx, L.off, &M.tStgOff)) ||
+            (rc = upload(ctx, L.list, &M.tStgCell)))
           return rc;
-        M.tStgMax = mx;
-        M.tStgLen = int(list.size());
-        M.tStgCpb = cpb;
+        M.tStgMax = L.mx;
+        M.tStgLen = int(L.list.size());
+        M.tStgCpb = kTracerBlock / tnpl;
+      }
+      const char *cs = std::getenv("MPAS_HIP_CELLS_CST");
+      const int cnpl = nT / 2;  // the cells kernel takes tracer pairs
+      if (kCellsBlock % cnpl == 0 && cs && cs[0] == '1' && !M.c2eLoc &&
+          cell_lists(m, kCellsBlock / cnpl, (nT + 1) * sizeof(double), L)) {
+        if ((rc = upload(ctx, L.loc, &M.c2cLocC)) || (rc = upload(ctx, L.off, &M.cStgOff)) ||
+            (rc = upload(ctx, L.list, &M.cStgCell)))
+          return rc;
+        M.cStgMax = L.mx;
+        M.cStgLen = int(L.list.size());
+        M.cStgCpb = kCellsBlock / cnpl;
       }
     }
     bool full = std::all_of(m.maxLevelCell.begin(), m.maxLevelCell.end(),

@@ -95,6 +95,12 @@ struct DMesh {
   const uint64_t *c2cLoc;
   const int *tStgOff, *tStgCell;
   int tStgMax, tStgLen, tStgCpb;
+  // The same lists for the one-level FCT cells kernel's workgroups
+  // (kCellsBlock lanes, cStgCpb cells), which stage their cells' and their
+  // neighbours' tracers and thickness.
+  const uint64_t *c2cLocC;
+  const int *cStgOff, *cStgCell;
+  int cStgMax, cStgLen, cStgCpb;
   const int *mlc;     // maxLevelCell [C]; nullptr = nL everywhere
   const int *mlet;    // maxLevelEdgeTop [E]; nullptr = nL everywhere
   const int *hoMask;  // highOrderAdvMask [E][nL]; nullptr = 1 everywhere

@@ -155,6 +155,16 @@ __global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_c
   stage_cells<NT, W, 0, tpl_of<NT>(MPAS_TPL_CELLS), MPAS_TRACER_FCT, 1, true>(M, S, xcd_item(),
                                                                                 lds);
 }
+// One-level FCT cells with its cells' and their neighbours' tracers and
+// thickness staged in LDS (DMesh::c2cLocC); dynamic LDS of (NT + 1) *
+// M.cStgMax doubles.
+template <int NT, int W>
+__global__ __launch_bounds__(BS_CELLS) MPAS_WAVES(MPAS_WPE_CELLS) void k_stage_cells_cst(DMesh M,
+                                                                                   StageIO S) {
+  extern __shared__ double lds[];
+  stage_cells<NT, W, 0, tpl_of<NT>(MPAS_TPL_CELLS), MPAS_TRACER_FCT, 1, false, true>(
+      M, S, xcd_item(), lds);
+}
 template <int NT, int W, int LOL>
 __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer(DMesh M,
                                                                                  StageIO S) {
@@ -265,6 +275,9 @@ Bytes stage_bytes(const DMesh &M, const StageIO &S, double sum_eoe, double sum_a
   b.cells += d * L * 3 * E;                                        // yu xu h_edge
   b.cells += d * L * C * (touches + 1 + (S.use_visc ? 1 : 0));     // h0/nh/yh, ke_y, div_y
   if (fct) b.cells += 4 * sum_eoc + d * L * (C + T * C + T * E + 3 * T * C);  // cOpp, xh, xtr, anti, rfac tlo
+  if (fct && M.c2cLocC && !M.c2eLoc && L == 1)  // positions and lists instead of cOpp
+    b.cells += 8 * C + 4.0 * M.cStgLen + 4 * ((C + M.cStgCpb - 1) / M.cStgCpb + 1) -
+               4 * sum_eoc - (S.ocean ? 0 : d * E);  // h_edge formed from the thickness
   if (cen) b.cells += 4 * sum_eoc + d * L * T * C * (touches + 1);  // cOpp, xtr + tr0/ntr/ytr
   // k_stage_tracer
   if (fct) {
@@ -371,6 +384,14 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
         if (M.c2eLoc) {
           hipLaunchKernelGGL((k_stage_cells_lds<NT, W>), grid, dim3(BS_CELLS),
                              size_t(M.stgMax) * (5 + NT) * sizeof(double), st, M, S);
+          staged = true;
+        }
+      }
+      if constexpr (LOL == 0 && NT > 0 && NT % 2 == 0 && W <= 6 &&
+                    tracers_per_lane<NT>() % 2 == 0 && MPAS_TPL_CELLS == 0) {
+        if (!staged && M.c2cLocC) {
+          hipLaunchKernelGGL((k_stage_cells_cst<NT, W>), grid, dim3(BS_CELLS),
+                             size_t(M.cStgMax) * (NT + 1) * sizeof(double), st, M, S);
           staged = true;
         }
       }

@@ -620,13 +620,51 @@ __device__ __forceinline__ void ocean_update(const StageIO &S, uint32_t ot, doub
 // distinct edges of its cells into LDS, ascending edge id, then every cell
 // reads them there instead of gathering them from L2; `lds` holds
 // 5 + NT arrays of M.stgMax doubles.
-template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1, int FIX = 0, bool STG = false>
+//
+// CST (one-level FCT launches of kCellsBlock lanes, DMesh::c2cLocC set):
+// the workgroup first copies the tracers and thickness of its own cells
+// and of the cells across their edges into LDS (`lds`: M.cStgMax records
+// of NT tracers, then M.cStgMax thicknesses); every cell reads its own and
+// its neighbours' values there by position, and forms the thickness at its
+// edges, 0.5 (h[c1] + h[c2]) (solve.cc:31-39, the diagnostics that made
+// h_edge), from them instead of loading h_edge.
+template <int NT, int W, int LOL = 0, int TPLX = 0, int SCH = -1, int FIX = 0, bool STG = false,
+          bool CST = false>
 __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, int item,
                                             double *lds = nullptr) {
   constexpr int TPL = NT > 0 ? (TPLX ? TPLX : tracers_per_lane<NT>()) : 1;
   constexpr int NPL = NT > 0 ? NT / TPL : 1;
   static_assert(!STG || (LOL == 0 && NT > 0 && SCH == MPAS_TRACER_FCT),
                 "LDS staging serves one-level FCT launches");
+  static_assert(!CST || (LOL == 0 && NT > 0 && NT % 2 == 0 && SCH == MPAS_TRACER_FCT && W <= 6 &&
+                         !STG && TPL % 2 == 0),
+                "cell staging serves one-level FCT launches of tracer pairs, six edges per cell");
+  const uint32_t cmx = CST ? uint32_t(M.cStgMax) : 0u;
+  double *const c_tr = lds, *const c_h = lds + NT * cmx;
+  int cb = 0;  // CST: the block's first cell
+  if constexpr (CST) {
+    // every lane of the workgroup takes part, before any returns
+    const int lb = item / kCellsBlock;
+    cb = lb * M.cStgCpb;
+    const int nb = (M.nC + M.cStgCpb - 1) / M.cStgCpb;
+    if (lb < nb) {
+      const int b0 = ld(M.cStgOff, uint32_t(lb)), nc = ld(M.cStgOff, uint32_t(lb + 1)) - b0;
+      // pieces: NT / 2 16-byte pieces of a cell's tracer record, then its
+      // thickness; consecutive lanes take consecutive pieces
+      constexpr int P = NT / 2 + 1;
+      const double2 *tr2 = reinterpret_cast<const double2 *>(S.xtr);
+      double2 *ltr = reinterpret_cast<double2 *>(c_tr);
+      for (int q = int(threadIdx.x); q < nc * P; q += int(blockDim.x)) {
+        const int i = q / P, r = q - i * P;
+        const uint32_t x = uint32_t(ld(M.cStgCell, uint32_t(b0 + i)));
+        if (r < NT / 2)
+          ltr[i * (NT / 2) + r] = ld(tr2, x * uint32_t(NT / 2) + uint32_t(r));
+        else
+          c_h[i] = ld(S.xh, x);
+      }
+    }
+    __syncthreads();
+  }
   const uint32_t mx = STG ? uint32_t(M.stgMax) : 0u;
   double *const s_yu = lds, *const s_xu = lds + mx, *const s_he = lds + 2 * mx;
   double *const s_dv = lds + 3 * mx, *const s_dc = lds + 4 * mx, *const s_an = lds + 5 * mx;
@@ -657,6 +695,18 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
   const int t0 = sub * TPL;
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC;
+  // CST: positions of the cells across the edge slots and (cpos[W]) of the
+  // cell itself in the workgroup's cell list
+  uint32_t cpos[W + 1];
+  if constexpr (CST) {
+    const uint64_t loc = ldc(M.c2cLocC, uint32_t(c));
+#pragma unroll
+    for (int j = 0; j < W; j++) cpos[j] = uint32_t(loc >> (10 * j)) & 1023u;
+    cpos[W] = uint32_t(c - cb);
+  }
+  // CST: h_edge formed from the staged thickness (not in the tracer-only
+  // run, whose h_edge is its own fixed field)
+  const bool he_h = CST && !S.ocean;
   uint32_t pos[W];  // STG: each slot's position in the workgroup's edge list
   if constexpr (STG) {
     const uint64_t loc = ldc(M.c2eLoc, uint32_t(c));
@@ -785,10 +835,15 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
       for (int m = 0; m < MS; m++) {
         xm[m] = ld(S.xu, me[m] * L + k);
-        hm[m] = ld(S.h_edge, me[m] * L + k);
+        if (!he_h) hm[m] = ld(S.h_edge, me[m] * L + k);
       }
       grp_row<NPL>(xu, xm, base);
-      grp_row<NPL>(he, hm, base);
+      if (he_h) {  // uniform: every lane of the cell takes this branch
+#pragma unroll
+        for (int j = 0; j < W; j++) he[j] = 0.5 * (c_h[cpos[W]] + c_h[cpos[j]]);
+      } else {
+        grp_row<NPL>(he, hm, base);
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < W; j++) {
@@ -830,9 +885,14 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       const double invArea = 1.0 / ar;
       const uint32_t ot = ol * NT + t0;
       double tc[TPL];
-      ld_group<TPL>(S.xtr, ot, tc);
       double tn[W][TPL];
-      {
+      if constexpr (CST) {
+        // the cell's tracers from LDS (its neighbours' are read there where
+        // the FCT sums take them)
+#pragma unroll
+        for (int g = 0; g < TPL; g++) tc[g] = c_tr[cpos[W] * NT + t0 + g];
+      } else {
+        ld_group<TPL>(S.xtr, ot, tc);
         uint32_t op[W];
         if constexpr (GRP) {
           uint32_t om[MS];
@@ -881,7 +941,7 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
       } else if (scheme == MPAS_TRACER_FCT) {
         // calc_provisional_layer_thicknesses (ocean/solve.cc:74-96)
         const double dtInvAreaCell = S.dt / ar;
-        const double lt = ld(S.xh, ol);
+        const double lt = CST ? c_h[cpos[W]] : ld(S.xh, ol);
         double hp = lt;
 #pragma unroll
         for (int j = 0; j < W; j++) {
@@ -909,14 +969,15 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 #pragma unroll
           for (int j = 0; j < W; j++) {
             const bool live = j < n;
-            tmax = live ? smax(tmax, tn[j][g]) : tmax;
-            tmin = live ? smin(tmin, tn[j][g]) : tmin;
+            const double tnj = CST ? c_tr[cpos[j] * NT + t0 + g] : tn[j][g];
+            tmax = live ? smax(tmax, tnj) : tmax;
+            tmin = live ? smin(tmin, tnj) : tmin;
             const double signedFactor = sign_on_cell(neg, j) * invArea;
             const double F = xu[j] * he[j];
             const double ntf1 = smax(0.0, F), ntf2 = smin(0.0, F);
             const bool first = (neg >> j) & 1;
-            const double t1 = first ? tc[g] : tn[j][g];
-            const double t2 = first ? tn[j][g] : tc[g];
+            const double t1 = first ? tc[g] : tnj;
+            const double t2 = first ? tnj : tc[g];
             const double lo = dv[j] * (ntf1 * t1 + ntf2 * t2);
             const double xe = te + signedFactor * lo;
             const double xi = fin + smax(0.0, signedFactor * an[j][g]);
@@ -963,7 +1024,9 @@ __device__ __forceinline__ void stage_cells(const DMesh &M, const StageIO &S, in
 // and of the cells across their edges into LDS (`lds`: M.tStgMax records
 // of 2 NT factors, then M.tStgMax records of NT tracers), then every cell
 // reads its own and its neighbours' values there by position, where the
-// sums take them.  A cell's neighbours are mostly the block's own cells or
+// sums take them.  (Forming h_edge from staged thickness as well, which
+// the copy would add as one scattered load per cell, made the kernel
+// slower: 82.5 against 66.0 us per launch at level 8.)  A cell's neighbours are mostly the block's own cells or
 // shared with other cells of the block, so the copy reads each record once
 // where the gathers read it about four times.
 template <int NT, int W, int LOL = 0, int TPLX = 0, int FIX = 0, bool STG = false>

@@ -452,7 +452,9 @@ def test_cells_lds_staging_bitwise(monkeypatch, nT):
 @pytest.mark.parametrize("var,nT", [("MPAS_HIP_EDGES_LDS", 0), ("MPAS_HIP_EDGES_LDS", 4),
                                     ("MPAS_HIP_HO_LDS", 2), ("MPAS_HIP_HO_LDS", 4),
                                     ("MPAS_HIP_HO_LDS", 8), ("MPAS_HIP_TRACER_LDS", 2),
-                                    ("MPAS_HIP_TRACER_LDS", 4), ("MPAS_HIP_TRACER_LDS", 8)])
+                                    ("MPAS_HIP_TRACER_LDS", 4), ("MPAS_HIP_TRACER_LDS", 8),
+                                    ("MPAS_HIP_CELLS_CST", 2), ("MPAS_HIP_CELLS_CST", 4),
+                                    ("MPAS_HIP_CELLS_CST", 8)])
 def test_lds_staging_bitwise(monkeypatch, var, nT):
     """The one-level momentum, high-order flux and tracer-update kernels
     with the values their stencils gather staged in LDS per workgroup (the
@@ -461,7 +463,10 @@ def test_lds_staging_bitwise(monkeypatch, var, nT):
     workgroups (60 for the flux kernel at 8 tracers, two lanes per edge);
     the tracer update's 2562 cells make 21, 41 and 81 workgroups of 128, 64
     and 32 cells at 2, 4 and 8 tracers, the last one partly empty, with
-    del2 mixing on (its neighbours' tracers come from LDS too)."""
+    del2 mixing on (its neighbours' tracers come from LDS too); the cells
+    kernel (two tracers per lane) stages its cells' and their neighbours'
+    tracers and thickness in 11, 21 and 41 workgroups of 256, 128 and 64
+    cells and forms h_edge from the thickness."""
     from mpas_hip import TRACER_FCT, Context, sw_workload_state
     m = mesh(4)
     u, h, tr = sw_workload_state(m, nT, seed=22)
