@@ -324,6 +324,13 @@ __device__ __forceinline__ int xcd_item() {
   const int b = blockIdx.x;
   return ((b % kXcds) * per + b / kXcds) * blockDim.x + threadIdx.x;
 }
+// The same for block b of a range of nb blocks (nb a multiple of the XCD
+// count) within a launch whose earlier ranges also hold multiples of it:
+// block b of the range runs on XCD b % 8 as well.
+__device__ __forceinline__ int xcd_item_in(int b, int nb) {
+  const int per = nb / kXcds;
+  return ((b % kXcds) * per + b / kXcds) * blockDim.x + threadIdx.x;
+}
 inline int xcd_grid(int64_t n, int bs) {
   const int64_t blocks = (n + bs - 1) / bs;
   return int((blocks + kXcds - 1) / kXcds * kXcds);

@@ -145,6 +145,22 @@ __global__ __launch_bounds__(BS_HO) MPAS_WAVES(MPAS_WPE_HO) void k_stage_ho_lds(
   extern __shared__ double lds[];
   stage_ho<NT, WA, 0, tpl_of<NT>(MPAS_TPL_HO), 1, 1, true>(M, S, xcd_item(), lds);
 }
+// The staged momentum kernel (the first `eblocks` workgroups) and the
+// staged high-order flux kernel (the rest) in one launch: the two read
+// the same stage input and are independent.
+#ifndef MPAS_MERGE_EH
+#define MPAS_MERGE_EH 1
+#endif
+template <int WE, int NT>
+__global__ __launch_bounds__(BS_EDGES) void k_stage_edges_ho(DMesh M, StageIO S, int eblocks) {
+  extern __shared__ double lds[];
+  const int b = int(blockIdx.x);
+  if (b < eblocks)
+    stage_edges<WE, 0, 1, 0, MPAS_QCH, true>(M, S, xcd_item_in(b, eblocks), lds);
+  else
+    stage_ho<NT, WE, 0, tpl_of<NT>(MPAS_TPL_HO), 1, 1, true>(
+        M, S, xcd_item_in(b - eblocks, int(gridDim.x) - eblocks), lds);
+}
 // One-level FCT cells with the workgroup's edge values staged in LDS
 // (DMesh::c2eLoc); dynamic LDS of (5 + NT) * M.stgMax doubles.
 static_assert(BS_CELLS == kCellsBlock, "the LDS staging lists are built per kCellsBlock lanes");
@@ -181,6 +197,27 @@ __global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage
   stage_tracer<NT, W, 0, tpl_of<NT>(tpl_tracer(0)), 1, true>(M, S, xcd_item(), lds);
 }
 
+// The staged tracer update (the first `tblocks` workgroups) and the
+// vertex diagnostics of the stage's output (the rest) in one launch: the
+// two are independent, and the vertex workgroups fill the CUs the tracer
+// update's last workgroups leave idle.
+#ifndef MPAS_MERGE_TV
+#define MPAS_MERGE_TV 1
+#endif
+static_assert(BS_TRACER == BS_DIAG, "one workgroup size for both parts");
+template <int NT, int W>
+__global__ __launch_bounds__(BS_TRACER) MPAS_WAVES(MPAS_WPE_TRACER) void k_stage_tracer_vertex(
+    DMesh M, StageIO S, int tblocks, double *vorticity, double *pv_vertex) {
+  extern __shared__ double lds[];
+  const int b = int(blockIdx.x);
+  if (b < tblocks) {
+    stage_tracer<NT, W, 0, tpl_of<NT>(tpl_tracer(0)), 1, true>(M, S, xcd_item_in(b, tblocks),
+                                                                lds);
+  } else {
+    const int v = xcd_item_in(b - tblocks, int(gridDim.x) - tblocks);
+    if (v < M.nV) diag_vertex<1>(M, v, S.yh, S.yu, nullptr, nullptr, vorticity, pv_vertex);
+  }
+}
 template <int LOL, int LP = 1>
 __global__ __launch_bounds__(BS_DIAG) void k_diag_vertex(DMesh M, StageIO S,
                                                          const double *__restrict__ h,
@@ -321,7 +358,22 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   do {                  \
     if (int rc = (expr)) return rc; \
   } while (0)
-  if (!S.ocean) {
+  bool merged_eh = false;  // momentum and high-order flux in one launch
+  if constexpr (LOL == 0 && WE <= 12 && NT > 0) {
+    if (MPAS_MERGE_EH && !S.ocean && fct && M.e2eLoc && M.a2cLoc && BS_EDGES == BS_HO) {
+      MPAS_STEP(prof_begin(ctx, "k_stage_edges_ho", b.edges + b.ho));
+      const int eg = xcd_grid(M.nE, BS_EDGES);
+      const int hg = xcd_grid(M.nE * lanes_per_cell<NT, MPAS_TPL_HO>(), BS_HO);
+      const size_t le = size_t(3) * M.eStgMax * sizeof(double);
+      const size_t lh = size_t(M.aStgMax) * NT * sizeof(double);
+      hipLaunchKernelGGL((k_stage_edges_ho<WE, NT>), dim3(eg + hg), dim3(BS_EDGES),
+                         le > lh ? le : lh, st, M, S, eg);
+      MPAS_STEP(after_launch("k_stage_edges_ho"));
+      MPAS_STEP(prof_end(ctx));
+      merged_eh = true;
+    }
+  }
+  if (!S.ocean && !merged_eh) {
     MPAS_STEP(prof_begin(ctx, "k_stage_edges", b.edges));
     if (pairs) {
       hipLaunchKernelGGL((k_stage_edges<WE, LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_EDGES)),
@@ -343,7 +395,7 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
     MPAS_STEP(prof_end(ctx));
   }
   if constexpr (NT > 0) {
-    if (fct) {
+    if (fct && !merged_eh) {
       MPAS_STEP(prof_begin(ctx, "k_stage_ho", b.ho));
       if (pairs)
         hipLaunchKernelGGL((k_stage_ho<NT, WE, LOL, 2>),
@@ -402,9 +454,9 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
   }
   MPAS_STEP(after_launch("k_stage_cells"));
   MPAS_STEP(prof_end(ctx));
+  bool merged = false;  // the vertex diagnostics went with the tracer update
   if constexpr (NT > 0) {
     if (fct) {
-      MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
       const dim3 grid(xcd_grid(M.nC * lv * lanes_per_cell<NT, tpl_tracer(LOL)>(), BS_TRACER));
       const size_t occ =
           occupancy_lds(LOL ? MPAS_WG_PER_CU_TRACER_LANES : MPAS_WG_PER_CU_TRACER);
@@ -413,27 +465,41 @@ int launch_stage_t(mpas_ctx *ctx, hipStream_t st, const DMesh &M, const StageIO 
         if (M.c2cLoc) {
           const size_t bytes = size_t(3) * NT * M.tStgMax * sizeof(double);
           const size_t occ_s = occupancy_lds(MPAS_WG_PER_CU_TRACER_LDS);
-          hipLaunchKernelGGL((k_stage_tracer_lds<NT, W>), grid, dim3(BS_TRACER),
-                             bytes > occ_s ? bytes : occ_s, st, M, S);
+          const size_t lds = bytes > occ_s ? bytes : occ_s;
+          if (MPAS_MERGE_TV && !S.ocean) {
+            MPAS_STEP(prof_begin(ctx, "k_stage_tracer_vertex", b.tracer + b.vertex));
+            const int vgrid = xcd_grid(M.nV, BS_DIAG);
+            hipLaunchKernelGGL((k_stage_tracer_vertex<NT, W>), dim3(grid.x + vgrid),
+                               dim3(BS_TRACER), lds, st, M, S, int(grid.x), vort_y, pv_vertex);
+            merged = true;
+          } else {
+            MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
+            hipLaunchKernelGGL((k_stage_tracer_lds<NT, W>), grid, dim3(BS_TRACER), lds, st, M, S);
+          }
           staged = true;
         }
       }
-      if (!staged)
+      if (!staged) {
+        MPAS_STEP(prof_begin(ctx, "k_stage_tracer", b.tracer));
         hipLaunchKernelGGL((k_stage_tracer<NT, W, LOL>), grid, dim3(BS_TRACER), occ, st, M, S);
+      }
       MPAS_STEP(after_launch("k_stage_tracer"));
       MPAS_STEP(prof_end(ctx));
     }
   }
   if (S.ocean) return 0;  // u and h do not change: no diagnostics
-  MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
-  if (pairs)
+  if (!merged) MPAS_STEP(prof_begin(ctx, "k_diag_vertex", b.vertex));
+  if (merged) {
+  } else if (pairs)
     hipLaunchKernelGGL((k_diag_vertex<LOL, 2>), dim3(xcd_grid(M.nV * (lv / 2), BS_DIAG)),
                        dim3(BS_DIAG), 0, st, M, S2, S.yh, S.yu, vort_y, pv_vertex);
   else
     hipLaunchKernelGGL((k_diag_vertex<LOL>), dim3(xcd_grid(M.nV * lv, BS_DIAG)), dim3(BS_DIAG),
                        0, st, M, S, S.yh, S.yu, vort_y, pv_vertex);
-  MPAS_STEP(after_launch("k_diag_vertex"));
-  MPAS_STEP(prof_end(ctx));
+  if (!merged) {
+    MPAS_STEP(after_launch("k_diag_vertex"));
+    MPAS_STEP(prof_end(ctx));
+  }
   MPAS_STEP(prof_begin(ctx, "k_diag_edge", b.edge_diag));
   if (pairs)
     hipLaunchKernelGGL((k_diag_edge<LOL, 2>), dim3(xcd_grid(M.nE * (lv / 2), BS_DIAG)),
