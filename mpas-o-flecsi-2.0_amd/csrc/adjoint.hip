@@ -28,6 +28,7 @@
 // bit-identical to the first pass), then sweeps the segment in reverse.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -106,6 +107,10 @@ struct AdjState {
   // or in turn; each segment's reverse sweep replayed from a hipGraph
   // (captured per tape, parity and length) or launched directly
   int ad_block = 256, tl_block = 256, overlap = 1, graph = 0;
+  // the stage-0 kernels of each step but a segment's last in the sweep run
+  // the next step's prep (k_ad_prep otherwise: MPAS_HIP_ADJ_FUSE_PREP=0
+  // when the context is made); the same bits either way
+  int fuse_prep = 1;
   std::map<std::tuple<const void *, const void *, int, int>, hipGraphExec_t> sweeps;
 };
 
@@ -398,13 +403,47 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_tl_edge_b(DMesh M, AdjParams P
 // the final division T = Q / h reversed (aw_h = h adjoint after it, aw_tr =
 // Q adjoint), the adjoint of the step's initial state seeded with the
 // direct paths, and the stage-3 tendency adjoints.
+// The prep of one edge level (o) from ub, the u adjoint of the step's
+// final state, and of one cell level (ol) from hb and trb[t], its h and
+// tracer adjoints; w_rk is the stage-3 weight.  k_ad_prep loads these
+// values, or the last kernels of the following step's sweep (k_ad_cell and
+// k_ad_edge_b at stage 0, which form them) call the same code.
+struct PrepOut {
+  double *ub_out, *hb_out, *trb_out, *aw_u, *aw_h, *aw_tr, *tb_u, *tb_h, *tb_tr;
+};
+__device__ __forceinline__ void prep_edge(uint32_t o, double ub, bool freeze_u, double w_rk,
+                                          const PrepOut &Q) {
+  st(Q.aw_u, o, ub);
+  st(Q.ub_out, o, ub);
+  st(Q.tb_u, o, freeze_u ? 0.0 : w_rk * ub);
+}
+__device__ __forceinline__ void prep_cell(uint32_t ol, uint32_t NT, double hb,
+                                          const double (&trb)[8], const Rec &Rf, const Rec &R0,
+                                          double w_rk, const PrepOut &Q) {
+  const double hf = ld(Rf.h, ol), h0 = ld(R0.h, ol);
+  double htot = hb;
+  for (uint32_t t = 0; t < NT; t++) {
+    const uint32_t ot = ol * NT + t;
+    htot -= trb[t] * ld(Rf.tr, ot) / hf;
+  }
+  double xh = htot;
+  for (uint32_t t = 0; t < NT; t++) {
+    const uint32_t ot = ol * NT + t;
+    const double qb = trb[t] / hf;
+    st(Q.aw_tr, ot, qb);
+    xh += qb * ld(R0.tr, ot);
+    st(Q.trb_out, ot, qb * h0);
+    st(Q.tb_tr, ot, w_rk * qb);
+  }
+  st(Q.aw_h, ol, htot);
+  st(Q.hb_out, ol, xh);
+  st(Q.tb_h, ol, w_rk * htot);
+}
+
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_prep(DMesh M, AdjParams P, Rec Rf, Rec R0,
                                                 const double *__restrict__ ub_in,
                                                 const double *__restrict__ hb_in,
-                                                const double *__restrict__ trb_in,
-                                                double *ub_out, double *hb_out, double *trb_out,
-                                                double *aw_u, double *aw_h, double *aw_tr,
-                                                double *tb_u, double *tb_h, double *tb_tr) {
+                                                const double *__restrict__ trb_in, PrepOut Q) {
   uint32_t k0, k1;
   int lane_;
   const int i = level_item(M.nL, k0, k1, lane_);
@@ -412,36 +451,28 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_prep(DMesh M, AdjParams P, 
   if (i < M.nE) {
     for (uint32_t k = k0; k < k1; k++) {
       const uint32_t o = i * L + k;
-      const double ub = ld(ub_in, o);
-      st(aw_u, o, ub);
-      st(ub_out, o, ub);
-      st(tb_u, o, P.freeze_u ? 0.0 : P.w_rk * ub);
+      prep_edge(o, ld(ub_in, o), P.freeze_u, P.w_rk, Q);
     }
   }
   if (i < M.nC) {
     for (uint32_t k = k0; k < k1; k++) {
       const uint32_t ol = i * L + k;
-      const double hf = ld(Rf.h, ol), h0 = ld(R0.h, ol);
-      double htot = ld(hb_in, ol);
-      for (uint32_t t = 0; t < NT; t++) {
-        const uint32_t ot = ol * NT + t;
-        htot -= ld(trb_in, ot) * ld(Rf.tr, ot) / hf;
-      }
-      double xh = htot;
-      for (uint32_t t = 0; t < NT; t++) {
-        const uint32_t ot = ol * NT + t;
-        const double qb = ld(trb_in, ot) / hf;
-        st(aw_tr, ot, qb);
-        xh += qb * ld(R0.tr, ot);
-        st(trb_out, ot, qb * h0);
-        st(tb_tr, ot, P.w_rk * qb);
-      }
-      st(aw_h, ol, htot);
-      st(hb_out, ol, xh);
-      st(tb_h, ol, P.w_rk * htot);
+      double trb[8];
+      for (uint32_t t = 0; t < NT; t++) trb[t] = ld(trb_in, ol * NT + t);
+      prep_cell(ol, NT, ld(hb_in, ol), trb, Rf, R0, P.w_rk, Q);
     }
   }
 }
+
+// The next step's prep, fused into the stage-0 kernels of a step's sweep:
+// on when `on`, with the records of the next step's start (R0n; its final
+// state is this step's start) and the stage-3 weight.
+struct PrepNext {
+  int on;
+  Rec R0n;
+  double w_rk;
+  PrepOut Q;
+};
 
 // Ragged lists are read in chunks of kChunk slots: the slot indices of a
 // chunk are clamped to the row (the tail repeats the last slot, masked out
@@ -676,8 +707,8 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     const double *__restrict__ heb, const double *__restrict__ hvT,
     const double *__restrict__ r1, const double *__restrict__ r2,
     const double *__restrict__ hbf, double *keT, double *divT,
-    double *xb_h, double *xb_tr, const double *__restrict__ aw_h,
-    const double *__restrict__ aw_tr, double *tb_h, double *tb_tr) {
+    double *xb_h, double *xb_tr, const double *aw_h, const double *aw_tr, double *tb_h,
+    double *tb_tr, PrepNext N) {
   static_assert(PL == 1 || PL == kCellLanes, "one lane or kCellLanes lanes per cell and level");
   constexpr int VPL = kCellLanes / PL;  // partials per lane
   uint32_t k0, k1;
@@ -752,11 +783,17 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
       st(xb_h, ol, xh);
       st(tb_h, ol, P.w_rk_m * ld(aw_h, ol) + P.w_sub_m * htot);
     } else {
-      st(xb_h, ol, ld(xb_h, ol) + hb);
+      // the adjoint of the step's initial state is complete here
+      const double xh = ld(xb_h, ol) + hb;
+      st(xb_h, ol, xh);
+      double xt[8];
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t ot = ol * NT + t;
-        st(xb_tr, ot, ld(xb_tr, ot) + Tb[t]);
+        xt[t] = ld(xb_tr, ot) + Tb[t];
+        st(xb_tr, ot, xt[t]);
       }
+      // the next step's prep (its final state is this step's start, R0)
+      if (N.on) prep_cell(ol, NT, xh, xt, R0, N.R0n, N.w_rk, N.Q);
     }
   }
 }
@@ -768,7 +805,8 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_b(DMesh M, AdjParams P
                                                   const double *__restrict__ keT,
                                                   const double *__restrict__ divT,
                                                   const double *__restrict__ vT, double *xb_u,
-                                                  const double *__restrict__ aw_u, double *tb_u) {
+                                                  const double *aw_u, double *tb_u,
+                                                  PrepNext N) {
   uint32_t k0, k1;
   int lane_;
   const int e = level_item(M.nL, k0, k1, lane_);
@@ -783,8 +821,10 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_b(DMesh M, AdjParams P
                 0.5 * dc * dv * ld(R.u, o) * (ld(keT, c1 * L + k) + ld(keT, c2 * L + k));
     if (P.use_visc) ub += dv * (ld(divT, c1 * L + k) - ld(divT, c2 * L + k));
     ub += dc * (ld(vT, uint32_t(vv.y) * L + k) - ld(vT, uint32_t(vv.x) * L + k));
-    st(xb_u, o, ld(xb_u, o) + ub);
+    const double xu = ld(xb_u, o) + ub;
+    st(xb_u, o, xu);
     if (P.s >= 1) st(tb_u, o, P.freeze_u ? 0.0 : P.w_rk_m * ld(aw_u, o) + P.w_sub_m * ub);
+    else if (N.on) prep_edge(o, xu, P.freeze_u, N.w_rk, N.Q);  // the next step's prep
   }
 }
 
@@ -902,6 +942,8 @@ int alloc_rec(AdjState &A, const DMesh &M, Rec &r) {
 int adj_alloc(mpas_ctx *ctx, int K) {
   const DMesh &M = ctx->dm;
   auto *A = new AdjState();
+  const char *fp = std::getenv("MPAS_HIP_ADJ_FUSE_PREP");
+  A->fuse_prep = !(fp && fp[0] == '0');
   ctx->adj = A;
   A->K = K;
   const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
@@ -1111,15 +1153,29 @@ int tlm_step(mpas_ctx *ctx, AdjState &A, double dt) {
 
 // Adjoint of step i of the tape (records 4i .. 4i+4): the adjoint of its
 // final state in xb[src] -> the adjoint of its initial state in xb[1-src].
-int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
+// prep: the step starts with k_ad_prep (else the previous step in the
+// sweep, step i + 1, ran its prep); fuse_next: the stage-0 kernels run
+// the prep of step i - 1.
+int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src, bool prep = true,
+             bool fuse_next = false) {
   const DMesh &M = ctx->dm;
   const int dst = 1 - src;
   const Rec &R0 = A.rec[4 * i], &Rf = A.rec[4 * i + 4];
-  {
+  if (prep) {
     const AdjParams P = adj_params(ctx, 3, dt);
     ADJ_LAUNCH(k_ad_prep, std::max(M.nE, M.nC), M, P, Rf, R0, A.xb_u[src], A.xb_h[src],
-               A.xb_tr[src], A.xb_u[dst], A.xb_h[dst], A.xb_tr[dst], A.aw_u, A.aw_h, A.aw_tr,
-               A.tb_u, A.tb_h, A.tb_tr);
+               A.xb_tr[src],
+               PrepOut{A.xb_u[dst], A.xb_h[dst], A.xb_tr[dst], A.aw_u, A.aw_h, A.aw_tr, A.tb_u,
+                       A.tb_h, A.tb_tr});
+  }
+  // step i - 1 sweeps from xb[dst] into xb[src]
+  PrepNext N{};
+  if (fuse_next && i > 0) {
+    N.on = 1;
+    N.R0n = A.rec[4 * (i - 1)];
+    N.w_rk = adj_params(ctx, 3, dt).w_rk;
+    N.Q = PrepOut{A.xb_u[src], A.xb_h[src], A.xb_tr[src], A.aw_u, A.aw_h, A.aw_tr, A.tb_u,
+                  A.tb_h, A.tb_tr};
   }
   for (int s = 3; s >= 0; s--) {
     const AdjParams P = adj_params(ctx, s, dt);
@@ -1133,15 +1189,17 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
       ADJ_LAUNCH(k_ad_edge_a<kEdgeLanes>, int64_t(M.nE) * kEdgeLanes, M, A.T, P, R, A.tb_u,
                  A.tb_h, A.tb_tr, A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
     ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
+    const PrepNext Ns = s == 0 ? N : PrepNext{};
     if (M.nL >= kOneLaneLevels)
       ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT, A.r1, A.r2, A.hbf,
-                 A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
+                 A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr,
+                 Ns);
     else
       ADJ_LAUNCH(k_ad_cell<kCellLanes>, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, A.tb_u,
                  A.heb, A.hvT, A.r1, A.r2, A.hbf, A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst],
-                 A.aw_h, A.aw_tr, A.tb_h, A.tb_tr);
+                 A.aw_h, A.aw_tr, A.tb_h, A.tb_tr, Ns);
     ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
-               A.tb_u);
+               A.tb_u, Ns);
   }
   return 0;
 }
@@ -1150,10 +1208,13 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src) {
 // xb[src]: launched directly, or replayed from a hipGraph captured the first
 // time for this tape, parity, length and time step.
 int sweep_segment(mpas_ctx *ctx, AdjState &A, int len, int src) {
+  // each step but the segment's last in the sweep runs the next step's
+  // prep in its stage-0 kernels
   auto direct = [&]() -> int {
     int x = src;
     for (int i = len - 1; i >= 0; i--) {
-      if (int rc = adj_step(ctx, A, i, A.dt, x)) return rc;
+      if (int rc = adj_step(ctx, A, i, A.dt, x, i == len - 1 || !A.fuse_prep, A.fuse_prep))
+        return rc;
       x = 1 - x;
     }
     return 0;
