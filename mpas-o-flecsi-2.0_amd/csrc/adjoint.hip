@@ -111,6 +111,11 @@ struct AdjState {
   // the next step's prep (k_ad_prep otherwise: MPAS_HIP_ADJ_FUSE_PREP=0
   // when the context is made); the same bits either way
   int fuse_prep = 1;
+  // 0: sweeps are never replayed from graphs (MPAS_HIP_ADJ_GRAPH=0 when the
+  // context is made; mpas_adj_config's graph is then ignored and the
+  // autotuner does not try it), for tools that cannot follow graph capture
+  // (rocprofv3's kernel tracing stops the process in it)
+  int graph_ok = 1;
   std::map<std::tuple<const void *, const void *, int, int>, hipGraphExec_t> sweeps;
 };
 
@@ -944,6 +949,8 @@ int adj_alloc(mpas_ctx *ctx, int K) {
   auto *A = new AdjState();
   const char *fp = std::getenv("MPAS_HIP_ADJ_FUSE_PREP");
   A->fuse_prep = !(fp && fp[0] == '0');
+  const char *gp = std::getenv("MPAS_HIP_ADJ_GRAPH");
+  A->graph_ok = !(gp && gp[0] == '0');
   ctx->adj = A;
   A->K = K;
   const size_t E = size_t(M.nE) * M.nL, C = size_t(M.nC) * M.nL, V = size_t(M.nV) * M.nL;
@@ -1219,7 +1226,7 @@ int sweep_segment(mpas_ctx *ctx, AdjState &A, int len, int src) {
     }
     return 0;
   };
-  if (!A.graph || ctx->prof.on) return direct();
+  if (!A.graph || !A.graph_ok || ctx->prof.on) return direct();
   // the tape's buffers: only records 0 and 4 ever trade places (the TLM's
   // step), so these two name the whole set
   const auto key = std::make_tuple(static_cast<const void *>(A.rec[0].u),
@@ -1478,7 +1485,7 @@ int mpas_adj_config(mpas_ctx *ctx, int ad_block, int tl_block, int overlap, int 
   A.ad_block = ad_block;
   A.tl_block = tl_block;
   A.overlap = overlap != 0;
-  A.graph = graph != 0;
+  A.graph = graph != 0 && A.graph_ok;
   return 0;
 }
 
@@ -1557,7 +1564,7 @@ int mpas_adj_autotune(mpas_ctx *ctx, double dt, int nsteps, char *report, size_t
   note("start", best);
   // the schedule: replays beside the sweep or in turn, sweeps from graphs or not
   for (int ov = 0; ov < 2; ov++)
-    for (int gr = 0; gr < 2; gr++) {
+    for (int gr = 0; gr < 1 + A.graph_ok; gr++) {
       if (ov == A.overlap && gr == A.graph) continue;
       const int po = A.overlap, pg = A.graph;
       A.overlap = ov;
