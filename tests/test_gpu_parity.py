@@ -180,6 +180,10 @@ FUSED_CASES = {
     # the most tracers the fused step takes (two lanes of four per edge in
     # the high-order flux)
     "sw_fct8": (3, 1, 8, 2, 1e5, 1e3, False, 600.0, 3),
+    # odd tracer counts: the momentum and flux phases in one launch, the
+    # tracer update (gathers) in a launch of its own
+    "sw_fct1": (3, 1, 1, 2, 1e5, 1e3, False, 600.0, 4),
+    "sw_fct3": (4, 1, 3, 2, 1e5, 1e3, False, 480.0, 3),
     "sw_centred2": (3, 1, 2, 1, 0.0, 1e3, False, 600.0, 4),
     # four centred tracers: the cells kernel's two tracer-pair lanes share
     # the cell's edge rows
