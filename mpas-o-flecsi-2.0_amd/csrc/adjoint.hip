@@ -63,6 +63,7 @@ struct AdjMesh {
   const uint8_t *nVc;
   const int *vc;
   const double *vk;
+  int wVc, wAt;  // padded widths of the vc and at rows
 };
 
 struct AdjState {
@@ -800,6 +801,11 @@ constexpr int kCellLanes = 4;
 // edges of a cell (at most octagons), and per partial
 constexpr int kMaxCellSlots = 8;
 constexpr int kCellSlots = kMaxCellSlots / kCellLanes;
+// advection pairs per partial read in one straight-line chunk (a cell of a
+// hexagon mesh sits in about 30 edges' stencils), and the tracers whose
+// own values the finishing lane has loaded with the gathers
+constexpr int kAtSlots = 8;
+constexpr int kHoistT = 2;
 __device__ __forceinline__ double cell_lanes_sum(double x) {
   x += __shfl_xor(x, 1);
   x += __shfl_xor(x, 2);
@@ -829,7 +835,10 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
   const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
   const uint8_t neg = ld(M.sgnNeg, c);
   const double ar = ld(M.area, c);
-  constexpr int Q = kChunk / kCellLanes;  // entries per partial and round
+  constexpr int Q = kChunk / kCellLanes;  // entries per partial and round of the tail
+  // row slots clamped into the padded rows (uniform widths, so the id
+  // loads need not wait for the row counts; the padding holds valid ids)
+  const int wc = M.narrow ? 6 : 8;
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t ol = c * L + k;
     double g1v[VPL], hsumv[VPL], Tbv[VPL][8];
@@ -837,19 +846,23 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     for (int m = 0; m < VPL; m++) {
       const int vl = lane + m * PL;
       double g1 = 0.0, hsum = 0.0, Tb[8] = {};
-      // the first kCellSlots edges and vertices of the partial's share read
-      // in straight-line code (ids clamped into the row: the ELL padding
-      // holds valid ids), the rest of longer rows after
-      uint32_t ej[kCellSlots], vj[kCellSlots];
+      // the first kCellSlots edges and vertices and kAtSlots advection
+      // pairs of the partial's share read in straight-line code, the rest
+      // of longer rows after
+      uint32_t ej[kCellSlots], vj[kCellSlots], aj[kAtSlots];
       double kv[kCellSlots];
 #pragma unroll
       for (int q = 0; q < kCellSlots; q++) {
         const int j = vl + q * kCellLanes;
-        ej[q] = ld(M.c2e, max(min(j, n - 1), 0) * C + c);
-        const int i = max(min(j, nv - 1), 0);
+        ej[q] = ld(M.c2e, min(j, wc - 1) * C + c);
+        const int i = min(j, A.wVc - 1);
         vj[q] = ld(A.vc, i * C + c);
         kv[q] = ld(A.vk, i * C + c);
       }
+      if (P.fct)
+#pragma unroll
+        for (int q = 0; q < kAtSlots; q++)
+          aj[q] = uint32_t(ld(A.at, min(vl + q * kCellLanes, A.wAt - 1) * C + c)) * NT;
 #pragma unroll
       for (int q = 0; q < kCellSlots; q++) {
         const int j = vl + q * kCellLanes;
@@ -867,21 +880,51 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
         hsum += vl + q * kCellLanes < nv ? ld(hvT, vj[q] * L + k) * kv[q] : 0.0;
       for (int i = vl + kCellSlots * kCellLanes; i < nv; i += kCellLanes)
         hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
-      for (int i0 = vl; i0 < na; i0 += kChunk) {
-        uint32_t os[Q];
-#pragma unroll
-        for (int j = 0; j < Q; j++)
-          os[j] = uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * NT;
+      if (P.fct) {
         for (uint32_t t = 0; t < NT; t++) {
 #pragma unroll
+          for (int q = 0; q < kAtSlots; q++)
+            Tb[t] += (vl + q * kCellLanes < na) ? ld(hbf, (aj[q] + t) * L + k) : 0.0;
+        }
+        for (int i0 = vl + kAtSlots * kCellLanes; i0 < na; i0 += kChunk) {
+          uint32_t os[Q];
+#pragma unroll
           for (int j = 0; j < Q; j++)
-            Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, (os[j] + t) * L + k) : 0.0;
+            os[j] = uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * NT;
+          for (uint32_t t = 0; t < NT; t++) {
+#pragma unroll
+            for (int j = 0; j < Q; j++)
+              Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, (os[j] + t) * L + k) : 0.0;
+          }
         }
       }
       g1v[m] = g1;
       hsumv[m] = hsum;
 #pragma unroll
       for (int t = 0; t < 8; t++) Tbv[m][t] = Tb[t];
+    }
+    // the finishing lane's own values (first kHoistT tracers), loaded by
+    // every lane of the cell before the exchange: behind the branch on the
+    // lane they would wait for the sums
+    const bool late = P.s >= 1;
+    const double xbh = ld(xb_h, ol);
+    double hs = 0.0, h0 = 0.0, awh = 0.0;
+    if (late) {
+      hs = ld(R.h, ol);
+      h0 = ld(R0.h, ol);
+      awh = ld(aw_h, ol);
+    }
+    double xbt[kHoistT] = {}, trs[kHoistT] = {}, tr0[kHoistT] = {}, awt[kHoistT] = {};
+#pragma unroll
+    for (int t = 0; t < kHoistT; t++) {
+      if (uint32_t(t) >= NT) break;  // uniform
+      const uint32_t ot = ol * NT + t;
+      xbt[t] = ld(xb_tr, ot);
+      if (late) {
+        trs[t] = ld(R.tr, ot);
+        tr0[t] = ld(R0.tr, ot);
+        awt[t] = ld(aw_tr, ot);
+      }
     }
     const double g1 = cell_lanes_sum<PL>(g1v), hsum = cell_lanes_sum<PL>(hsumv);
     double Tb[8];
@@ -898,28 +941,29 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
     const double hb = -M.gravity * g1 + hsum;
     st(keT, ol, -g1 / ar);
     if (P.use_visc) st(divT, ol, P.visc_coef * g1 / ar);
-    if (P.s >= 1) {
-      const double hs = ld(R.h, ol), h0 = ld(R0.h, ol);
+    if (late) {
       double htot = hb;
-      for (uint32_t t = 0; t < NT; t++) htot -= Tb[t] * ld(R.tr, ol * NT + t) / hs;
-      double xh = ld(xb_h, ol) + htot;
+      for (uint32_t t = 0; t < NT; t++)
+        htot -= Tb[t] * (t < kHoistT ? trs[t] : ld(R.tr, ol * NT + t)) / hs;
+      double xh = xbh + htot;
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t ot = ol * NT + t;
+        const bool hx = t < kHoistT;
         const double pb = Tb[t] / hs;
-        xh += pb * ld(R0.tr, ot);
-        st(xb_tr, ot, ld(xb_tr, ot) + pb * h0);
-        st(tb_tr, ot, P.w_rk_m * ld(aw_tr, ot) + P.w_sub_m * pb);
+        xh += pb * (hx ? tr0[t] : ld(R0.tr, ot));
+        st(xb_tr, ot, (hx ? xbt[t] : ld(xb_tr, ot)) + pb * h0);
+        st(tb_tr, ot, P.w_rk_m * (hx ? awt[t] : ld(aw_tr, ot)) + P.w_sub_m * pb);
       }
       st(xb_h, ol, xh);
-      st(tb_h, ol, P.w_rk_m * ld(aw_h, ol) + P.w_sub_m * htot);
+      st(tb_h, ol, P.w_rk_m * awh + P.w_sub_m * htot);
     } else {
       // the adjoint of the step's initial state is complete here
-      const double xh = ld(xb_h, ol) + hb;
+      const double xh = xbh + hb;
       st(xb_h, ol, xh);
       double xt[8];
       for (uint32_t t = 0; t < NT; t++) {
         const uint32_t ot = ol * NT + t;
-        xt[t] = ld(xb_tr, ot) + Tb[t];
+        xt[t] = (t < kHoistT ? xbt[t] : ld(xb_tr, ot)) + Tb[t];
         st(xb_tr, ot, xt[t]);
       }
       // the next step's prep (its final state is this step's start, R0)
@@ -1079,8 +1123,11 @@ int build_transposes(mpas_ctx *ctx, AdjState &A) {
                 !(pe && pe[0] == '0');
   if (A.edge_fast && (rc = upload_vec(A, rslot, &T.rslot))) return rc;
   if ((rc = upload_ragged(A, vC, false, &w, &T.nVc, &T.vc, &T.vk, nullptr))) return rc;
+  T.wVc = w;
   const double *unused = nullptr;
-  if (m.fct && (rc = upload_ragged(A, aT, false, &w, &T.nAt, &T.at, &unused, nullptr))) return rc;
+  T.wAt = 1;
+  if (m.fct && (rc = upload_ragged(A, aT, false, &T.wAt, &T.nAt, &T.at, &unused, nullptr)))
+    return rc;
   return 0;
 }
 

@@ -245,6 +245,49 @@ def test_fused_prep_keeps_the_gradient(monkeypatch, nL):
         assert bitwise_equal(a, b)
 
 
+@pytest.mark.parametrize("nL", [1, 8])
+def test_paired_stencils_keep_the_gradient(monkeypatch, nL):
+    """The edge part reading the slot-paired stencils, with the transposed
+    edgesOnEdge weights found through each slot's reverse slot (the default
+    on meshes of at most hexagons), against the slot-major rows and the
+    transposed lists (MPAS_HIP_ADJ_PAIRED=0, read at mpas_adj_init): the
+    sums run in another order, so the gradients agree to rounding
+    (1e-12 relative to the largest component) and both pass the dot-product
+    test, with one level and with levels on lanes."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(4)
+    u, h, tr = sw_workload_state(m, 2, seed=7)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MPAS_HIP_ADJ_PAIRED", flag)
+        ctx = Context(m, nLevels=nL, nTracers=2, tracer_scheme=TRACER_FCT, visc2=1e5,
+                      kappa2=1e3)
+        rep = lambda a: np.repeat(a.reshape(a.shape[0], 1, -1), nL, axis=1).ravel()
+        ctx.sw_upload(rep(h.reshape(-1, 1)), rep(u.reshape(-1, 1)), rep(tr.reshape(m.nCells, -1)),
+                      np.full(m.nCells, 4500.0))
+        ctx.adj_init(3)
+        x0 = ctx.state_arrays()
+        x0[0].upload(rep(u.reshape(-1, 1)))
+        x0[1].upload(rep(h.reshape(-1, 1)))
+        x0[2].upload(rep(tr.reshape(m.nCells, -1)))
+        rng = np.random.default_rng(41)
+        y_h, y = _rand_state(ctx, rng)
+        dx_h, dx = _rand_state(ctx, rng)
+        xb, dxN = ctx.state_arrays(), ctx.state_arrays()
+        ctx.adj_forward(600.0, 7, x0)  # segments of 3, 3 and 1 steps
+        ctx.adj_backward(y, xb)
+        ctx.tlm_forward(600.0, 7, x0, dx, dx1=dxN)
+        ctx.sync()
+        g = _get(ctx, xb)
+        lhs, rhs = _dot(_get(ctx, dxN), y_h), _dot(g, dx_h)
+        assert abs(lhs - rhs) <= DOT_TOL * abs(lhs), (flag, lhs, rhs)
+        out.append(g)
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        scale = np.max(np.abs(a))
+        assert np.max(np.abs(a - b)) <= 1e-12 * scale
+
+
 def test_adjoint_error_paths():
     from mpas_hip import TRACER_CENTRED, Context
     from mpas_hip._lib import MpasError
