@@ -123,6 +123,11 @@ struct AdjState {
   // the edge part reads the slot-paired stencils (k_ad_edge_a's FAST form;
   // MPAS_HIP_ADJ_PAIRED=0 when the context is made keeps the slot-major rows)
   int edge_fast = 0;
+  // with fewer than kOneLaneLevels levels the vertex part runs in the cell
+  // part's launch, whose cells form their vertices' h_vertex adjoints
+  // themselves (MPAS_HIP_ADJ_MERGE_VERTEX=0 when the context is made keeps
+  // two launches; the same bits either way)
+  int merge_vertex = 1;
   std::map<std::tuple<const void *, const void *, int, int>, hipGraphExec_t> sweeps;
 };
 
@@ -166,14 +171,18 @@ __device__ __forceinline__ double fsign(double F) { return (F < 0.0) ? -1.0 : do
 // adjacent lanes (with one level, the entity is the item).  groups > 1
 // keeps `groups` adjacent lanes per (entity, level), lane = their index.
 template <int groups = 1>
-__device__ __forceinline__ int level_item(uint32_t L, uint32_t &k0, uint32_t &k1, int &lane) {
-  const int item = xcd_item();
+__device__ __forceinline__ int level_item_at(int item, uint32_t L, uint32_t &k0, uint32_t &k1,
+                                             int &lane) {
   const int q = item / groups;
   lane = item - q * groups;
   const int ent = q / int(L);
   k0 = uint32_t(q - ent * int(L));
   k1 = k0 + 1;
   return ent;
+}
+template <int groups = 1>
+__device__ __forceinline__ int level_item(uint32_t L, uint32_t &k0, uint32_t &k1, int &lane) {
+  return level_item_at<groups>(xcd_item(), L, k0, k1, lane);
 }
 
 // Slope of the rescaled antidiffusive flux in the flux x with the limiter
@@ -756,39 +765,64 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
 
 // Vertex part: pv_vertex = (f + vorticity) / h_vertex and the viscosity's
 // vorticity term reversed; vT and hvT are the vorticity and h_vertex
-// adjoints divided by areaTriangle.
+// adjoints divided by areaTriangle.  vertex_pvvb forms the pv_vertex
+// adjoint and the recorded h_vertex and pv_vertex of one vertex level; the
+// cell part, merged with the vertex part into one launch, forms hvT of its
+// vertices itself with the same code (the same bits).
+struct VertexAdj {
+  int x[3];
+  double pvvb, hv, pvv, at;
+};
+__device__ __forceinline__ VertexAdj vertex_pvvb(const DMesh &M, const Rec &R,
+                                                 const double *__restrict__ pveb, int v,
+                                                 uint32_t k) {
+  const uint32_t L = M.nL, V = M.nV;
+  VertexAdj a;
+  uint32_t c[3];
+  double kt[3];
+  for (int i = 0; i < 3; i++) {
+    a.x[i] = ld(M.v2e, i * V + v);
+    c[i] = ld(M.v2c, i * V + v);
+    kt[i] = ld(M.kite, i * V + v);
+  }
+  a.at = ld(M.areaTri, v);
+  a.pvvb = 0.0;
+  for (int i = 0; i < 3; i++) a.pvvb += 0.5 * ld(pveb, v2e_id(a.x[i]) * L + k);
+  vertex_state(M, v, k, R, c, kt, a.at, a.hv, a.pvv);
+  return a;
+}
+__device__ __forceinline__ double vertex_hvT(const VertexAdj &a) {
+  return -a.pvvb * a.pvv / a.hv / a.at;
+}
+// HV: write hvT too (the separate vertex launch; the merged launch's cells
+// form it themselves)
+template <bool HV>
+__device__ __forceinline__ void ad_vertex_body(const DMesh &M, const AdjParams &P, const Rec &R,
+                                               const double *__restrict__ pveb,
+                                               const double *__restrict__ tb_u, double *vT,
+                                               double *hvT, int item) {
+  uint32_t k0, k1;
+  int lane_;
+  const int v = level_item_at(item, M.nL, k0, k1, lane_);
+  if (v >= M.nV) return;
+  const uint32_t L = M.nL;
+  for (uint32_t k = k0; k < k1; k++) {
+    const VertexAdj a = vertex_pvvb(M, R, pveb, v, k);
+    double vb = a.pvvb / a.hv;
+    if (P.use_visc && !P.freeze_u)
+      for (int i = 0; i < 3; i++) {
+        const uint32_t e = v2e_id(a.x[i]);
+        vb += (-v2e_sign(a.x[i])) * P.visc_coef * ld(tb_u, e * L + k) / ld(M.dv, e);
+      }
+    st(vT, v * L + k, vb / a.at);
+    if (HV) st(hvT, v * L + k, vertex_hvT(a));
+  }
+}
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_vertex(DMesh M, AdjParams P, Rec R,
                                                   const double *__restrict__ pveb,
                                                   const double *__restrict__ tb_u, double *vT,
                                                   double *hvT) {
-  uint32_t k0, k1;
-  int lane_;
-  const int v = level_item(M.nL, k0, k1, lane_);
-  if (v >= M.nV) return;
-  const uint32_t L = M.nL, V = M.nV;
-  int x[3];
-  uint32_t c[3];
-  double kt[3];
-  for (int i = 0; i < 3; i++) {
-    x[i] = ld(M.v2e, i * V + v);
-    c[i] = ld(M.v2c, i * V + v);
-    kt[i] = ld(M.kite, i * V + v);
-  }
-  const double at = ld(M.areaTri, v);
-  for (uint32_t k = k0; k < k1; k++) {
-    double pvvb = 0.0;
-    for (int i = 0; i < 3; i++) pvvb += 0.5 * ld(pveb, v2e_id(x[i]) * L + k);
-    double hv, pvv;
-    vertex_state(M, v, k, R, c, kt, at, hv, pvv);
-    double vb = pvvb / hv;
-    if (P.use_visc && !P.freeze_u)
-      for (int i = 0; i < 3; i++) {
-        const uint32_t e = v2e_id(x[i]);
-        vb += (-v2e_sign(x[i])) * P.visc_coef * ld(tb_u, e * L + k) / ld(M.dv, e);
-      }
-    st(vT, v * L + k, vb / at);
-    st(hvT, v * L + k, -pvvb * pvv / hv / at);
-  }
+  ad_vertex_body<true>(M, P, R, pveb, tb_u, vT, hvT, xcd_item());
 }
 
 // Cell part: h and tracer adjoints of the stage input, ke and divergence
@@ -817,19 +851,33 @@ __device__ __forceinline__ double cell_lanes_sum(const double (&v)[kCellLanes / 
   if constexpr (PL == kCellLanes) return cell_lanes_sum(v[0]);
   else return (v[0] + v[1]) + (v[2] + v[3]);
 }
-template <int PL>
-__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
-    DMesh M, AdjMesh A, AdjParams P, Rec R, Rec R0, const double *__restrict__ tb_u,
-    const double *__restrict__ heb, const double *__restrict__ hvT,
-    const double *__restrict__ r1, const double *__restrict__ r2,
-    const double *__restrict__ hbf, double *keT, double *divT,
-    double *xb_h, double *xb_tr, const double *aw_h, const double *aw_tr, double *tb_h,
-    double *tb_tr, PrepNext N) {
+// The cell part's arguments (one struct: the merged launch passes them on).
+struct CellArgs {
+  const double *tb_u, *heb, *hvT, *pveb, *r1, *r2, *hbf;
+  double *keT, *divT, *xb_h, *xb_tr;
+  const double *aw_h, *aw_tr;
+  double *tb_h, *tb_tr;
+};
+// VF: the h_vertex adjoints of the cell's vertices formed here from pveb
+// (vertex_pvvb), not read from hvT
+template <int PL, bool VF>
+__device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, const AdjParams &P,
+                                             const Rec &R, const Rec &R0, const CellArgs &G,
+                                             const PrepNext &N, int item) {
   static_assert(PL == 1 || PL == kCellLanes, "one lane or kCellLanes lanes per cell and level");
   constexpr int VPL = kCellLanes / PL;  // partials per lane
+  const double *__restrict__ tb_u = G.tb_u;
+  const double *__restrict__ heb = G.heb;
+  const double *__restrict__ hvT = G.hvT;
+  const double *__restrict__ r1 = G.r1;
+  const double *__restrict__ r2 = G.r2;
+  const double *__restrict__ hbf = G.hbf;
+  double *keT = G.keT, *divT = G.divT, *xb_h = G.xb_h, *xb_tr = G.xb_tr;
+  const double *aw_h = G.aw_h, *aw_tr = G.aw_tr;
+  double *tb_h = G.tb_h, *tb_tr = G.tb_tr;
   uint32_t k0, k1;
   int lane;
-  const int c = level_item<PL>(M.nL, k0, k1, lane);
+  const int c = level_item_at<PL>(item, M.nL, k0, k1, lane);
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
   const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
@@ -876,10 +924,16 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
                         : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < kCellSlots; q++)
-        hsum += vl + q * kCellLanes < nv ? ld(hvT, vj[q] * L + k) * kv[q] : 0.0;
-      for (int i = vl + kCellSlots * kCellLanes; i < nv; i += kCellLanes)
-        hsum += ld(hvT, uint32_t(ld(A.vc, i * C + c)) * L + k) * ld(A.vk, i * C + c);
+      for (int q = 0; q < kCellSlots; q++) {
+        const double hv =
+            VF ? vertex_hvT(vertex_pvvb(M, R, G.pveb, vj[q], k)) : ld(hvT, vj[q] * L + k);
+        hsum += vl + q * kCellLanes < nv ? hv * kv[q] : 0.0;
+      }
+      for (int i = vl + kCellSlots * kCellLanes; i < nv; i += kCellLanes) {
+        const uint32_t v = ld(A.vc, i * C + c);
+        const double hv = VF ? vertex_hvT(vertex_pvvb(M, R, G.pveb, v, k)) : ld(hvT, v * L + k);
+        hsum += hv * ld(A.vk, i * C + c);
+      }
       if (P.fct) {
         for (uint32_t t = 0; t < NT; t++) {
 #pragma unroll
@@ -970,6 +1024,25 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(
       if (N.on) prep_cell(ol, NT, xh, xt, R0, N.R0n, N.w_rk, N.Q);
     }
   }
+}
+
+template <int PL>
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(DMesh M, AdjMesh A, AdjParams P, Rec R,
+                                                Rec R0, CellArgs G, PrepNext N) {
+  ad_cell_body<PL, false>(M, A, P, R, R0, G, N, xcd_item());
+}
+// The vertex part (vT only: the workgroups before vblocks) and the cell
+// part (forming hvT itself) in one launch.
+template <int PL>
+__global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell_vertex(DMesh M, AdjMesh A, AdjParams P,
+                                                       Rec R, Rec R0, CellArgs G, PrepNext N,
+                                                       double *vT, int vblocks) {
+  const int b = blockIdx.x;
+  if (b < vblocks)
+    ad_vertex_body<false>(M, P, R, G.pveb, G.tb_u, vT, nullptr, xcd_item_in(b, vblocks));
+  else
+    ad_cell_body<PL, true>(M, A, P, R, R0, G, N,
+                           xcd_item_in(b - vblocks, int(gridDim.x) - vblocks));
 }
 
 // Edge part 2: the u adjoint of the stage input completed with the paths
@@ -1148,6 +1221,8 @@ int adj_alloc(mpas_ctx *ctx, int K) {
   auto *A = new AdjState();
   const char *fp = std::getenv("MPAS_HIP_ADJ_FUSE_PREP");
   A->fuse_prep = !(fp && fp[0] == '0');
+  const char *mv = std::getenv("MPAS_HIP_ADJ_MERGE_VERTEX");
+  A->merge_vertex = !(mv && mv[0] == '0');
   const char *gp = std::getenv("MPAS_HIP_ADJ_GRAPH");
   A->graph_ok = !(gp && gp[0] == '0');
   ctx->adj = A;
@@ -1402,16 +1477,26 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src, bool prep = 
       ADJ_LAUNCH((k_ad_edge_a<kEdgeLanes, false>), int64_t(M.nE) * kEdgeLanes, M, A.T, P, R,
                  A.tb_u, A.tb_h, A.tb_tr, A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
     }
-    ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
     const PrepNext Ns = s == 0 ? N : PrepNext{};
-    if (M.nL >= kOneLaneLevels)
-      ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, A.tb_u, A.heb, A.hvT, A.r1, A.r2, A.hbf,
-                 A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr,
-                 Ns);
-    else
-      ADJ_LAUNCH(k_ad_cell<kCellLanes>, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, A.tb_u,
-                 A.heb, A.hvT, A.r1, A.r2, A.hbf, A.keT, A.divT, A.xb_h[dst], A.xb_tr[dst],
-                 A.aw_h, A.aw_tr, A.tb_h, A.tb_tr, Ns);
+    const CellArgs G{A.tb_u, A.heb,    A.hvT,      A.pveb,       A.r1,   A.r2,   A.hbf, A.keT,
+                     A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr};
+    if (M.nL >= kOneLaneLevels) {
+      ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
+      ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, G, Ns);
+    } else if (A.merge_vertex) {
+      // one launch: vertex workgroups first, then the cells'
+      const int bs = A.ad_block;
+      const int vb = xcd_grid(int64_t(M.nV) * M.nL, bs);
+      const int cb = xcd_grid(int64_t(M.nC) * kCellLanes * M.nL, bs);
+      if (int rc_ = prof_begin(ctx, "k_ad_cell_vertex", 0.0)) return rc_;
+      hipLaunchKernelGGL(k_ad_cell_vertex<kCellLanes>, dim3(vb + cb), dim3(bs), 0, ctx->stream,
+                         M, A.T, P, R, R0, G, Ns, A.vT, vb);
+      if (int rc_ = after_launch("k_ad_cell_vertex")) return rc_;
+      if (int rc_ = prof_end(ctx)) return rc_;
+    } else {
+      ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
+      ADJ_LAUNCH(k_ad_cell<kCellLanes>, int64_t(M.nC) * kCellLanes, M, A.T, P, R, R0, G, Ns);
+    }
     ADJ_LAUNCH(k_ad_edge_b, M.nE, M, P, R, A.ubA, A.keT, A.divT, A.vT, A.xb_u[dst], A.aw_u,
                A.tb_u, Ns);
   }

@@ -245,6 +245,41 @@ def test_fused_prep_keeps_the_gradient(monkeypatch, nL):
         assert bitwise_equal(a, b)
 
 
+@pytest.mark.parametrize("nL", [1, 2])
+def test_merged_vertex_part_keeps_the_gradient(monkeypatch, nL):
+    """Below 8 levels the vertex part runs in the cell part's launch, whose
+    cells form their vertices' h_vertex adjoints themselves: the gradient
+    and the TLM are the bits of the two separate launches
+    (MPAS_HIP_ADJ_MERGE_VERTEX=0, read at mpas_adj_init), over segments of
+    several steps."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(4)
+    u, h, tr = sw_workload_state(m, 2, seed=9)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MPAS_HIP_ADJ_MERGE_VERTEX", flag)
+        ctx = Context(m, nLevels=nL, nTracers=2, tracer_scheme=TRACER_FCT, visc2=1e5,
+                      kappa2=1e3)
+        rep = lambda a: np.repeat(a.reshape(a.shape[0], 1, -1), nL, axis=1).ravel()
+        ctx.sw_upload(rep(h.reshape(-1, 1)), rep(u.reshape(-1, 1)), rep(tr.reshape(m.nCells, -1)),
+                      np.full(m.nCells, 4500.0))
+        ctx.adj_init(3)
+        x0 = ctx.state_arrays()
+        x0[0].upload(rep(u.reshape(-1, 1)))
+        x0[1].upload(rep(h.reshape(-1, 1)))
+        x0[2].upload(rep(tr.reshape(m.nCells, -1)))
+        y_h, y = _rand_state(ctx, np.random.default_rng(43))
+        xb = ctx.state_arrays()
+        ctx.adj_forward(600.0, 7, x0)  # segments of 3, 3 and 1 steps
+        ctx.adj_backward(y, xb)
+        ctx.adj_backward(y, ctx.state_arrays())  # a second sweep replays the graphs
+        ctx.sync()
+        out.append(_get(ctx, xb))
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        assert bitwise_equal(a, b)
+
+
 @pytest.mark.parametrize("nL", [1, 8])
 def test_paired_stencils_keep_the_gradient(monkeypatch, nL):
     """The edge part reading the slot-paired stencils, with the transposed
