@@ -1089,7 +1089,7 @@ int dalloc(AdjState &A, size_t n, double **out) {
     MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(double) * n) +
                                " bytes for the adjoint tape failed");
   A.bufs.push_back(p);
-  MPAS_HIP(hipMemset(p, 0, sizeof(double) * n));
+  MPAS_HIP(zero_now(p, sizeof(double) * n));
   *out = static_cast<double *>(p);
   return 0;
 }

@@ -987,7 +987,7 @@ int mpas_batch_create(mpas_ctx *const *meshes, int nmeshes, int nmembers, const 
     MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(B->arena_bytes) +
                                " bytes for the ensemble failed");
   }
-  if (hipMemset(B->arena, 0, B->arena_bytes) != hipSuccess ||
+  if (zero_now(B->arena, B->arena_bytes) != hipSuccess ||
       hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&B->d_grp, sizeof(BatchGroup) * B->groups.size()) != hipSuccess) {
     release_batch(B.get());

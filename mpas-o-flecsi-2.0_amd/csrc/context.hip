@@ -145,7 +145,7 @@ int alloc_field(mpas_ctx *ctx, size_t n, double **out) {
     MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(sizeof(double) * n) +
                                " bytes for the state failed");
   ctx->mesh_bufs.push_back(p);
-  MPAS_HIP(hipMemset(p, 0, sizeof(double) * n));
+  MPAS_HIP(zero_now(p, sizeof(double) * n));
   *out = static_cast<double *>(p);
   return 0;
 }

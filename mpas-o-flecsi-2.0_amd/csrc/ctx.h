@@ -118,4 +118,13 @@ int prof_end(mpas_ctx *ctx, hipStream_t st = nullptr);
 // Frees the tangent-linear / adjoint tape of a context (no-op without one).
 void adj_free(mpas_ctx *ctx);
 
+// Zeroes freshly allocated device memory and waits for it: the context's,
+// the ensemble's and the replay streams are non-blocking, so nothing else
+// would order a null-stream fill before their kernels (a fill still queued
+// when a kernel on another stream writes the buffer would clobber it).
+inline hipError_t zero_now(void *p, size_t bytes) {
+  const hipError_t e = hipMemsetAsync(p, 0, bytes, nullptr);
+  return e != hipSuccess ? e : hipStreamSynchronize(nullptr);
+}
+
 }  // namespace mpasx
