@@ -397,8 +397,9 @@ int mpas_adj_frozen_forward(mpas_ctx *ctx, int nsteps, const double *u0,
    mpas_adj_autotune times gradients (forward with checkpoints + sweep) of
    nsteps steps from the resident state for each candidate (schedule, then
    the adjoint kernels' size) and TLM runs for the tangent-linear kernels'
-   size, keeps the fastest, restores the resident state and writes the
-   trials as JSON into report; the tape is left empty. */
+   size, each the best of two runs after a warm-up run, keeps the fastest,
+   restores the resident state and writes the trials as JSON into report;
+   the tape is left empty. */
 int mpas_adj_config(mpas_ctx *ctx, int ad_block, int tl_block, int overlap, int graph);
 int mpas_adj_get_config(const mpas_ctx *ctx, int *ad_block, int *tl_block, int *overlap,
                         int *graph);
