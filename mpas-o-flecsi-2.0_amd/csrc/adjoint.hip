@@ -188,11 +188,15 @@ __device__ __forceinline__ int level_item(uint32_t L, uint32_t &k0, uint32_t &k1
 // Slope of the rescaled antidiffusive flux in the flux x with the limiter
 // factors frozen: max(0,x)*min(out1,in2) + min(0,x)*min(in1,out2)
 // (ocean/solve.cc:270-282).
+__device__ __forceinline__ double slope_of(double in1, double out1, double in2, double out2,
+                                           double x) {
+  return x > 0.0 ? smin(out1, in2) : (x < 0.0 ? smin(in1, out2) : 0.0);
+}
 __device__ __forceinline__ double frozen_slope(const double *__restrict__ rfac, uint32_t i1,
                                                uint32_t i2, double x) {
   const double in1 = ld(rfac, 2 * i1), out1 = ld(rfac, 2 * i1 + 1);
   const double in2 = ld(rfac, 2 * i2), out2 = ld(rfac, 2 * i2 + 1);
-  return x > 0.0 ? smin(out1, in2) : (x < 0.0 ? smin(in1, out2) : 0.0);
+  return slope_of(in1, out1, in2, out2, x);
 }
 
 // h_vertex and pv_vertex of the recorded state, formed as the forward forms
@@ -716,29 +720,9 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
             cf[m][j] = ld(M.advCoef, i * E + e) + ld(M.advCoef3, i * E + e) * M.coef3rd * sgn;
           }
       }
-      for (uint32_t t = 0; t < NT; t++) {
-        const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
-        const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
-        const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
-        const double lob = (1.0 - lam) * phib, hob = lam * phib;
-        const double hf = hob * F;
-        double hsv[VPL];
-#pragma unroll
-        for (int m = 0; m < VPL; m++) {
-          double hsx = 0.0;
-#pragma unroll
-          for (int j = 0; j < NS; j++) {
-            const int i = edge_slot<FAST>(lane + m * PL, j);
-            if (i < na) {
-              hsx += ld(R.tr, ib[m][j] * NT + t) * cf[m][j];
-              st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf * cf[m][j]);
-            }
-          }
-          hsv[m] = hsx;
-        }
-        const double hs = edge_lanes_sum<PL>(hsv);
-        if (lane) continue;
-        const double T1 = ld(R.tr, i1), T2 = ld(R.tr, i2);
+      // the finishing lane's part of tracer t: the upwind and mixing paths
+      auto finish = [&](uint32_t t, double phib, double lob, double hob, double hs, double T1,
+                        double T2) {
         Fb += lob * dv * ((F > 0.0 ? T1 : 0.0) + (F < 0.0 ? T2 : 0.0)) + hob * hs;
         double a1 = lob * dv * Fp, a2 = lob * dv * Fm;
         if (P.use_hmix) {  // mixing flux h_edge (T2 - T1) rho enters with -sign/area
@@ -750,6 +734,83 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
         }
         st(r1, (e * NT + t) * L + k, a1);
         st(r2, (e * NT + t) * L + k, a2);
+      };
+      if (PL == 1 && NT % 2 == 0) {
+        // levels on lanes: tracers two at a time, each pair's tracers,
+        // limiter factors, tendency adjoints and antidiffusive fluxes
+        // adjacent in their records and read with 16-byte accesses; each
+        // tracer's arithmetic and order as one at a time (with two lanes
+        // per edge the second tracer's registers would cost a wave per
+        // SIMD)
+        for (uint32_t t = 0; t < NT; t += 2) {
+          const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
+          const mpas_d2v b1 = ld_vec<false, mpas_d2v>(tb_tr, i1 / 2);
+          const mpas_d2v b2 = ld_vec<false, mpas_d2v>(tb_tr, i2 / 2);
+          const mpas_d2v f1a = ld_vec<false, mpas_d2v>(R.rfac, i1);
+          const mpas_d2v f1b = ld_vec<false, mpas_d2v>(R.rfac, i1 + 1);
+          const mpas_d2v f2a = ld_vec<false, mpas_d2v>(R.rfac, i2);
+          const mpas_d2v f2b = ld_vec<false, mpas_d2v>(R.rfac, i2 + 1);
+          const mpas_d2v an = ld_vec<false, mpas_d2v>(R.anti, (o * NT + t) / 2);
+          const double phib[2] = {-b1.x / A1 + b2.x / A2, -b1.y / A1 + b2.y / A2};
+          const double lam[2] = {slope_of(f1a.x, f1a.y, f2a.x, f2a.y, an.x),
+                                 slope_of(f1b.x, f1b.y, f2b.x, f2b.y, an.y)};
+          double lob[2], hob[2], hf[2];
+#pragma unroll
+          for (int q = 0; q < 2; q++) {
+            lob[q] = (1.0 - lam[q]) * phib[q];
+            hob[q] = lam[q] * phib[q];
+            hf[q] = hob[q] * F;
+          }
+          double hsv0[VPL], hsv1[VPL];
+#pragma unroll
+          for (int m = 0; m < VPL; m++) {
+            double h0 = 0.0, h1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+              const int i = edge_slot<FAST>(lane + m * PL, j);
+              if (i < na) {
+                const mpas_d2v tv = ld_vec<false, mpas_d2v>(R.tr, (ib[m][j] * NT + t) / 2);
+                h0 += tv.x * cf[m][j];
+                h1 += tv.y * cf[m][j];
+                st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf[0] * cf[m][j]);
+                st(hbf, ((uint32_t(i) * E + e) * NT + t + 1) * L + k, hf[1] * cf[m][j]);
+              }
+            }
+            hsv0[m] = h0;
+            hsv1[m] = h1;
+          }
+          const double hs0 = edge_lanes_sum<PL>(hsv0), hs1 = edge_lanes_sum<PL>(hsv1);
+          if (lane) continue;
+          const mpas_d2v T1 = ld_vec<false, mpas_d2v>(R.tr, i1 / 2);
+          const mpas_d2v T2 = ld_vec<false, mpas_d2v>(R.tr, i2 / 2);
+          finish(t, phib[0], lob[0], hob[0], hs0, T1.x, T2.x);
+          finish(t + 1, phib[1], lob[1], hob[1], hs1, T1.y, T2.y);
+        }
+      } else {
+        for (uint32_t t = 0; t < NT; t++) {
+          const uint32_t i1 = (c1 * L + k) * NT + t, i2 = (c2 * L + k) * NT + t;
+          const double phib = -ld(tb_tr, i1) / A1 + ld(tb_tr, i2) / A2;
+          const double lam = frozen_slope(R.rfac, i1, i2, ld(R.anti, o * NT + t));
+          const double lob = (1.0 - lam) * phib, hob = lam * phib;
+          const double hf = hob * F;
+          double hsv[VPL];
+#pragma unroll
+          for (int m = 0; m < VPL; m++) {
+            double hsx = 0.0;
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+              const int i = edge_slot<FAST>(lane + m * PL, j);
+              if (i < na) {
+                hsx += ld(R.tr, ib[m][j] * NT + t) * cf[m][j];
+                st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf * cf[m][j]);
+              }
+            }
+            hsv[m] = hsx;
+          }
+          const double hs = edge_lanes_sum<PL>(hsv);
+          if (lane) continue;
+          finish(t, phib, lob, hob, hs, ld(R.tr, i1), ld(R.tr, i2));
+        }
       }
     }
     if (lane) continue;
