@@ -362,8 +362,9 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
     observation cells (observations from a seeded-perturbed run of the
     member).  The members of a mesh are the levels of one context, one level
     per lane; a mesh's members are split over as few equal contexts as keep
-    every tape array below the 4 GiB the kernels address (the largest holds
-    edges x levels x tracers x 16 stencil slots); forward with checkpoints and
+    every tape array (each stencil slot's part of the per-slot flux products
+    counting as one) below the 4 GiB the kernels address (the largest holds
+    edges x levels x tracers); forward with checkpoints and
     adjoint sweep of every context run back to back.  Baseline: every member
     alone (its own single-level adjoint, the context tuned likewise), all of
     them timed; the first `sample` members per mesh also compare their
@@ -388,7 +389,7 @@ def adjoint_ensemble_bench(steps=100, n_tr=2, k=10, n_obs=2000,
                     0.02 * rng.standard_normal(C * n_tr))
             members.append(((u, h, tr.ravel()), obs, pert))
         total += n
-        cap = max(1, (2 ** 32 - 1) // (8 * E * n_tr * 16))
+        cap = max(1, (2 ** 32 - 1) // (8 * E * n_tr))
         nctx = -(-n // cap)
         per_ctx = -(-n // nctx)
         for a in range(0, n, per_ctx):

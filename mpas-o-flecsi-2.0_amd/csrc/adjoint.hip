@@ -54,8 +54,8 @@ struct AdjMesh {
   // where edgesOnEdge is symmetric: nibble j, the slot of e in the row of
   // edgesOnEdge(e, j) (0 past the row)
   const uint64_t *rslot;
-  // (edge, slot) pairs whose advCellsForEdge entry is c, as the flat slot
-  // index slot * nEdges + edge
+  // (edge, slot) pairs whose advCellsForEdge entry is c, as
+  // slot << kAtSlotShift | edge
   const uint8_t *nAt;
   const int *at;
   // vertices of c with the kite area that h_vertex pairs with c (the i-th
@@ -64,7 +64,12 @@ struct AdjMesh {
   const int *vc;
   const double *vk;
   int wVc, wAt;  // padded widths of the vc and at rows
+  // elements of one stencil slot's part of hbf (edges x levels x tracers):
+  // each slot's part is addressed from its own base, so that every 32-bit
+  // offset stays within one part while the whole may pass 4 GiB
+  uint64_t hslot;
 };
+constexpr int kAtSlotShift = 24;  // edges per mesh below 2^24
 
 struct AdjState {
   int K = 0;           // steps per checkpoint segment
@@ -128,6 +133,11 @@ struct AdjState {
   // themselves (MPAS_HIP_ADJ_MERGE_VERTEX=0 when the context is made keeps
   // two launches; the same bits either way)
   int merge_vertex = 1;
+  // with a multiple of 64 levels the edge and cell parts take their entity
+  // from the wave's first lane and load its static rows as scalars
+  // (MPAS_HIP_ADJ_UNIFORM=0 when the context is made keeps vector loads;
+  // the same bits either way)
+  int uni_ok = 1;
   std::map<std::tuple<const void *, const void *, int, int>, hipGraphExec_t> sweeps;
 };
 
@@ -166,6 +176,31 @@ struct AdjParams {
 };
 
 __device__ __forceinline__ double fsign(double F) { return (F < 0.0) ? -1.0 : double(F > 0.); }
+
+// An advection transpose entry (slot << kAtSlotShift | edge): its edge, and
+// the base of its slot's part of hbf.
+__device__ __forceinline__ uint32_t at_edge(uint32_t a) {
+  return a & ((1u << kAtSlotShift) - 1u);
+}
+__device__ __forceinline__ const double *hbf_at(const AdjMesh &A, const double *hbf, uint32_t a) {
+  return hbf + uint64_t(a >> kAtSlotShift) * A.hslot;
+}
+
+// Loads of the static mesh and transpose arrays.  U: the wave holds one
+// entity (levels on lanes, a multiple of 64 levels) whose index was taken
+// from the wave's first lane, so the index is wave-uniform and the
+// constant-space load (ldc) becomes a scalar load, off the vector memory
+// path; otherwise an ordinary global load.
+template <bool U, class T>
+__device__ __forceinline__ T ldm(const T *__restrict__ p, uint32_t i) {
+  if constexpr (U) return ldc(p, i);
+  else return ld(p, i);
+}
+template <bool U, class V, class S>
+__device__ __forceinline__ V ldmv(const S *__restrict__ p, uint32_t i) {
+  if constexpr (U) return ldc_vec<V>(p, i);
+  else return ld_vec<false, V>(p, i);
+}
 
 // Work item -> (entity, level): one level per lane, an entity's levels on
 // adjacent lanes (with one level, the entity is the item).  groups > 1
@@ -547,7 +582,7 @@ __device__ __forceinline__ int edge_slot(int vl, int i) {
   if constexpr (FAST) return 2 * (vl + (i / 2) * kEdgeLanes) + (i % 2);
   else return vl + i * kEdgeLanes;
 }
-template <int PL, bool FAST>
+template <int PL, bool FAST, bool UNI = false>
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, AdjParams P, Rec R,
                                                   const double *__restrict__ tb_u,
                                                   const double *__restrict__ tb_h,
@@ -560,20 +595,22 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
   constexpr int NS = FAST ? 2 * kFastPairs : Q;  // advection slots per partial
   uint32_t k0, k1;
   int lane;
-  const int e = level_item<PL>(M.nL, k0, k1, lane);
+  static_assert(!UNI || PL == 1, "one edge per wave with levels on lanes only");
+  int e = level_item<PL>(M.nL, k0, k1, lane);
+  if constexpr (UNI) e = __builtin_amdgcn_readfirstlane(e);
   if (e >= M.nE) return;
   const uint32_t L = M.nL, E = M.nE, NT = P.nT;
-  const int2 cc = ld(M.e2c, e);
+  const int2 cc = ldm<UNI>(M.e2c, e);
   const uint32_t c1 = cc.x, c2 = cc.y;
-  const double A1 = ld(M.area, c1), A2 = ld(M.area, c2);
-  const double dv = ld(M.dv, e), dc = ld(M.dc, e);
-  const int na = P.fct ? ld(M.nAdv, e) : 0;
+  const double A1 = ldm<UNI>(M.area, c1), A2 = ldm<UNI>(M.area, c2);
+  const double dv = ldm<UNI>(M.dv, e), dc = ldm<UNI>(M.dc, e);
+  const int na = P.fct ? ldm<UNI>(M.nAdv, e) : 0;
   for (uint32_t k = k0; k < k1; k++) {
     const uint32_t o = e * L + k;
     const double ue = ld(R.u, o), hee = ld(R.he, o), pvee = ld(R.pve, o);
     double pb = 0.0, s1 = 0.0, s2 = 0.0;
     if (!P.freeze_u) {
-      const int n = ld(M.nEoE, e);
+      const int n = ldm<UNI>(M.nEoE, e);
       const double tue = ld(tb_u, o);
       double pbv[VPL], s1v[VPL], s2v[VPL];
       if constexpr (FAST) {
@@ -581,7 +618,7 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
         // e2 = edgesOnEdge(e, j) weighted by weightsOnEdge(e2, r), r the
         // slot of e in e2's row (nibble j of rslot), gathered from the
         // slot-paired weights that the edges near e stream themselves
-        const uint64_t rv = ld(A.rslot, e);
+        const uint64_t rv = ldm<UNI>(A.rslot, e);
 #pragma unroll
         for (int m = 0; m < VPL; m++) {
           const int vl = lane + m * PL;
@@ -590,8 +627,8 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
 #pragma unroll
           for (int q = 0; q < kFastPairs; q++) {
             const int p = min(vl + q * kEdgeLanes, kNarrowPairs - 1);
-            const mpas_i2v ij = ld_vec<false, mpas_i2v>(M.eoeP, p * E + e);
-            const mpas_d2v wv = ld_vec<false, mpas_d2v>(M.woeP, p * E + e);
+            const mpas_i2v ij = ldmv<UNI, mpas_i2v>(M.eoeP, p * E + e);
+            const mpas_d2v wv = ldmv<UNI, mpas_d2v>(M.woeP, p * E + e);
             oj[2 * q] = uint32_t(ij.x);
             oj[2 * q + 1] = uint32_t(ij.y);
             w[2 * q] = wv.x;
@@ -601,7 +638,7 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
           for (int i = 0; i < 2 * kFastPairs; i++) {
             const int j = edge_slot<true>(vl, i);
             const uint32_t r = uint32_t(rv >> (4 * j)) & 15u;
-            wr[i] = ld(M.woeP, ((r / 2) * E + oj[i]) * 2 + (r % 2));
+            wr[i] = ldm<UNI>(M.woeP, ((r / 2) * E + oj[i]) * 2 + (r % 2));
           }
           double pbx = 0.0, s1x = 0.0, s2x = 0.0;
 #pragma unroll
@@ -678,12 +715,12 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
       if constexpr (FAST) {
         // advCoef_3rd from its 2-bit side code: +-advCoef, or the edge's
         // explicit value numbered by the explicit slots before it
-        const uint32_t side = ld(M.advSide, e);
+        const uint32_t side = ldm<UNI>(M.advSide, e);
         const uint32_t expl = ~(side | (side >> 1)) & 0x55555555u;  // bit 2i: slot i explicit
         double x[kAdvExplicit];
 #pragma unroll
         for (int p = 0; p < kAdvExplicit / 2; p++) {
-          const mpas_d2v v = ld_vec<false, mpas_d2v>(M.advC3xP, p * E + e);
+          const mpas_d2v v = ldmv<UNI, mpas_d2v>(M.advC3xP, p * E + e);
           x[2 * p] = v.x;
           x[2 * p + 1] = v.y;
         }
@@ -693,8 +730,8 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
 #pragma unroll
           for (int q = 0; q < kFastPairs; q++) {
             const int p = min(vl + q * kEdgeLanes, kNarrowPairs - 1);
-            const mpas_i2v ij = ld_vec<false, mpas_i2v>(M.advP, p * E + e);
-            const mpas_d2v a1 = ld_vec<false, mpas_d2v>(M.advCoefP, p * E + e);
+            const mpas_i2v ij = ldmv<UNI, mpas_i2v>(M.advP, p * E + e);
+            const mpas_d2v a1 = ldmv<UNI, mpas_d2v>(M.advCoefP, p * E + e);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
               const int i = 2 * p + h;  // the (clamped) slot
@@ -772,8 +809,9 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
                 const mpas_d2v tv = ld_vec<false, mpas_d2v>(R.tr, (ib[m][j] * NT + t) / 2);
                 h0 += tv.x * cf[m][j];
                 h1 += tv.y * cf[m][j];
-                st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf[0] * cf[m][j]);
-                st(hbf, ((uint32_t(i) * E + e) * NT + t + 1) * L + k, hf[1] * cf[m][j]);
+                double *const hs = hbf + uint64_t(i) * A.hslot;
+                st(hs, (e * NT + t) * L + k, hf[0] * cf[m][j]);
+                st(hs, (e * NT + t + 1) * L + k, hf[1] * cf[m][j]);
               }
             }
             hsv0[m] = h0;
@@ -802,7 +840,7 @@ __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_edge_a(DMesh M, AdjMesh A, 
               const int i = edge_slot<FAST>(lane + m * PL, j);
               if (i < na) {
                 hsx += ld(R.tr, ib[m][j] * NT + t) * cf[m][j];
-                st(hbf, ((uint32_t(i) * E + e) * NT + t) * L + k, hf * cf[m][j]);
+                st(hbf + uint64_t(i) * A.hslot, (e * NT + t) * L + k, hf * cf[m][j]);
               }
             }
             hsv[m] = hsx;
@@ -921,7 +959,7 @@ struct CellArgs {
 };
 // VF: the h_vertex adjoints of the cell's vertices formed here from pveb
 // (vertex_pvvb), not read from hvT
-template <int PL, bool VF>
+template <int PL, bool VF, bool UNI = false>
 __device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, const AdjParams &P,
                                              const Rec &R, const Rec &R0, const CellArgs &G,
                                              const PrepNext &N, int item) {
@@ -938,12 +976,15 @@ __device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, c
   double *tb_h = G.tb_h, *tb_tr = G.tb_tr;
   uint32_t k0, k1;
   int lane;
-  const int c = level_item_at<PL>(item, M.nL, k0, k1, lane);
+  static_assert(!UNI || PL == 1, "one cell per wave with levels on lanes only");
+  int c = level_item_at<PL>(item, M.nL, k0, k1, lane);
+  if constexpr (UNI) c = __builtin_amdgcn_readfirstlane(c);  // one cell per wave
   if (c >= M.nC) return;
   const uint32_t L = M.nL, C = M.nC, NT = P.nT;
-  const int n = ld(M.nEoC, c), nv = ld(A.nVc, c), na = P.fct ? ld(A.nAt, c) : 0;
-  const uint8_t neg = ld(M.sgnNeg, c);
-  const double ar = ld(M.area, c);
+  const int n = ldm<UNI>(M.nEoC, c), nv = ldm<UNI>(A.nVc, c);
+  const int na = P.fct ? ldm<UNI>(A.nAt, c) : 0;
+  const uint8_t neg = ldm<UNI>(M.sgnNeg, c);
+  const double ar = ldm<UNI>(M.area, c);
   constexpr int Q = kChunk / kCellLanes;  // entries per partial and round of the tail
   // row slots clamped into the padded rows (uniform widths, so the id
   // loads need not wait for the row counts; the padding holds valid ids)
@@ -963,21 +1004,21 @@ __device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, c
 #pragma unroll
       for (int q = 0; q < kCellSlots; q++) {
         const int j = vl + q * kCellLanes;
-        ej[q] = ld(M.c2e, min(j, wc - 1) * C + c);
+        ej[q] = ldm<UNI>(M.c2e, min(j, wc - 1) * C + c);
         const int i = min(j, A.wVc - 1);
-        vj[q] = ld(A.vc, i * C + c);
-        kv[q] = ld(A.vk, i * C + c);
+        vj[q] = ldm<UNI>(A.vc, i * C + c);
+        kv[q] = ldm<UNI>(A.vk, i * C + c);
       }
       if (P.fct)
 #pragma unroll
         for (int q = 0; q < kAtSlots; q++)
-          aj[q] = uint32_t(ld(A.at, min(vl + q * kCellLanes, A.wAt - 1) * C + c)) * NT;
+          aj[q] = uint32_t(ldm<UNI>(A.at, min(vl + q * kCellLanes, A.wAt - 1) * C + c));
 #pragma unroll
       for (int q = 0; q < kCellSlots; q++) {
         const int j = vl + q * kCellLanes;
         const bool live = j < n;
         const uint32_t e = ej[q], oe = e * L + k;
-        if (!P.freeze_u) g1 += live ? sign_on_cell(neg, j) * ld(tb_u, oe) / ld(M.dc, e) : 0.0;
+        if (!P.freeze_u) g1 += live ? sign_on_cell(neg, j) * ld(tb_u, oe) / ldm<UNI>(M.dc, e) : 0.0;
         hsum += live ? 0.5 * ld(heb, oe) : 0.0;
         const bool first = (neg >> j) & 1;
         for (uint32_t t = 0; t < NT; t++)
@@ -999,17 +1040,17 @@ __device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, c
         for (uint32_t t = 0; t < NT; t++) {
 #pragma unroll
           for (int q = 0; q < kAtSlots; q++)
-            Tb[t] += (vl + q * kCellLanes < na) ? ld(hbf, (aj[q] + t) * L + k) : 0.0;
+            Tb[t] += (vl + q * kCellLanes < na) ? ld(hbf_at(A, hbf, aj[q]), (at_edge(aj[q]) * NT + t) * L + k) : 0.0;
         }
         for (int i0 = vl + kAtSlots * kCellLanes; i0 < na; i0 += kChunk) {
           uint32_t os[Q];
 #pragma unroll
           for (int j = 0; j < Q; j++)
-            os[j] = uint32_t(ld(A.at, min(i0 + j * kCellLanes, na - 1) * C + c)) * NT;
+            os[j] = uint32_t(ldm<UNI>(A.at, min(i0 + j * kCellLanes, na - 1) * C + c));
           for (uint32_t t = 0; t < NT; t++) {
 #pragma unroll
             for (int j = 0; j < Q; j++)
-              Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf, (os[j] + t) * L + k) : 0.0;
+              Tb[t] += (i0 + j * kCellLanes < na) ? ld(hbf_at(A, hbf, os[j]), (at_edge(os[j]) * NT + t) * L + k) : 0.0;
           }
         }
       }
@@ -1087,10 +1128,10 @@ __device__ __forceinline__ void ad_cell_body(const DMesh &M, const AdjMesh &A, c
   }
 }
 
-template <int PL>
+template <int PL, bool UNI = false>
 __global__ __launch_bounds__(kAdjMaxBlock) void k_ad_cell(DMesh M, AdjMesh A, AdjParams P, Rec R,
                                                 Rec R0, CellArgs G, PrepNext N) {
-  ad_cell_body<PL, false>(M, A, P, R, R0, G, N, xcd_item());
+  ad_cell_body<PL, false, UNI>(M, A, P, R, R0, G, N, xcd_item());
 }
 // The vertex part (vT only: the workgroups before vblocks) and the cell
 // part (forming hvT itself) in one launch.
@@ -1155,6 +1196,27 @@ int dalloc(AdjState &A, size_t n, double **out) {
   return 0;
 }
 
+// `parts` arrays of `part` doubles each in one allocation: each part stays
+// within the 4 GiB one 32-bit offset addresses (the kernels address each
+// from its own base), the whole may pass it.
+int dalloc_parts(AdjState &A, size_t part, int parts, double **out) {
+  *out = nullptr;
+  if (!part || parts < 1) return 0;
+  if (part > size_t(UINT32_MAX) / sizeof(double))
+    MPAS_FAIL(MPAS_ESHAPE, "a part of an adjoint tape array of " +
+                               std::to_string(sizeof(double) * part) +
+                               " bytes exceeds the 4 GiB the kernels address");
+  const size_t bytes = sizeof(double) * part * size_t(parts);
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess)
+    MPAS_FAIL(MPAS_ENOMEM, "hipMalloc of " + std::to_string(bytes) +
+                               " bytes for the adjoint tape failed");
+  A.bufs.push_back(p);
+  MPAS_HIP(zero_now(p, bytes));
+  *out = static_cast<double *>(p);
+  return 0;
+}
+
 template <class T>
 int upload_vec(AdjState &A, const std::vector<T> &v, const T **out) {
   *out = nullptr;
@@ -1213,11 +1275,14 @@ int build_transposes(mpas_ctx *ctx, AdjState &A) {
       eT.id[e].push_back(e2);
       eT.a[e].push_back(m.weightsOnEdge[size_t(e2) * 16 + j]);
     }
+  if (m.fct && E >= (1 << kAtSlotShift))
+    MPAS_FAIL(MPAS_ESHAPE, "the adjoint's advection transpose holds edge ids below 2^24 (" +
+                               std::to_string(E) + " edges)");
   if (m.fct)
     for (int e = 0; e < E; e++)
       for (int i = 0; i < m.nAdvCellsForEdge[e]; i++) {
         const int c = m.advCellsForEdge[size_t(e) * 16 + i];
-        aT.id[c].push_back(i * E + e);
+        aT.id[c].push_back(i << kAtSlotShift | e);
         aT.a[c].push_back(0.0);
       }
   for (int v = 0; v < V; v++)
@@ -1282,6 +1347,8 @@ int adj_alloc(mpas_ctx *ctx, int K) {
   auto *A = new AdjState();
   const char *fp = std::getenv("MPAS_HIP_ADJ_FUSE_PREP");
   A->fuse_prep = !(fp && fp[0] == '0');
+  const char *un = std::getenv("MPAS_HIP_ADJ_UNIFORM");
+  A->uni_ok = !(un && un[0] == '0');
   const char *mv = std::getenv("MPAS_HIP_ADJ_MERGE_VERTEX");
   A->merge_vertex = !(mv && mv[0] == '0');
   const char *gp = std::getenv("MPAS_HIP_ADJ_GRAPH");
@@ -1311,7 +1378,12 @@ int adj_alloc(mpas_ctx *ctx, int K) {
     if ((rc = dalloc(*A, T, b))) return rc;
   for (auto b : et_bufs)
     if ((rc = dalloc(*A, ET, b))) return rc;
-  if (M.nT && (rc = dalloc(*A, ET * std::max(1, M.wAdv), &A->hbf))) return rc;
+  // hbf: one part of E x L x T per stencil slot (the kernels address each
+  // part from its own base)
+  if (M.nT) {
+    if ((rc = dalloc_parts(*A, ET, std::max(1, M.wAdv), &A->hbf))) return rc;
+    A->T.hslot = ET;
+  }
   for (auto b : v_bufs)
     if ((rc = dalloc(*A, V, b))) return rc;
   for (int s = 0; s < 5; s++)
@@ -1524,8 +1596,13 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src, bool prep = 
     const Rec &R = A.rec[4 * i + s];
     // levels on lanes (kOneLaneLevels or more): one lane per entity and
     // level, each holding all of the sums' partials
+    // levels on lanes with a multiple of 64 levels: one entity per wave
+    const bool uni = M.nL >= kOneLaneLevels && M.nL % 64 == 0 && A.uni_ok;
     if (A.edge_fast) {
-      if (M.nL >= kOneLaneLevels)
+      if (uni)
+        ADJ_LAUNCH((k_ad_edge_a<1, true, true>), M.nE, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr,
+                   A.ubA, A.heb, A.pveb, A.r1, A.r2, A.hbf);
+      else if (M.nL >= kOneLaneLevels)
         ADJ_LAUNCH((k_ad_edge_a<1, true>), M.nE, M, A.T, P, R, A.tb_u, A.tb_h, A.tb_tr, A.ubA,
                    A.heb, A.pveb, A.r1, A.r2, A.hbf);
       else
@@ -1543,7 +1620,10 @@ int adj_step(mpas_ctx *ctx, AdjState &A, int i, double dt, int src, bool prep = 
                      A.divT, A.xb_h[dst], A.xb_tr[dst], A.aw_h, A.aw_tr, A.tb_h, A.tb_tr};
     if (M.nL >= kOneLaneLevels) {
       ADJ_LAUNCH(k_ad_vertex, M.nV, M, P, R, A.pveb, A.tb_u, A.vT, A.hvT);
-      ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, G, Ns);
+      if (uni)
+        ADJ_LAUNCH((k_ad_cell<1, true>), M.nC, M, A.T, P, R, R0, G, Ns);
+      else
+        ADJ_LAUNCH(k_ad_cell<1>, M.nC, M, A.T, P, R, R0, G, Ns);
     } else if (A.merge_vertex) {
       // one launch: vertex workgroups first, then the cells'
       const int bs = A.ad_block;

@@ -186,6 +186,12 @@ __device__ __forceinline__ T ldc(const T *__restrict__ p, uint32_t i) {
   const MPAS_CONST char *b = (const MPAS_CONST char *)(p);
   return *(const MPAS_CONST T *)(b + i * uint32_t(sizeof(T)));
 }
+// Record i of V-sized records (a native 2-vector of S) the same way.
+template <class V, class S>
+__device__ __forceinline__ V ldc_vec(const S *__restrict__ p, uint32_t i) {
+  const MPAS_CONST char *b = (const MPAS_CONST char *)(p);
+  return *(const MPAS_CONST V *)(b + i * uint32_t(sizeof(V)));
+}
 // Streams read or written once per launch (the slot-major stencils, a
 // lane's own state entries): with NT true, non-temporal accesses, which
 // spare the caches for the neighbour gathers.  The switches select which

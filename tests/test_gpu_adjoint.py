@@ -323,6 +323,36 @@ def test_paired_stencils_keep_the_gradient(monkeypatch, nL):
         assert np.max(np.abs(a - b)) <= 1e-12 * scale
 
 
+def test_uniform_entity_path_keeps_the_gradient(monkeypatch):
+    """A 64-level context's edge and cell parts take their entity from the
+    wave's first lane (the default; MPAS_HIP_ADJ_UNIFORM=0, read at
+    mpas_adj_init, keeps per-lane indices): the same gradient bits."""
+    from mpas_hip import TRACER_FCT, Context, sw_workload_state
+    m = mesh(3)
+    nM, nT = 64, 2
+    members = [sw_workload_state(m, nT, seed=60 + g, u0=10.0 + 0.2 * g) for g in range(nM)]
+    U, H, T = _pack_members(m, [(u, h, tr.ravel()) for u, h, tr in members], nT)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MPAS_HIP_ADJ_UNIFORM", flag)
+        ctx = Context(m, nLevels=nM, nTracers=nT, tracer_scheme=TRACER_FCT, visc2=1e5,
+                      kappa2=1e3)
+        ctx.sw_upload(H, U, T, np.full(m.nCells, 4500.0))
+        ctx.adj_init(3)
+        x0 = ctx.state_arrays()
+        for a, v in zip(x0, (U, H, T)):
+            a.upload(v)
+        y_h, y = _rand_state(ctx, np.random.default_rng(47))
+        xb = ctx.state_arrays()
+        ctx.adj_forward(600.0, 7, x0)  # segments of 3, 3 and 1 steps
+        ctx.adj_backward(y, xb)
+        ctx.sync()
+        out.append(_get(ctx, xb))
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        assert bitwise_equal(a, b)
+
+
 def test_adjoint_error_paths():
     from mpas_hip import TRACER_CENTRED, Context
     from mpas_hip._lib import MpasError
@@ -352,7 +382,7 @@ def _pack_members(m, members, nT):
     return U, H, T
 
 
-@pytest.mark.parametrize("nM", [6, 12])
+@pytest.mark.parametrize("nM", [6, 12, 64])
 def test_members_as_levels_equal_single_runs(nM):
     """Levels of one context are independent, so a context whose levels are
     ensemble members (the batched adjoint) gives every member the final
@@ -360,7 +390,8 @@ def test_members_as_levels_equal_single_runs(nM):
     levels run one per lane (adjacent lanes, adjacent members).  With 12
     members the adjoint's edge and cell parts take one lane per entity and
     member, which adds the partial sums the single run spreads over 2 or 4
-    lanes in the same tree."""
+    lanes in the same tree; with 64 each wave holds one entity, whose static
+    rows those parts load as scalars."""
     from mpas_hip import TRACER_FCT, Context, sw_workload_state
     m = mesh(3)
     nT, K, nsteps, dt = 2, 3, 7, 600.0
