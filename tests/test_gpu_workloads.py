@@ -292,8 +292,9 @@ def test_batched_adjoint_ensemble_128_members():
                     0.02 * rng.standard_normal(C * nT))
             members.append(((u, h, tr.ravel()), obs, pert))
         # a mesh's members as the levels of as few equal contexts as keep
-        # every tape array below 4 GiB
-        cap = max(1, (2 ** 32 - 1) // (8 * E * nT * 16))
+        # every tape array (each stencil slot's part of the flux products
+        # counting as one) below 4 GiB: one context of 64 per mesh
+        cap = max(1, (2 ** 32 - 1) // (8 * E * nT))
         nctx = -(-n // cap)
         per_ctx = -(-n // nctx)
         _say(f"batched adjoint: level {L}, {n} members in {nctx} contexts of {per_ctx}")
